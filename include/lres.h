@@ -1,0 +1,130 @@
+/* C-ABI of the MI355X LRes window step (liblres_hip.so).
+ *
+ * Plain pointers and sizes only.  Every entry point returns 0 on success and a
+ * non-zero status otherwise; lres_last_error() then gives the message (the
+ * Python host layer raises RuntimeError with it, the exception type the
+ * reference's pybind11 module raises for LAPACK failures, src/lrft_cpp.cpp:94,110).
+ * Nothing throws across this boundary.
+ *
+ * Pointer conventions: arguments named h_* are host memory, d_* device memory
+ * (caller-owned, e.g. torch tensors).  All device work is ordered on the
+ * stream passed to lres_plan_create (0 = the null stream); entry points that
+ * return host scalars synchronise that stream.
+ *
+ * Reference interface replaced: the pybind11 module lrft._lrft_cpp
+ * (src/lrft_cpp.cpp:399-404), function lrft_windows (src/lrft_cpp.cpp:129-233),
+ * and the per-geometry set-up that LRFTLCAO/LRFTLCAOCpp do in Python
+ * (lrft/lcao3d.py:132-161, 343-357).  The eigen route of the reference becomes
+ * the pole route: D_B = 2 Re sum_j w_j (z_j S_W - H_W)^-1 [T,T] per window. */
+#ifndef LRES_H
+#define LRES_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lres_plan lres_plan;
+
+/* Precision of the pole solves. */
+enum { LRES_FP64 = 0, LRES_MIXED = 1 };
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* lres_last_error(void);
+
+/* Number of HIP devices visible (0 when no GPU); never fails. */
+int lres_device_count(void);
+
+/* Window and ownership tables of a geometry (lrft/lcao3d.py:132-161).
+ *   cells (ncell x 3), atom_pos (natom x 3), atom_of_fn (nb), box (3), grid shape (3).
+ * Atoms go to their nearest cell centre (minimum image, lowest index on ties);
+ * grid point (i,j,k) at (i,j,k)*dx goes to the centre with the smallest squared
+ * minimum-image distance, lowest index on ties; W(B) lists, ascending, the
+ * functions whose cell lies within R_w of B.  Outputs: h_Woff (ncell+1),
+ * h_Ooff (ncell+1), h_Oflat (ng) always; h_Wflat may be NULL on a first call,
+ * which then only fills h_Woff so that the caller can size h_Wflat. */
+int lres_build_windows(const double* h_cells, int64_t ncell, const double* h_atom_pos, int64_t natom,
+                       const int64_t* h_atom_of_fn, int64_t nb, const double* h_box, const int64_t* h_shape,
+                       double dx, double R_w, int64_t* h_Wflat, int64_t* h_Woff, int64_t* h_Oflat,
+                       int64_t* h_Ooff);
+
+/* Per-geometry plan (replaces LRFTLCAOCpp.__init__'s flattening, lcao3d.py:343-357,
+ * plus everything lrft_windows recomputes per call that only depends on the
+ * geometry: T_B, Q_B, the owned rows of X, the dense S_W).  CSR arrays are
+ * int64 indptr/indices + float64 values as produced by _csr64 (lcao3d.py:297-300).
+ * M: number of circle poles (even).  precision: LRES_FP64 or LRES_MIXED.
+ * stream: hipStream_t (NULL = null stream). */
+int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, int precision,
+                     const int64_t* h_Wflat, const int64_t* h_Woff, int64_t nwin,
+                     const int64_t* h_Oflat, const int64_t* h_Ooff,
+                     const int64_t* h_Sp, const int64_t* h_Si, const double* h_Sv,
+                     const int64_t* h_Xp, const int64_t* h_Xi, const double* h_Xv, void* stream);
+
+int lres_plan_destroy(lres_plan* plan);
+
+/* Window sizes n_W and |T_B| (functions with support on own(B)); h_n, h_k of length nwin. */
+int lres_plan_sizes(const lres_plan* plan, int32_t* h_n, int32_t* h_k);
+
+/* Set precision (LRES_FP64 / LRES_MIXED) for later passes. */
+int lres_plan_set_precision(lres_plan* plan, int precision);
+
+/* Load H(v) as a host CSR (the Hc argument triple of lrft_windows,
+ * src/lrft_cpp.cpp:129) and gather the dense window blocks H_W on the device. */
+int lres_set_hamiltonian_csr(lres_plan* plan, const int64_t* h_Hp, const int64_t* h_Hi, const double* h_Hv,
+                             int64_t nnz);
+
+/* Same, with the CSR already in device memory (d_Hp has nb+1 entries). */
+int lres_set_hamiltonian_csr_device(lres_plan* plan, const int64_t* d_Hp, const int64_t* d_Hi,
+                                    const double* d_Hv);
+
+/* One window pass at (mu, e_lo): the pole form of lrft_windows
+ * (src/lrft_cpp.cpp:129-233).  Per window B it forms
+ *   D_B = 2 Re sum_{j<M/2} w_j G_j[T,T],  F_B = 2 Re sum_j w_j z_j G_j[T,T],
+ *   G_j = (z_j S_W - H_W)^-1, z_j = c + R e^{i pi (2j+1)/M}, w_j = R e^{i pi (2j+1)/M} / M,
+ * and writes N_B = Tr(D_B Q_B) (electrons of own(B), no spin factor),
+ * E_B = Tr(F_B Q_B) and, if d_dens != NULL, d_dens[g] = x_g^T D_B x_g for every
+ * owned grid point g (no spin factor, as lrft_windows' dens).
+ * d_NB, d_EB: device arrays of nwin doubles (may be NULL).  h_sums (host, may
+ * be NULL) receives {sum_B N_B, sum_B E_B} in a fixed order (bitwise reproducible). */
+int lres_window_pass(lres_plan* plan, double mu, double e_lo, double* d_dens, double* d_NB, double* d_EB,
+                     double* h_sums);
+
+/* Inertia counts: for each shift sigma_s (s < nshift, host array), the number of
+ * window eigenvalues (H_W, S_W) below sigma_s summed over windows (h_total[s])
+ * and per window (d_counts, nwin x nshift int32, may be NULL).  Sylvester's law
+ * on an unpivoted LDL^T of sigma S_W - H_W in FP64. */
+int lres_inertia(lres_plan* plan, const double* h_sigma, int nshift, int64_t* h_total, int32_t* d_counts);
+
+/* Lowest window eigenvalue over all windows (min_B lambda_min(B), what the cold
+ * path of lcao3d.py:450-451 takes from full spectra) by inertia bisection to
+ * absolute tolerance tol. */
+int lres_lambda_min(lres_plan* plan, double tol, double* h_lambda_min);
+
+/* Chemical potential: root of 2 sum_B N_B(mu) - N by Brent's method on device
+ * passes, bracket [lo, hi] with count(lo) <= 0 <= count(hi); f_lo / f_hi are the
+ * counts already known at lo / hi (NaN: evaluated here); xtol as brentq's.
+ * Replaces _solve_mu (lcao3d.py:387-398), which the eigen route evaluates in
+ * closed form.  h_info (may be NULL) receives {passes, final count}. */
+int lres_mu_solve(lres_plan* plan, double N, double e_lo, double lo, double hi, double f_lo, double f_hi,
+                  double xtol, double* h_mu, double* h_info);
+
+/* One-shot form with lrft_windows' argument list (src/lrft_cpp.cpp:129-132):
+ * host CSR H, S, X, window tables, nb, dV, mu, e_lo, M; outputs h_dens (ng, may be
+ * NULL), h_NB, h_EB (nwin).  Builds and destroys a plan on the null stream. */
+int lres_windows(const int64_t* h_Hp, const int64_t* h_Hi, const double* h_Hv,
+                 const int64_t* h_Sp, const int64_t* h_Si, const double* h_Sv,
+                 const int64_t* h_Xp, const int64_t* h_Xi, const double* h_Xv,
+                 const int64_t* h_Wflat, const int64_t* h_Woff, int64_t nwin,
+                 const int64_t* h_Oflat, const int64_t* h_Ooff, int64_t nb, int64_t ng, double dV,
+                 double mu, double e_lo, int M, double* h_dens, double* h_NB, double* h_EB);
+
+/* Timing of the last lres_window_pass, in milliseconds, measured with HIP events
+ * on the plan's stream: {gather (set_hamiltonian), pole solves, reduction}. */
+int lres_last_timing(const lres_plan* plan, double* h_ms3);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LRES_H */

@@ -1,0 +1,32 @@
+// Internal declarations shared by the kernel and host translation units of liblres_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lres {
+
+// One window of a plan (all offsets in elements).
+struct WinDesc {
+    int64_t mat_off;   // n x n dense blocks in SW / HW
+    int64_t kk_off;    // k x k blocks in Q / D / F (times the pole-group count in Dpart / Fpart)
+    int64_t own_off;   // first owned point in the owned-row CSR
+    int32_t n, k, m, nown;
+};
+
+}  // namespace lres
+
+hipError_t launch_gather_csr(const lres::WinDesc* wins, int nwin, const int64_t* Wflat, const int64_t* Woff,
+                             const int32_t* pos, const int64_t* Hp, const int64_t* Hi, const double* Hv,
+                             double* out, hipStream_t st);
+hipError_t launch_pole_gj(int ts, int fp32, const lres::WinDesc* wins, const int32_t* ids, int nids,
+                          const double* SW, const double* HW, int M, int ngroup, double c, double Rr,
+                          double* Dpart, double* Fpart, hipStream_t st);
+hipError_t launch_reduce(const lres::WinDesc* wins, int nwin, int ngroup, const double* Dpart, const double* Fpart,
+                         const double* Q, double* D, double* F, double* NB, double* EB, hipStream_t st);
+hipError_t launch_density(const lres::WinDesc* wins, int nwin, const double* D, const int64_t* own_g,
+                          const int64_t* xptr, const int32_t* xidx, const double* xval, double* dens,
+                          hipStream_t st);
+hipError_t launch_sum2(const double* a, const double* b, int n, double* out, hipStream_t st);
+hipError_t launch_inertia(int ts, const lres::WinDesc* wins, const int32_t* ids, int nids, const double* SW,
+                          const double* HW, const double* sigma, int nshift, int32_t* counts,
+                          unsigned long long* total, hipStream_t st);

@@ -1,0 +1,453 @@
+// Device kernels of the LRes window step for CDNA4 (gfx950).
+//
+// Data layout in HBM (per plan):
+//   SW, HW   dense window blocks, float64, row-major n x n, rows/columns in the
+//            "T-last" order: the m = n - k window functions without support on
+//            own(B) first, then the k functions of T_B (both ascending).
+//   Q        per window k x k float64, Q_B = sum_{g in own(B)} x_g x_g^T dV.
+//   Dpart/Fpart  per (window, pole group) k x k float64 partial pole sums.
+//   D/F      per window k x k float64: D_B = 2 Re sum_j w_j G_j[T,T], F_B likewise with w_j z_j.
+//   owned rows of X restricted to T: CSR over owned points, int32 T-local columns.
+//
+// The pole solve of one (window, pole) is an in-place Gauss-Jordan sweep of
+// A = z S_W - H_W held in registers: a 16 x 16 thread grid, thread (ty, tx)
+// owning rows ty + 16 i and columns tx + 16 j (i, j < TS).  Pivots 0..m-1 only
+// form the Schur complement on T (the rows of G outside T are never needed);
+// pivots m..n-1 invert it, so the T x T block ends as (A^-1)[T,T].  No pivoting:
+// Im(A) = Im(z) S_W is positive definite for Im z > 0, so every leading block
+// is nonsingular.  Each step broadcasts the pivot row and column through LDS
+// (double-buffered: one barrier per step) and applies a rank-1 update to the
+// register tile.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lres_internal.h"
+
+namespace lres {
+
+template <typename R>
+struct cx {
+    R re, im;
+};
+
+template <typename R>
+__device__ __forceinline__ cx<R> cmul(cx<R> a, cx<R> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+template <typename R>
+__device__ __forceinline__ cx<R> crecip(cx<R> a) {
+    const R d = a.re * a.re + a.im * a.im;
+    const R s = R(1) / d;
+    return {a.re * s, -a.im * s};
+}
+
+// z_j = c + R e^{i theta_j}, w_j = R e^{i theta_j} / M, theta_j = pi (2j+1)/M (lrft/core.py:130-139)
+__device__ __forceinline__ void pole(int j, int M, double c, double Rr, cx<double>& z, cx<double>& w,
+                                     cx<double>& wz) {
+    double s, co;
+    sincospi((double)(2 * j + 1) / (double)M, &s, &co);
+    z = {c + Rr * co, Rr * s};
+    w = {Rr * co / M, Rr * s / M};
+    wz = cmul(w, z);
+}
+
+// ---------------------------------------------------------------------------
+// gather of dense H_W from a CSR H (src/lrft_cpp.cpp:117-127), one block per window
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_gather_csr(const WinDesc* __restrict__ wins,
+                                                    const int64_t* __restrict__ Wflat,
+                                                    const int64_t* __restrict__ Woff,
+                                                    const int32_t* __restrict__ pos_of_local,
+                                                    const int64_t* __restrict__ Hp,
+                                                    const int64_t* __restrict__ Hi,
+                                                    const double* __restrict__ Hv, double* __restrict__ out) {
+    const int w = blockIdx.x;
+    const WinDesc d = wins[w];
+    const int n = d.n;
+    const int64_t* W = Wflat + Woff[w];
+    const int32_t* pos = pos_of_local + Woff[w];
+    double* A = out + d.mat_off;
+    for (int64_t q = threadIdx.x; q < (int64_t)n * n; q += blockDim.x) A[q] = 0.0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    for (int a = wave; a < n; a += nwave) {
+        const int64_t g = W[a];
+        const int pa = pos[a];
+        for (int64_t p = Hp[g] + lane; p < Hp[g + 1]; p += 64) {
+            const int64_t col = Hi[p];
+            int lo = 0, hi = n - 1, hit = -1;
+            while (lo <= hi) {   // W is ascending
+                const int mid = (lo + hi) >> 1;
+                const int64_t v = W[mid];
+                if (v == col) { hit = mid; break; }
+                if (v < col) lo = mid + 1; else hi = mid - 1;
+            }
+            if (hit >= 0) A[(int64_t)pa * n + pos[hit]] = Hv[p];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// pole solves: register-tiled Gauss-Jordan, TS x TS complex tile per thread
+// ---------------------------------------------------------------------------
+template <int TS, typename R>
+__global__ void __launch_bounds__(256) k_pole_gj(const WinDesc* __restrict__ wins, const int32_t* __restrict__ ids,
+                                                 const double* __restrict__ SW, const double* __restrict__ HW,
+                                                 int M, int ngroup, double c, double Rr,
+                                                 double* __restrict__ Dpart, double* __restrict__ Fpart) {
+    constexpr int L = 16 * TS;
+    __shared__ cx<R> rowbuf[2][L];
+    __shared__ cx<R> colbuf[2][L];
+    const int w = ids[blockIdx.x];
+    const int g = blockIdx.y;
+    const WinDesc d = wins[w];
+    const int n = d.n, m = d.m, k = d.k;
+    const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
+    const double* S = SW + d.mat_off;
+    const double* H = HW + d.mat_off;
+
+    R dacc[TS][TS], facc[TS][TS];
+#pragma unroll
+    for (int i = 0; i < TS; ++i)
+#pragma unroll
+        for (int j = 0; j < TS; ++j) dacc[i][j] = facc[i][j] = R(0);
+
+    cx<R> A[TS][TS];
+    for (int jp = g; jp < M / 2; jp += ngroup) {
+        cx<double> z, wt, wz;
+        pole(jp, M, c, Rr, z, wt, wz);
+#pragma unroll
+        for (int i = 0; i < TS; ++i) {
+            const int r = ty + 16 * i;
+#pragma unroll
+            for (int j = 0; j < TS; ++j) {
+                const int col = tx + 16 * j;
+                if (r < n && col < n) {
+                    const double s = S[(int64_t)r * n + col], h = H[(int64_t)r * n + col];
+                    A[i][j] = {(R)(z.re * s - h), (R)(z.im * s)};
+                } else {
+                    A[i][j] = {(R)(r == col ? 1 : 0), R(0)};
+                }
+            }
+        }
+        for (int p = 0; p < n; ++p) {
+            const int buf = p & 1, pr = p & 15, pi = p >> 4;
+            if (ty == pr) {
+#pragma unroll
+                for (int i = 0; i < TS; ++i)
+                    if (i == pi) {
+#pragma unroll
+                        for (int j = 0; j < TS; ++j) rowbuf[buf][tx + 16 * j] = A[i][j];
+                    }
+            }
+            if (tx == pr) {
+#pragma unroll
+                for (int j = 0; j < TS; ++j)
+                    if (j == pi) {
+#pragma unroll
+                        for (int i = 0; i < TS; ++i) colbuf[buf][ty + 16 * i] = A[i][j];
+                    }
+            }
+            __syncthreads();
+            const cx<R> inv = crecip(rowbuf[buf][p]);
+            cx<R> rs[TS], cc[TS];
+#pragma unroll
+            for (int j = 0; j < TS; ++j) rs[j] = cmul(rowbuf[buf][tx + 16 * j], inv);
+#pragma unroll
+            for (int i = 0; i < TS; ++i) cc[i] = colbuf[buf][ty + 16 * i];
+#pragma unroll
+            for (int i = 0; i < TS; ++i)
+#pragma unroll
+                for (int j = 0; j < TS; ++j) {
+                    A[i][j].re -= cc[i].re * rs[j].re - cc[i].im * rs[j].im;
+                    A[i][j].im -= cc[i].re * rs[j].im + cc[i].im * rs[j].re;
+                }
+            if (p >= m) {   // Gauss-Jordan part: pivot row/column take their final values
+                if (ty == pr) {
+#pragma unroll
+                    for (int i = 0; i < TS; ++i)
+                        if (i == pi) {
+#pragma unroll
+                            for (int j = 0; j < TS; ++j) A[i][j] = rs[j];
+                        }
+                }
+                if (tx == pr) {
+#pragma unroll
+                    for (int j = 0; j < TS; ++j)
+                        if (j == pi) {
+#pragma unroll
+                            for (int i = 0; i < TS; ++i) {
+                                const cx<R> t = cmul(cc[i], inv);
+                                A[i][j] = {-t.re, -t.im};
+                            }
+                        }
+                }
+                if (ty == pr && tx == pr) {
+#pragma unroll
+                    for (int i = 0; i < TS; ++i)
+                        if (i == pi) {
+#pragma unroll
+                            for (int j = 0; j < TS; ++j)
+                                if (j == pi) A[i][j] = inv;
+                        }
+                }
+            }
+        }
+        __syncthreads();   // the next pole's first step rewrites buffer 0
+        const cx<R> w_r = {(R)wt.re, (R)wt.im}, wz_r = {(R)wz.re, (R)wz.im};
+#pragma unroll
+        for (int i = 0; i < TS; ++i)
+#pragma unroll
+            for (int j = 0; j < TS; ++j) {
+                dacc[i][j] += w_r.re * A[i][j].re - w_r.im * A[i][j].im;
+                facc[i][j] += wz_r.re * A[i][j].re - wz_r.im * A[i][j].im;
+            }
+    }
+    const int64_t base = (d.kk_off * ngroup) + (int64_t)g * k * k;
+#pragma unroll
+    for (int i = 0; i < TS; ++i) {
+        const int r = ty + 16 * i - m;
+#pragma unroll
+        for (int j = 0; j < TS; ++j) {
+            const int col = tx + 16 * j - m;
+            if (r >= 0 && col >= 0 && r < k && col < k) {
+                Dpart[base + (int64_t)r * k + col] = (double)dacc[i][j];
+                Fpart[base + (int64_t)r * k + col] = (double)facc[i][j];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// reduction over pole groups, N_B / E_B traces and the owned-point density
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    // fixed-shape tree: deterministic for a fixed blockDim
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t += red[q];
+    return t;   // valid in thread 0
+}
+
+__global__ void __launch_bounds__(256) k_reduce(const WinDesc* __restrict__ wins, int ngroup,
+                                                const double* __restrict__ Dpart, const double* __restrict__ Fpart,
+                                                const double* __restrict__ Q, double* __restrict__ D,
+                                                double* __restrict__ F, double* __restrict__ NB,
+                                                double* __restrict__ EB) {
+    __shared__ double red[16];
+    const int w = blockIdx.x;
+    const WinDesc d = wins[w];
+    const int64_t kk = (int64_t)d.k * d.k;
+    const double* dp = Dpart + d.kk_off * ngroup;
+    const double* fp = Fpart + d.kk_off * ngroup;
+    double sn = 0.0, se = 0.0;
+    for (int64_t q = threadIdx.x; q < kk; q += blockDim.x) {
+        double dv = 0.0, fv = 0.0;
+        for (int gq = 0; gq < ngroup; ++gq) {
+            dv += dp[gq * kk + q];
+            fv += fp[gq * kk + q];
+        }
+        dv *= 2.0;
+        fv *= 2.0;
+        D[d.kk_off + q] = dv;
+        F[d.kk_off + q] = fv;
+        const double qv = Q[d.kk_off + q];
+        sn += dv * qv;
+        se += fv * qv;
+    }
+    const double tn = block_sum(sn, red);
+    const double te = block_sum(se, red);
+    if (threadIdx.x == 0) {
+        if (NB) NB[w] = tn;
+        if (EB) EB[w] = te;
+    }
+}
+
+// n(g) = x_g^T D_B x_g on the owned points (src/lrft_cpp.cpp:183-191 in D form)
+__global__ void __launch_bounds__(256) k_density(const WinDesc* __restrict__ wins, const double* __restrict__ D,
+                                                 const int64_t* __restrict__ own_g,
+                                                 const int64_t* __restrict__ xptr,
+                                                 const int32_t* __restrict__ xidx,
+                                                 const double* __restrict__ xval, double* __restrict__ dens) {
+    const int w = blockIdx.x;
+    const WinDesc d = wins[w];
+    const double* Dw = D + d.kk_off;
+    const int k = d.k;
+    for (int r = threadIdx.x; r < d.nown; r += blockDim.x) {
+        const int64_t o = d.own_off + r;
+        const int64_t p0 = xptr[o], p1 = xptr[o + 1];
+        double s = 0.0;
+        for (int64_t a = p0; a < p1; ++a) {
+            const double xa = xval[a];
+            const double* row = Dw + (int64_t)xidx[a] * k;
+            double t = 0.0;
+            for (int64_t b = p0; b < p1; ++b) t += row[xidx[b]] * xval[b];
+            s += xa * t;
+        }
+        dens[own_g[o]] = s;
+    }
+}
+
+// sum of nwin values in a fixed order into out[0] (one block)
+__global__ void k_sum2(const double* __restrict__ a, const double* __restrict__ b, int n, double* __restrict__ out) {
+    __shared__ double red[16];
+    double sa = 0.0, sb = 0.0;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        sa += a[q];
+        sb += b[q];
+    }
+    const double ta = block_sum(sa, red);
+    const double tb = block_sum(sb, red);
+    if (threadIdx.x == 0) {
+        out[0] = ta;
+        out[1] = tb;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// inertia: unpivoted LDL^T of sigma S_W - H_W, count of positive pivots
+// = number of window eigenvalues below sigma (Sylvester)
+// ---------------------------------------------------------------------------
+template <int TS>
+__global__ void __launch_bounds__(256) k_inertia(const WinDesc* __restrict__ wins, const int32_t* __restrict__ ids,
+                                                 const double* __restrict__ SW, const double* __restrict__ HW,
+                                                 const double* __restrict__ sigma, int nshift,
+                                                 int32_t* __restrict__ counts, unsigned long long* __restrict__ total) {
+    constexpr int L = 16 * TS;
+    __shared__ double rowbuf[2][L];
+    __shared__ double colbuf[2][L];
+    const int w = ids[blockIdx.x];
+    const int s_id = blockIdx.y;
+    const WinDesc d = wins[w];
+    const int n = d.n;
+    const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
+    const double sg = sigma[s_id];
+    const double* S = SW + d.mat_off;
+    const double* H = HW + d.mat_off;
+    double A[TS][TS];
+#pragma unroll
+    for (int i = 0; i < TS; ++i) {
+        const int r = ty + 16 * i;
+#pragma unroll
+        for (int j = 0; j < TS; ++j) {
+            const int col = tx + 16 * j;
+            A[i][j] = (r < n && col < n) ? sg * S[(int64_t)r * n + col] - H[(int64_t)r * n + col]
+                                         : (r == col ? 1.0 : 0.0);
+        }
+    }
+    int npos = 0;
+    for (int p = 0; p < n; ++p) {
+        const int buf = p & 1, pr = p & 15, pi = p >> 4;
+        if (ty == pr) {
+#pragma unroll
+            for (int i = 0; i < TS; ++i)
+                if (i == pi) {
+#pragma unroll
+                    for (int j = 0; j < TS; ++j) rowbuf[buf][tx + 16 * j] = A[i][j];
+                }
+        }
+        if (tx == pr) {
+#pragma unroll
+            for (int j = 0; j < TS; ++j)
+                if (j == pi) {
+#pragma unroll
+                    for (int i = 0; i < TS; ++i) colbuf[buf][ty + 16 * i] = A[i][j];
+                }
+        }
+        __syncthreads();
+        const double piv = rowbuf[buf][p];
+        npos += piv > 0.0 ? 1 : 0;
+        const double inv = 1.0 / piv;
+        double rs[TS], cc[TS];
+#pragma unroll
+        for (int j = 0; j < TS; ++j) rs[j] = rowbuf[buf][tx + 16 * j] * inv;
+#pragma unroll
+        for (int i = 0; i < TS; ++i) cc[i] = colbuf[buf][ty + 16 * i];
+#pragma unroll
+        for (int i = 0; i < TS; ++i)
+#pragma unroll
+            for (int j = 0; j < TS; ++j) A[i][j] -= cc[i] * rs[j];
+    }
+    if (threadIdx.x == 0) {
+        if (counts) counts[(int64_t)w * nshift + s_id] = npos;
+        atomicAdd(&total[s_id], (unsigned long long)npos);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers (host side of this translation unit)
+// ---------------------------------------------------------------------------
+}  // namespace lres
+
+using namespace lres;
+
+hipError_t launch_gather_csr(const WinDesc* wins, int nwin, const int64_t* Wflat, const int64_t* Woff,
+                             const int32_t* pos, const int64_t* Hp, const int64_t* Hi, const double* Hv,
+                             double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_gather_csr, dim3(nwin), dim3(256), 0, st, wins, Wflat, Woff, pos, Hp, Hi, Hv, out);
+    return hipGetLastError();
+}
+
+template <typename R>
+static hipError_t launch_pole_ts(int ts, const WinDesc* wins, const int32_t* ids, int nids, const double* SW,
+                                 const double* HW, int M, int ngroup, double c, double Rr, double* Dpart,
+                                 double* Fpart, hipStream_t st) {
+    dim3 grid(nids, ngroup), blk(256);
+    switch (ts) {
+        case 2: hipLaunchKernelGGL((k_pole_gj<2, R>), grid, blk, 0, st, wins, ids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart); break;
+        case 3: hipLaunchKernelGGL((k_pole_gj<3, R>), grid, blk, 0, st, wins, ids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart); break;
+        case 4: hipLaunchKernelGGL((k_pole_gj<4, R>), grid, blk, 0, st, wins, ids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart); break;
+        case 5: hipLaunchKernelGGL((k_pole_gj<5, R>), grid, blk, 0, st, wins, ids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart); break;
+        case 6: hipLaunchKernelGGL((k_pole_gj<6, R>), grid, blk, 0, st, wins, ids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_pole_gj(int ts, int fp32, const WinDesc* wins, const int32_t* ids, int nids, const double* SW,
+                          const double* HW, int M, int ngroup, double c, double Rr, double* Dpart, double* Fpart,
+                          hipStream_t st) {
+    if (fp32) return launch_pole_ts<float>(ts, wins, ids, nids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart, st);
+    return launch_pole_ts<double>(ts, wins, ids, nids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart, st);
+}
+
+hipError_t launch_reduce(const WinDesc* wins, int nwin, int ngroup, const double* Dpart, const double* Fpart,
+                         const double* Q, double* D, double* F, double* NB, double* EB, hipStream_t st) {
+    hipLaunchKernelGGL(k_reduce, dim3(nwin), dim3(256), 0, st, wins, ngroup, Dpart, Fpart, Q, D, F, NB, EB);
+    return hipGetLastError();
+}
+
+hipError_t launch_density(const WinDesc* wins, int nwin, const double* D, const int64_t* own_g,
+                          const int64_t* xptr, const int32_t* xidx, const double* xval, double* dens,
+                          hipStream_t st) {
+    hipLaunchKernelGGL(k_density, dim3(nwin), dim3(256), 0, st, wins, D, own_g, xptr, xidx, xval, dens);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum2(const double* a, const double* b, int n, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_sum2, dim3(1), dim3(256), 0, st, a, b, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_inertia(int ts, const WinDesc* wins, const int32_t* ids, int nids, const double* SW,
+                          const double* HW, const double* sigma, int nshift, int32_t* counts,
+                          unsigned long long* total, hipStream_t st) {
+    dim3 grid(nids, nshift), blk(256);
+    switch (ts) {
+        case 2: hipLaunchKernelGGL((k_inertia<2>), grid, blk, 0, st, wins, ids, SW, HW, sigma, nshift, counts, total); break;
+        case 3: hipLaunchKernelGGL((k_inertia<3>), grid, blk, 0, st, wins, ids, SW, HW, sigma, nshift, counts, total); break;
+        case 4: hipLaunchKernelGGL((k_inertia<4>), grid, blk, 0, st, wins, ids, SW, HW, sigma, nshift, counts, total); break;
+        case 5: hipLaunchKernelGGL((k_inertia<5>), grid, blk, 0, st, wins, ids, SW, HW, sigma, nshift, counts, total); break;
+        case 6: hipLaunchKernelGGL((k_inertia<6>), grid, blk, 0, st, wins, ids, SW, HW, sigma, nshift, counts, total); break;
+        case 7: hipLaunchKernelGGL((k_inertia<7>), grid, blk, 0, st, wins, ids, SW, HW, sigma, nshift, counts, total); break;
+        case 8: hipLaunchKernelGGL((k_inertia<8>), grid, blk, 0, st, wins, ids, SW, HW, sigma, nshift, counts, total); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}

@@ -1,0 +1,629 @@
+// Host side of liblres_hip.so: per-geometry plan, window tables, the C-ABI
+// entry points of include/lres.h, inertia bisection and the Brent mu solve.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/lres.h"
+
+#include "lres_internal.h"
+
+using lres::WinDesc;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Fail {
+    std::string msg;
+};
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Fail{std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+void require(bool ok, const std::string& msg) {
+    if (!ok) throw Fail{msg};
+}
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count) {
+        release();
+        if (count == 0) return;
+        hip_check(hipMalloc((void**)&p, count * sizeof(T)), "hipMalloc");
+        n = count;
+    }
+    void upload(const T* h, size_t count, hipStream_t st) {
+        if (count > n) alloc(count);
+        if (count) hip_check(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, st), "upload");
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+constexpr int kMaxTileRegister = 6;   // register-resident Gauss-Jordan up to n = 96
+
+}  // namespace
+
+struct lres_plan {
+    int64_t nb = 0, ng = 0, nwin = 0;
+    double dV = 0.0;
+    int M = 0;
+    int precision = LRES_FP64;
+    int ngroup = 1;
+    hipStream_t st = nullptr;
+    std::vector<WinDesc> wins;
+    std::vector<int32_t> n_of, k_of;
+    std::vector<std::vector<int32_t>> bucket;   // window ids per register tile size (index = TS)
+    DevBuf<WinDesc> d_wins;
+    DevBuf<int64_t> d_Wflat, d_Woff;
+    DevBuf<int32_t> d_pos;                      // local window index -> T-last position
+    DevBuf<int32_t> d_bucket_ids;
+    std::vector<int64_t> bucket_off;            // offsets into d_bucket_ids per TS
+    DevBuf<double> d_SW, d_HW, d_Q, d_D, d_F, d_Dpart, d_Fpart, d_NB, d_EB, d_sums;
+    DevBuf<int64_t> d_own_g, d_xptr;
+    DevBuf<int32_t> d_xidx;
+    DevBuf<double> d_xval;
+    DevBuf<int64_t> d_Hp, d_Hi;
+    DevBuf<double> d_Hv;
+    DevBuf<double> d_sigma;
+    DevBuf<unsigned long long> d_itot;
+    bool have_H = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double timing[3] = {0, 0, 0};
+    ~lres_plan() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+namespace {
+
+int fail_from(const Fail& f) {
+    g_err = f.msg;
+    return 1;
+}
+
+template <typename F>
+int guarded(F&& fn) {
+    try {
+        g_err.clear();
+        fn();
+        return 0;
+    } catch (const Fail& f) {
+        return fail_from(f);
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+void gather_h(lres_plan* P, const int64_t* Hp, const int64_t* Hi, const double* Hv) {
+    hip_check(hipEventRecord(P->ev[0], P->st), "event");
+    hip_check(launch_gather_csr(P->d_wins.p, (int)P->nwin, P->d_Wflat.p, P->d_Woff.p, P->d_pos.p, Hp, Hi, Hv,
+                                P->d_HW.p, P->st),
+              "gather kernel");
+    hip_check(hipEventRecord(P->ev[1], P->st), "event");
+    P->have_H = true;
+}
+
+// pole solves + reduction (+ density) at (mu, e_lo); sums to host if requested
+void run_pass(lres_plan* P, double mu, double e_lo, double* d_dens, double* d_NB, double* d_EB, double* h_sums) {
+    require(P->have_H, "no Hamiltonian loaded: call lres_set_hamiltonian_csr first");
+    require(mu > e_lo, "mu must lie above e_lo");
+    const double c = 0.5 * (mu + e_lo), R = 0.5 * (mu - e_lo);
+    hip_check(hipEventRecord(P->ev[2], P->st), "event");
+    const int fp32 = P->precision == LRES_MIXED ? 1 : 0;
+    for (int ts = 1; ts <= kMaxTileRegister; ++ts) {
+        const int cnt = (int)P->bucket[ts].size();
+        if (!cnt) continue;
+        hip_check(launch_pole_gj(ts, fp32, P->d_wins.p, P->d_bucket_ids.p + P->bucket_off[ts], cnt, P->d_SW.p,
+                                 P->d_HW.p, P->M, P->ngroup, c, R, P->d_Dpart.p, P->d_Fpart.p, P->st),
+                  "pole kernel");
+    }
+    hip_check(hipEventRecord(P->ev[3], P->st), "event");
+    double* NB = d_NB ? d_NB : P->d_NB.p;
+    double* EB = d_EB ? d_EB : P->d_EB.p;
+    hip_check(launch_reduce(P->d_wins.p, (int)P->nwin, P->ngroup, P->d_Dpart.p, P->d_Fpart.p, P->d_Q.p, P->d_D.p,
+                            P->d_F.p, NB, EB, P->st),
+              "reduce kernel");
+    if (d_dens)
+        hip_check(launch_density(P->d_wins.p, (int)P->nwin, P->d_D.p, P->d_own_g.p, P->d_xptr.p, P->d_xidx.p,
+                                 P->d_xval.p, d_dens, P->st),
+                  "density kernel");
+    hip_check(launch_sum2(NB, EB, (int)P->nwin, P->d_sums.p, P->st), "sum kernel");
+    if (h_sums) {
+        hip_check(hipMemcpyAsync(h_sums, P->d_sums.p, 2 * sizeof(double), hipMemcpyDeviceToHost, P->st), "copy");
+        hip_check(hipStreamSynchronize(P->st), "sync");
+        float t0 = 0, t1 = 0;
+        if (hipEventElapsedTime(&t0, P->ev[0], P->ev[1]) == hipSuccess) P->timing[0] = t0;
+        if (hipEventElapsedTime(&t1, P->ev[2], P->ev[3]) == hipSuccess) P->timing[1] = t1;
+    }
+}
+
+void inertia(lres_plan* P, const double* sig, int ns, int64_t* h_total, int32_t* d_counts) {
+    require(P->have_H, "no Hamiltonian loaded");
+    require(ns > 0 && ns <= 1024, "nshift must be in 1..1024");
+    P->d_sigma.upload(sig, ns, P->st);
+    if ((int)P->d_itot.n < ns) P->d_itot.alloc(ns);
+    hip_check(hipMemsetAsync(P->d_itot.p, 0, ns * sizeof(unsigned long long), P->st), "memset");
+    for (int ts = 1; ts <= 8; ++ts) {
+        const int cnt = (int)P->bucket[ts].size();
+        if (!cnt) continue;
+        hip_check(launch_inertia(std::max(ts, 2), P->d_wins.p, P->d_bucket_ids.p + P->bucket_off[ts], cnt,
+                                 P->d_SW.p, P->d_HW.p, P->d_sigma.p, ns, d_counts, P->d_itot.p, P->st),
+                  "inertia kernel");
+    }
+    std::vector<unsigned long long> tot(ns);
+    hip_check(hipMemcpyAsync(tot.data(), P->d_itot.p, ns * sizeof(unsigned long long), hipMemcpyDeviceToHost, P->st),
+              "copy");
+    hip_check(hipStreamSynchronize(P->st), "sync");
+    for (int s = 0; s < ns; ++s) h_total[s] = (int64_t)tot[s];
+}
+
+// Brent's root finder (R. P. Brent, Algorithms for Minimization without
+// Derivatives, 1973, ch. 4): inverse quadratic interpolation / secant steps
+// safeguarded by bisection; tolerance 2 eps |b| + xtol/2 as in the textbook.
+template <typename Fn>
+double brent_root(Fn f, double a, double b, double fa, double fb, double xtol, int maxit, int* evals) {
+    const double eps = std::numeric_limits<double>::epsilon();
+    require(fa * fb <= 0.0, "mu bracket does not change sign");
+    if (fa == 0.0) return a;
+    if (fb == 0.0) return b;
+    double c = a, fc = fa, d = b - a, e = d;
+    for (int it = 0; it < maxit; ++it) {
+        if ((fb > 0 && fc > 0) || (fb < 0 && fc < 0)) {
+            c = a;
+            fc = fa;
+            d = e = b - a;
+        }
+        if (std::fabs(fc) < std::fabs(fb)) {
+            a = b; b = c; c = a;
+            fa = fb; fb = fc; fc = fa;
+        }
+        const double tol = 2.0 * eps * std::fabs(b) + 0.5 * xtol;
+        const double mid = 0.5 * (c - b);
+        if (std::fabs(mid) <= tol || fb == 0.0) return b;
+        if (std::fabs(e) >= tol && std::fabs(fa) > std::fabs(fb)) {
+            double p, q, r;
+            const double s = fb / fa;
+            if (a == c) {
+                p = 2.0 * mid * s;
+                q = 1.0 - s;
+            } else {
+                q = fa / fc;
+                r = fb / fc;
+                p = s * (2.0 * mid * q * (q - r) - (b - a) * (r - 1.0));
+                q = (q - 1.0) * (r - 1.0) * (s - 1.0);
+            }
+            if (p > 0) q = -q; else p = -p;
+            if (2.0 * p < std::min(3.0 * mid * q - std::fabs(tol * q), std::fabs(e * q))) {
+                e = d;
+                d = p / q;
+            } else {
+                d = mid;
+                e = d;
+            }
+        } else {
+            d = mid;
+            e = d;
+        }
+        a = b;
+        fa = fb;
+        b += (std::fabs(d) > tol) ? d : (mid > 0 ? tol : -tol);
+        fb = f(b);
+        ++*evals;
+    }
+    throw Fail{"mu solve did not converge"};
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lres_last_error(void) { return g_err.c_str(); }
+
+int lres_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int lres_build_windows(const double* cells, int64_t ncell, const double* apos, int64_t natom,
+                       const int64_t* atom_of_fn, int64_t nb, const double* box, const int64_t* shape, double dx,
+                       double R_w, int64_t* Wflat, int64_t* Woff, int64_t* Oflat, int64_t* Ooff) {
+    return guarded([&] {
+        require(ncell > 0 && natom > 0 && nb > 0, "empty geometry");
+        // minimum image exactly as d - L * round(d / L) with round half to even
+        auto mi = [&](double d, int ax) { return d - box[ax] * std::nearbyint(d / box[ax]); };
+        std::vector<int64_t> cell_of_atom(natom);
+        for (int64_t a = 0; a < natom; ++a) {
+            double best = std::numeric_limits<double>::infinity();
+            int64_t bi = 0;
+            for (int64_t B = 0; B < ncell; ++B) {
+                double s = 0.0;
+                for (int ax = 0; ax < 3; ++ax) {
+                    const double d = mi(apos[3 * a + ax] - cells[3 * B + ax], ax);
+                    s += d * d;
+                }
+                const double r = std::sqrt(s);
+                if (r < best) {
+                    best = r;
+                    bi = B;
+                }
+            }
+            cell_of_atom[a] = bi;
+        }
+        std::vector<int64_t> cell_of_fn(nb);
+        for (int64_t f = 0; f < nb; ++f) {
+            require(atom_of_fn[f] >= 0 && atom_of_fn[f] < natom, "atom_of_fn out of range");
+            cell_of_fn[f] = cell_of_atom[atom_of_fn[f]];
+        }
+        // W(B): functions whose cell centre lies within R_w of B
+        Woff[0] = 0;
+        std::vector<char> near(ncell);
+        for (int64_t B = 0; B < ncell; ++B) {
+            for (int64_t C = 0; C < ncell; ++C) {
+                double s = 0.0;
+                for (int ax = 0; ax < 3; ++ax) {
+                    const double d = mi(cells[3 * C + ax] - cells[3 * B + ax], ax);
+                    s += d * d;
+                }
+                near[C] = std::sqrt(s) <= R_w;
+            }
+            int64_t cnt = 0;
+            for (int64_t f = 0; f < nb; ++f)
+                if (near[cell_of_fn[f]]) {
+                    if (Wflat) Wflat[Woff[B] + cnt] = f;
+                    ++cnt;
+                }
+            Woff[B + 1] = Woff[B] + cnt;
+        }
+        // Voronoi owners of the grid points, ties to the lowest B (strict '<' in ascending B)
+        const int64_t n1 = shape[0], n2 = shape[1], n3 = shape[2], ng = n1 * n2 * n3;
+        std::vector<int64_t> owner(ng);
+        for (int64_t g = 0; g < ng; ++g) {
+            const int64_t i = g / (n2 * n3), j = (g / n3) % n2, k = g % n3;
+            const double r[3] = {(double)i * dx, (double)j * dx, (double)k * dx};
+            double best = std::numeric_limits<double>::infinity();
+            int64_t bi = 0;
+            for (int64_t B = 0; B < ncell; ++B) {
+                const double d0 = mi(r[0] - cells[3 * B + 0], 0);
+                const double d1 = mi(r[1] - cells[3 * B + 1], 1);
+                const double d2 = mi(r[2] - cells[3 * B + 2], 2);
+                const double s = d0 * d0 + d1 * d1 + d2 * d2;
+                if (s < best) {
+                    best = s;
+                    bi = B;
+                }
+            }
+            owner[g] = bi;
+        }
+        std::vector<int64_t> cnt(ncell + 1, 0);
+        for (int64_t g = 0; g < ng; ++g) cnt[owner[g] + 1]++;
+        Ooff[0] = 0;
+        for (int64_t B = 0; B < ncell; ++B) Ooff[B + 1] = Ooff[B] + cnt[B + 1];
+        std::vector<int64_t> fill(Ooff, Ooff + ncell);
+        for (int64_t g = 0; g < ng; ++g) Oflat[fill[owner[g]]++] = g;
+    });
+}
+
+int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, int precision,
+                     const int64_t* Wflat, const int64_t* Woff, int64_t nwin, const int64_t* Oflat,
+                     const int64_t* Ooff, const int64_t* Sp, const int64_t* Si, const double* Sv, const int64_t* Xp,
+                     const int64_t* Xi, const double* Xv, void* stream) {
+    lres_plan* P = nullptr;
+    const int rc = guarded([&] {
+        require(out != nullptr, "null plan pointer");
+        require(M >= 2 && M % 2 == 0, "M must be even");
+        require(nwin > 0 && nb > 0 && ng > 0, "empty plan");
+        require(precision == LRES_FP64 || precision == LRES_MIXED, "unknown precision");
+        int ndev = 0;
+        hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+        require(ndev > 0, "no HIP device");
+        P = new lres_plan();
+        P->nb = nb;
+        P->ng = ng;
+        P->nwin = nwin;
+        P->dV = dV;
+        P->M = M;
+        P->precision = precision;
+        P->st = (hipStream_t)stream;
+        for (auto& e : P->ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+        P->wins.resize(nwin);
+        P->n_of.resize(nwin);
+        P->k_of.resize(nwin);
+        P->bucket.assign(9, {});
+        std::vector<int32_t> pos_all(Woff[nwin]);
+        std::vector<int32_t> loc(nb, -1);
+        std::vector<double> SW_all, Q_all;
+        std::vector<int64_t> own_g, xptr(1, 0);
+        std::vector<int32_t> xidx;
+        std::vector<double> xval;
+        int64_t mat_off = 0, kk_off = 0;
+        for (int64_t w = 0; w < nwin; ++w) {
+            const int64_t* W = Wflat + Woff[w];
+            const int n = (int)(Woff[w + 1] - Woff[w]);
+            require(n > 0, "empty window");
+            for (int a = 0; a < n; ++a) {
+                require(W[a] >= 0 && W[a] < nb, "window function index out of range");
+                require(a == 0 || W[a] > W[a - 1], "window lists must be strictly ascending");
+                loc[W[a]] = a;
+            }
+            // T_B: window functions with support on own(B) (src/lrft_cpp.cpp:168-177 touches exactly these)
+            std::vector<char> inT(n, 0);
+            for (int64_t o = Ooff[w]; o < Ooff[w + 1]; ++o) {
+                const int64_t g = Oflat[o];
+                require(g >= 0 && g < ng, "owned grid index out of range");
+                for (int64_t p = Xp[g]; p < Xp[g + 1]; ++p) {
+                    const int a = loc[Xi[p]];
+                    if (a >= 0) inT[a] = 1;
+                }
+            }
+            int k = 0;
+            for (int a = 0; a < n; ++a) k += inT[a];
+            const int m = n - k;
+            int32_t* pos = pos_all.data() + Woff[w];
+            int nxt_nT = 0, nxt_T = m;
+            for (int a = 0; a < n; ++a) pos[a] = inT[a] ? nxt_T++ : nxt_nT++;
+            // dense S_W in T-last order
+            const size_t s0 = SW_all.size();
+            SW_all.resize(s0 + (size_t)n * n, 0.0);
+            for (int a = 0; a < n; ++a)
+                for (int64_t p = Sp[W[a]]; p < Sp[W[a] + 1]; ++p) {
+                    const int b = loc[Si[p]];
+                    if (b >= 0) SW_all[s0 + (size_t)pos[a] * n + pos[b]] = Sv[p];
+                }
+            // owned rows of X in T-local columns, and Q_B = sum x x^T dV
+            const size_t q0 = Q_all.size();
+            Q_all.resize(q0 + (size_t)k * k, 0.0);
+            for (int64_t o = Ooff[w]; o < Ooff[w + 1]; ++o) {
+                const int64_t g = Oflat[o];
+                const size_t e0 = xidx.size();
+                for (int64_t p = Xp[g]; p < Xp[g + 1]; ++p) {
+                    const int a = loc[Xi[p]];
+                    if (a < 0) continue;
+                    xidx.push_back(pos[a] - m);
+                    xval.push_back(Xv[p]);
+                }
+                for (size_t e = e0; e < xidx.size(); ++e)
+                    for (size_t f = e0; f < xidx.size(); ++f)
+                        Q_all[q0 + (size_t)xidx[e] * k + xidx[f]] += xval[e] * xval[f] * dV;
+                own_g.push_back(g);
+                xptr.push_back((int64_t)xidx.size());
+            }
+            WinDesc d;
+            d.mat_off = mat_off;
+            d.kk_off = kk_off;
+            d.own_off = Ooff[w] - Ooff[0];
+            d.n = n;
+            d.k = k;
+            d.m = m;
+            d.nown = (int32_t)(Ooff[w + 1] - Ooff[w]);
+            P->wins[w] = d;
+            P->n_of[w] = n;
+            P->k_of[w] = k;
+            mat_off += (int64_t)n * n;
+            kk_off += (int64_t)k * k;
+            const int ts = std::max(2, (n + 15) / 16);
+            require(ts <= kMaxTileRegister,
+                    "window of " + std::to_string(n) + " functions exceeds the register-resident solver (n <= 96)");
+            P->bucket[ts].push_back((int32_t)w);
+            for (int a = 0; a < n; ++a) loc[W[a]] = -1;
+        }
+        // pole groups: enough (window, group) blocks to fill 256 CUs a few times over
+        P->ngroup = (int)std::min<int64_t>(M / 2, std::max<int64_t>(1, (2048 + nwin - 1) / nwin));
+        std::vector<int32_t> ids;
+        P->bucket_off.assign(9, 0);
+        for (int ts = 0; ts <= 8; ++ts) {
+            P->bucket_off[ts] = (int64_t)ids.size();
+            ids.insert(ids.end(), P->bucket[ts].begin(), P->bucket[ts].end());
+        }
+        hipStream_t st = P->st;
+        P->d_wins.upload(P->wins.data(), nwin, st);
+        P->d_Wflat.upload(Wflat, Woff[nwin], st);
+        std::vector<int64_t> woff_rel(nwin + 1);
+        for (int64_t w = 0; w <= nwin; ++w) woff_rel[w] = Woff[w] - Woff[0];
+        P->d_Woff.upload(woff_rel.data(), nwin + 1, st);
+        P->d_pos.upload(pos_all.data(), pos_all.size(), st);
+        P->d_bucket_ids.upload(ids.data(), ids.size(), st);
+        P->d_SW.upload(SW_all.data(), SW_all.size(), st);
+        P->d_HW.alloc(SW_all.size());
+        P->d_Q.upload(Q_all.data(), Q_all.size(), st);
+        P->d_D.alloc(std::max<size_t>(1, Q_all.size()));
+        P->d_F.alloc(std::max<size_t>(1, Q_all.size()));
+        P->d_Dpart.alloc(std::max<size_t>(1, Q_all.size() * P->ngroup));
+        P->d_Fpart.alloc(std::max<size_t>(1, Q_all.size() * P->ngroup));
+        P->d_NB.alloc(nwin);
+        P->d_EB.alloc(nwin);
+        P->d_sums.alloc(2);
+        P->d_own_g.upload(own_g.data(), own_g.size(), st);
+        P->d_xptr.upload(xptr.data(), xptr.size(), st);
+        P->d_xidx.upload(xidx.data(), std::max<size_t>(xidx.size(), 0), st);
+        P->d_xval.upload(xval.data(), xval.size(), st);
+        hip_check(hipStreamSynchronize(st), "sync");
+        *out = P;
+    });
+    if (rc) delete P;
+    return rc;
+}
+
+int lres_plan_destroy(lres_plan* P) {
+    return guarded([&] {
+        if (!P) return;
+        (void)hipStreamSynchronize(P->st);
+        delete P;
+    });
+}
+
+int lres_plan_sizes(const lres_plan* P, int32_t* n, int32_t* k) {
+    return guarded([&] {
+        require(P != nullptr, "null plan");
+        std::copy(P->n_of.begin(), P->n_of.end(), n);
+        std::copy(P->k_of.begin(), P->k_of.end(), k);
+    });
+}
+
+int lres_plan_set_precision(lres_plan* P, int precision) {
+    return guarded([&] {
+        require(P != nullptr, "null plan");
+        require(precision == LRES_FP64 || precision == LRES_MIXED, "unknown precision");
+        P->precision = precision;
+    });
+}
+
+int lres_set_hamiltonian_csr(lres_plan* P, const int64_t* Hp, const int64_t* Hi, const double* Hv, int64_t nnz) {
+    return guarded([&] {
+        require(P != nullptr, "null plan");
+        require(Hp[P->nb] == nnz && Hp[0] == 0, "H indptr inconsistent with nnz");
+        P->d_Hp.upload(Hp, P->nb + 1, P->st);
+        P->d_Hi.upload(Hi, nnz, P->st);
+        P->d_Hv.upload(Hv, nnz, P->st);
+        gather_h(P, P->d_Hp.p, P->d_Hi.p, P->d_Hv.p);
+    });
+}
+
+int lres_set_hamiltonian_csr_device(lres_plan* P, const int64_t* Hp, const int64_t* Hi, const double* Hv) {
+    return guarded([&] {
+        require(P != nullptr, "null plan");
+        gather_h(P, Hp, Hi, Hv);
+    });
+}
+
+int lres_window_pass(lres_plan* P, double mu, double e_lo, double* d_dens, double* d_NB, double* d_EB,
+                     double* h_sums) {
+    return guarded([&] {
+        require(P != nullptr, "null plan");
+        run_pass(P, mu, e_lo, d_dens, d_NB, d_EB, h_sums);
+    });
+}
+
+int lres_inertia(lres_plan* P, const double* sig, int ns, int64_t* h_total, int32_t* d_counts) {
+    return guarded([&] {
+        require(P != nullptr, "null plan");
+        inertia(P, sig, ns, h_total, d_counts);
+    });
+}
+
+int lres_lambda_min(lres_plan* P, double tol, double* out) {
+    return guarded([&] {
+        require(P != nullptr, "null plan");
+        require(tol > 0, "tol must be positive");
+        // upper bound: the smallest Rayleigh quotient H_aa / S_aa over all windows
+        std::vector<double> hw(P->d_HW.n), sw(P->d_SW.n);
+        hip_check(hipMemcpyAsync(hw.data(), P->d_HW.p, hw.size() * 8, hipMemcpyDeviceToHost, P->st), "copy");
+        hip_check(hipMemcpyAsync(sw.data(), P->d_SW.p, sw.size() * 8, hipMemcpyDeviceToHost, P->st), "copy");
+        hip_check(hipStreamSynchronize(P->st), "sync");
+        double hi = std::numeric_limits<double>::infinity();
+        for (const auto& d : P->wins)
+            for (int a = 0; a < d.n; ++a) {
+                const size_t q = d.mat_off + (size_t)a * d.n + a;
+                hi = std::min(hi, hw[q] / sw[q]);
+            }
+        hi += 1e-12 * (1.0 + std::fabs(hi));
+        int64_t cnt = 0;
+        inertia(P, &hi, 1, &cnt, nullptr);
+        require(cnt > 0, "inertia count at the Rayleigh bound is zero (indefinite S?)");
+        double step = 1.0, lo = hi - step;
+        for (int it = 0;; ++it) {
+            inertia(P, &lo, 1, &cnt, nullptr);
+            if (cnt == 0) break;
+            require(it < 60, "no lower bound for the window spectra");
+            hi = lo;
+            step *= 2.0;
+            lo -= step;
+        }
+        // multisection with 15 interior shifts per launch
+        const int K = 15;
+        std::vector<double> sg(K);
+        std::vector<int64_t> tot(K);
+        while (hi - lo > tol) {
+            for (int s = 0; s < K; ++s) sg[s] = lo + (hi - lo) * (s + 1) / (K + 1);
+            inertia(P, sg.data(), K, tot.data(), nullptr);
+            double nlo = lo, nhi = hi;
+            for (int s = 0; s < K; ++s) {
+                if (tot[s] == 0) nlo = sg[s];
+                else { nhi = sg[s]; break; }
+            }
+            if (nlo == lo && nhi == hi) break;   // no progress at double resolution
+            lo = nlo;
+            hi = nhi;
+        }
+        *out = 0.5 * (lo + hi);
+    });
+}
+
+int lres_mu_solve(lres_plan* P, double N, double e_lo, double lo, double hi, double f_lo, double f_hi,
+                  double xtol, double* h_mu, double* h_info) {
+    return guarded([&] {
+        require(P != nullptr, "null plan");
+        require(lo < hi, "mu bracket must satisfy lo < hi");
+        int evals = 0;
+        double sums[2] = {0.0, 0.0};
+        auto f = [&](double mu) {
+            run_pass(P, mu, e_lo, nullptr, nullptr, nullptr, sums);
+            return 2.0 * sums[0] - N;
+        };
+        double flo = f_lo, fhi = f_hi;
+        if (std::isnan(flo)) { flo = f(lo); ++evals; }
+        if (std::isnan(fhi)) { fhi = f(hi); ++evals; }
+        const double mu = brent_root(f, lo, hi, flo, fhi, xtol, 200, &evals);
+        *h_mu = mu;
+        if (h_info) {
+            h_info[0] = evals;
+            h_info[1] = 2.0 * sums[0] - N;
+        }
+    });
+}
+
+int lres_windows(const int64_t* Hp, const int64_t* Hi, const double* Hv, const int64_t* Sp, const int64_t* Si,
+                 const double* Sv, const int64_t* Xp, const int64_t* Xi, const double* Xv, const int64_t* Wflat,
+                 const int64_t* Woff, int64_t nwin, const int64_t* Oflat, const int64_t* Ooff, int64_t nb, int64_t ng,
+                 double dV, double mu, double e_lo, int M, double* h_dens, double* h_NB, double* h_EB) {
+    lres_plan* P = nullptr;
+    int rc = lres_plan_create(&P, nb, ng, dV, M, LRES_FP64, Wflat, Woff, nwin, Oflat, Ooff, Sp, Si, Sv, Xp, Xi, Xv,
+                              nullptr);
+    if (rc) return rc;
+    rc = guarded([&] {
+        const int64_t nnz = Hp[nb];
+        require(lres_set_hamiltonian_csr(P, Hp, Hi, Hv, nnz) == 0, g_err);
+        DevBuf<double> dens;
+        if (h_dens) {
+            dens.alloc(ng);
+            hip_check(hipMemsetAsync(dens.p, 0, ng * sizeof(double), P->st), "memset");
+        }
+        double sums[2];
+        run_pass(P, mu, e_lo, h_dens ? dens.p : nullptr, nullptr, nullptr, sums);
+        if (h_dens) hip_check(hipMemcpy(h_dens, dens.p, ng * sizeof(double), hipMemcpyDeviceToHost), "copy");
+        if (h_NB) hip_check(hipMemcpy(h_NB, P->d_NB.p, nwin * sizeof(double), hipMemcpyDeviceToHost), "copy");
+        if (h_EB) hip_check(hipMemcpy(h_EB, P->d_EB.p, nwin * sizeof(double), hipMemcpyDeviceToHost), "copy");
+    });
+    std::string keep = g_err;
+    lres_plan_destroy(P);
+    g_err = keep;
+    return rc;
+}
+
+int lres_last_timing(const lres_plan* P, double* ms3) {
+    return guarded([&] {
+        require(P != nullptr, "null plan");
+        ms3[0] = P->timing[0];
+        ms3[1] = P->timing[1];
+        ms3[2] = P->timing[2];
+    });
+}
+
+}  // extern "C"

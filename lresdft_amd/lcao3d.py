@@ -1,0 +1,295 @@
+"""Drop-in LRes back end on MI355X: ``LRFTLCAOCpp`` with the reference's call surface.
+
+Replaces lrft.lcao3d.LRFTLCAOCpp (lrft/lcao3d.py:340-456) and the native
+lrft._lrft_cpp.lrft_windows (src/lrft_cpp.cpp:129-233).  The constructor takes
+the same arguments, ``density(v, N=None)`` returns the same triple
+``(dens[cr.shape] incl. spin factor 2, e_band, mu)``, and ``.mu`` / ``.e_lo`` /
+``.windows`` / ``.own`` / ``.nW`` / ``.passes`` / ``.M`` / ``.margin`` / ``.b``
+keep their meaning, so scf_lcao, the CLI's make_solvers and the reference's
+tests use it unchanged.
+
+What changes underneath is the window step: the eigen route (dsygvd/dsygvx per
+window) is replaced by the pole route, M/2 complex shifted solves per window
+on the GPU, whose density and band energy equal the eigen route's with full
+window spectra (Lemma 1, lrft/lcao3d.py:17-24).  Consequences for the
+orchestration of lcao3d.py:407-456:
+
+* the filter edge e_lo comes from inertia counts of sigma S_W - H_W instead of
+  window eigenvalues (cold path: lambda_min by bisection; warm path: the
+  keep-edge test e_lo + margin/3 < lambda_min < e_lo + 2 margin as two counts);
+* mu comes from Brent's method on device passes (each count needs new solves)
+  instead of brentq on the closed form, with the same keep-mu rule
+  |count| <= 1e-8 N and the same xtol;
+* ``reuse_vectors`` has no eigenvectors to reuse; a mu re-solve always ends with
+  a pass at the new mu (the reference's second-pass branch, lcao3d.py:436-446);
+* the dsygvx truncation at c + 1.5 R (lcao3d.py:383-385) does not exist: all
+  window states enter with their exact r_M weight.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import _native as nat
+
+
+class StaleMu(RuntimeError):
+    """Counterpart of lcao3d._StaleMu: no mu bracket within the search range."""
+
+
+def csr64(A):
+    """int64 CSR triple with sorted indices (lcao3d.py:297-300)."""
+    A = A.tocsr()
+    A.sort_indices()
+    return nat.i64(A.indptr), nat.i64(A.indices), nat.f64(A.data)
+
+
+def build_windows(basis, cells, R_w):
+    """Window lists W(B) and Voronoi-owned grid points own(B), natively
+    (lcao3d.py:132-161 semantics; see lres_build_windows in include/lres.h)."""
+    cr = basis.cr
+    L = nat.lib()
+    cells = nat.f64(cells).reshape(-1, 3)
+    pos = nat.f64(cr.pos).reshape(-1, 3)
+    aof = nat.i64(basis.atom_of_fn)
+    box = nat.f64(cr.cell)
+    shape = nat.i64(cr.shape)
+    ncell, ng = len(cells), int(np.prod(cr.shape))
+    Woff = np.zeros(ncell + 1, np.int64)
+    Ooff = np.zeros(ncell + 1, np.int64)
+    Oflat = np.zeros(ng, np.int64)
+    args = (nat.pf(cells), ncell, nat.pf(pos), len(pos), nat.p64(aof), len(aof), nat.pf(box), nat.p64(shape),
+            float(cr.dx), float(R_w))
+    nat.check(L.lres_build_windows(*args, None, nat.p64(Woff), nat.p64(Oflat), nat.p64(Ooff)))
+    Wflat = np.zeros(int(Woff[-1]), np.int64)
+    nat.check(L.lres_build_windows(*args, nat.p64(Wflat), nat.p64(Woff), nat.p64(Oflat), nat.p64(Ooff)))
+    return Wflat, Woff, Oflat, Ooff
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("LRFTLCAOCpp (MI355X) needs a HIP device; none is visible")
+    return torch
+
+
+class WindowPlan:
+    """Owns one lres_plan (per-geometry device data) and the device buffers of a pass."""
+
+    def __init__(self, S, X, Wflat, Woff, Oflat, Ooff, nb, ng, dV, M, precision="fp64"):
+        torch = _torch()
+        self.torch = torch
+        self.nb, self.ng, self.nwin, self.M = int(nb), int(ng), len(Woff) - 1, int(M)
+        self._keep = (S, X, Wflat, Woff, Oflat, Ooff)
+        self.stream = torch.cuda.current_stream()
+        h = nat.C.c_void_p()
+        L = nat.lib()
+        prec = nat.LRES_MIXED if precision == "mixed" else nat.LRES_FP64
+        nat.check(L.lres_plan_create(nat.C.byref(h), self.nb, self.ng, float(dV), self.M, prec,
+                                     nat.p64(Wflat), nat.p64(Woff), self.nwin, nat.p64(Oflat), nat.p64(Ooff),
+                                     nat.p64(S[0]), nat.p64(S[1]), nat.pf(S[2]),
+                                     nat.p64(X[0]), nat.p64(X[1]), nat.pf(X[2]),
+                                     nat.C.c_void_p(self.stream.cuda_stream)))
+        self.h = h
+        self.n = np.zeros(self.nwin, np.int32)
+        self.k = np.zeros(self.nwin, np.int32)
+        nat.check(L.lres_plan_sizes(h, nat.pi32(self.n), nat.pi32(self.k)))
+        self.dens = torch.zeros(self.ng, dtype=torch.float64, device="cuda")
+        self.NB = torch.zeros(self.nwin, dtype=torch.float64, device="cuda")
+        self.EB = torch.zeros(self.nwin, dtype=torch.float64, device="cuda")
+        self.passes = 0
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            try:
+                nat.lib().lres_plan_destroy(h)
+            except Exception:
+                pass
+            self.h = None
+
+    def set_precision(self, precision):
+        nat.check(nat.lib().lres_plan_set_precision(self.h, nat.LRES_MIXED if precision == "mixed"
+                                                    else nat.LRES_FP64))
+
+    def set_h(self, Hc):
+        nat.check(nat.lib().lres_set_hamiltonian_csr(self.h, nat.p64(Hc[0]), nat.p64(Hc[1]), nat.pf(Hc[2]),
+                                                     int(Hc[0][-1])))
+
+    def set_h_device(self, Hp, Hi, Hv):
+        nat.check(nat.lib().lres_set_hamiltonian_csr_device(self.h, Hp.data_ptr(), Hi.data_ptr(), Hv.data_ptr()))
+
+    def run(self, mu, e_lo, want_density=True):
+        """One pass; returns (sum_B N_B, sum_B E_B) (no spin factor)."""
+        sums = np.zeros(2)
+        self.passes += 1
+        nat.check(nat.lib().lres_window_pass(self.h, float(mu), float(e_lo),
+                                             self.dens.data_ptr() if want_density else None,
+                                             self.NB.data_ptr(), self.EB.data_ptr(), nat.pf(sums)))
+        return float(sums[0]), float(sums[1])
+
+    def inertia(self, sigmas):
+        s = nat.f64(np.atleast_1d(sigmas))
+        tot = np.zeros(len(s), np.int64)
+        nat.check(nat.lib().lres_inertia(self.h, nat.pf(s), len(s), nat.p64(tot), None))
+        return tot
+
+    def lambda_min(self, tol=1e-10):
+        out = np.zeros(1)
+        nat.check(nat.lib().lres_lambda_min(self.h, float(tol), nat.pf(out)))
+        return float(out[0])
+
+    def mu_solve(self, N, e_lo, lo, hi, f_lo=math.nan, f_hi=math.nan, xtol=1e-12):
+        mu = np.zeros(1)
+        info = np.zeros(2)
+        nat.check(nat.lib().lres_mu_solve(self.h, float(N), float(e_lo), float(lo), float(hi), float(f_lo),
+                                          float(f_hi), float(xtol), nat.pf(mu), nat.pf(info)))
+        self.passes += int(info[0])
+        return float(mu[0]), info
+
+    def timing(self):
+        t = np.zeros(3)
+        nat.check(nat.lib().lres_last_timing(self.h, nat.pf(t)))
+        return t
+
+
+class LRFTLCAOCpp:
+    """Nearsighted LRes with the window step on the GPU (pole route).
+
+    Same constructor as lrft.lcao3d.LRFTLCAOCpp (lcao3d.py:343) plus
+    ``precision`` ("fp64" or "mixed").  ``nthreads`` is accepted and ignored
+    (one GPU); ``lag_mu`` keeps the reference's meaning (lcao3d.py:423-427)."""
+
+    def __init__(self, basis, cells, R_w=12.1, M=64, margin=0.3, nthreads=1, lag_mu=False, reuse_vectors=True,
+                 precision="fp64", mu_xtol=1e-12):
+        if M % 2:
+            raise ValueError("M must be even")
+        self.b, self.M, self.margin = basis, M, margin
+        self.lag_mu, self.reuse_vectors, self.nthreads = lag_mu, reuse_vectors, nthreads
+        self.mu_xtol = mu_xtol
+        cr = basis.cr
+        self.Wflat, self.Woff, self.Oflat, self.Ooff = build_windows(basis, cells, R_w)
+        self.windows = [self.Wflat[self.Woff[i]:self.Woff[i + 1]] for i in range(len(self.Woff) - 1)]
+        self.own = [self.Oflat[self.Ooff[i]:self.Ooff[i + 1]] for i in range(len(self.Ooff) - 1)]
+        self.nW = np.diff(self.Woff)
+        self.S = csr64(basis.S)
+        self.X = csr64(basis.Xr)
+        self.plan = WindowPlan(self.S, self.X, self.Wflat, self.Woff, self.Oflat, self.Ooff, basis.nb, cr.ng,
+                               cr.dV, M, precision)
+        self.mu = None
+        self.e_lo = None
+        self.history = []
+
+    @property
+    def passes(self):
+        return self.plan.passes
+
+    def set_precision(self, precision):
+        self.plan.set_precision(precision)
+
+    # -- mu search -----------------------------------------------------------
+    def _count(self, mu, e_lo, N):
+        n, _ = self.plan.run(mu, e_lo, want_density=False)
+        return 2.0 * n - N
+
+    def _solve_mu_cold(self, lam_min, e_lo, N):
+        """Bracket as _solve_mu (lcao3d.py:390-397), then Brent on device counts."""
+        lo = max(lam_min, e_lo + 1e-6)
+        f_lo = self._count(lo, e_lo, N)
+        if f_lo > 0:
+            raise RuntimeError("electron count positive at the bottom of the spectrum")
+        hi = lo + 1.0
+        f_hi = self._count(hi, e_lo, N)
+        while f_hi < 0:
+            lo, f_lo = hi, f_hi
+            hi += 0.5
+            if hi > lam_min + 50.0:
+                raise StaleMu("no electron-count bracket within 50 Ha of the window spectra")
+            f_hi = self._count(hi, e_lo, N)
+        mu, _ = self.plan.mu_solve(N, e_lo, lo, hi, f_lo, f_hi, self.mu_xtol)
+        return mu
+
+    def _solve_mu_warm(self, mu0, f0, e_lo, N):
+        """Bracket outward from the previous mu (count f0 there), then Brent."""
+        step = 1e-3
+        direction = 1.0 if f0 < 0 else -1.0
+        a, fa = mu0, f0
+        for _ in range(40):
+            b = a + direction * step
+            if b <= e_lo:
+                b = 0.5 * (a + e_lo)
+            fb = self._count(b, e_lo, N)
+            if (fa < 0) != (fb < 0) or fb == 0:
+                lo, hi, flo, fhi = (a, b, fa, fb) if a < b else (b, a, fb, fa)
+                mu, _ = self.plan.mu_solve(N, e_lo, lo, hi, flo, fhi, self.mu_xtol)
+                return mu
+            a, fa = b, fb
+            step *= 4.0
+        raise StaleMu("no electron-count bracket around the previous mu")
+
+    # -- density (lcao3d.py:400-456) ------------------------------------------
+    def density(self, v, N=None):
+        try:
+            return self._density(v, N)
+        except StaleMu:
+            self.mu = None
+            return self._density(v, N)
+
+    def _finish(self, mu, e_lo):
+        n, e = self.plan.run(mu, e_lo, want_density=True)
+        dens = self.plan.dens.cpu().numpy()
+        return 2.0 * dens.reshape(self.b.cr.shape), 2.0 * e, n
+
+    def _density(self, v, N=None):
+        cr = self.b.cr
+        N = cr.n_elec if N is None else N
+        self.plan.set_h(csr64(self.b.H(v)))
+        return self._density_loaded(N)
+
+    def _density_loaded(self, N):
+        tol = 1e-8 * N
+        m = self.margin
+        if self.mu is not None:
+            mu, e_lo = self.mu, self.e_lo
+            c_lo, c_hi = self.plan.inertia([e_lo + m / 3, e_lo + 2 * m])
+            if c_lo == 0 and c_hi > 0:            # keep the filter edge (lcao3d.py:420)
+                dens, e_band, nsum = self._finish(mu, e_lo)
+                cnt = 2.0 * nsum - N
+                if self.lag_mu and abs(cnt) < 0.25:
+                    self.history.append("warm-lag")
+                    return dens, e_band, mu
+                if abs(cnt) > tol:
+                    mu = self._solve_mu_warm(mu, cnt, e_lo, N)
+                    dens, e_band, _ = self._finish(mu, e_lo)
+                    self.history.append("warm-resolve")
+                else:
+                    self.history.append("warm-keep")
+                self.mu = mu
+                return dens, e_band, mu
+        lam_min = self.plan.lambda_min()
+        e_lo = lam_min - m
+        mu = self._solve_mu_cold(lam_min, e_lo, N)
+        dens, e_band, _ = self._finish(mu, e_lo)
+        self.mu, self.e_lo = mu, e_lo
+        self.history.append("cold")
+        return dens, e_band, mu
+
+
+def lres_windows(Hc, S, X, Wflat, Woff, Oflat, Ooff, nb, ng, dV, mu, e_lo, M, want_density=True):
+    """One-shot pass with lrft_windows' inputs (src/lrft_cpp.cpp:129-132).
+
+    Returns (dens (ng,) or empty, N_B, E_B) per window, no spin factor, where
+    N_B = sum_k c_Bk r_M(lam_Bk) and E_B = sum_k c_Bk lam_Bk r_M(lam_Bk) in the
+    reference's notation."""
+    nwin = len(Woff) - 1
+    dens = np.zeros(ng if want_density else 0)
+    NB, EB = np.zeros(nwin), np.zeros(nwin)
+    nat.check(nat.lib().lres_windows(nat.p64(Hc[0]), nat.p64(Hc[1]), nat.pf(Hc[2]),
+                                     nat.p64(S[0]), nat.p64(S[1]), nat.pf(S[2]),
+                                     nat.p64(X[0]), nat.p64(X[1]), nat.pf(X[2]),
+                                     nat.p64(nat.i64(Wflat)), nat.p64(nat.i64(Woff)), nwin,
+                                     nat.p64(nat.i64(Oflat)), nat.p64(nat.i64(Ooff)), int(nb), int(ng), float(dV),
+                                     float(mu), float(e_lo), int(M), nat.pf(dens) if want_density else None,
+                                     nat.pf(NB), nat.pf(EB)))
+    return dens, NB, EB

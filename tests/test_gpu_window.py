@@ -1,0 +1,174 @@
+"""GPU parity of the pole-route window step against the CPU oracle.
+
+Bars (tolerances written per assertion):
+  fixed (v, mu, e_lo) pass vs the oracle pole pass and the full-spectrum eigen
+  pass: density max-abs 1e-12, N_B / E_B 1e-12 (the pole/eigen agreement of
+  results/window_resolvent.json is <= 2.6e-14);
+  full density(v), cold then warm, vs the oracle with full spectra: e_band 1e-9,
+  density max-abs 1e-10 (the C++-vs-Python bar of tests/test_three_d.py:86-100);
+  reruns bitwise identical (tests/test_three_d.py:103-111 becomes run-to-run).
+"""
+import numpy as np
+import pytest
+from scipy import linalg
+
+from oracle.lres_oracle import (BASIS, KSDense, LCAOBasis, LResEigen, build_windows, cell_centres,
+                                eig_window_pass, guess_v, h2_crystal, ks_lcao, pole_window_pass, r_M, scf_lcao)
+
+pytestmark = pytest.mark.gpu
+
+
+def crystal(n, a=6.0):
+    return h2_crystal((n, n, n), a=a, pts_per_cell=10, rc=0.6)
+
+
+@pytest.fixture(scope="module")
+def sys3():
+    cr = crystal(3)
+    b = LCAOBasis(cr, **BASIS)
+    return cr, b, guess_v(cr)
+
+
+def _plan(b, cells, Rw, M, precision="fp64"):
+    from lresdft_amd.lcao3d import LRFTLCAOCpp
+    return LRFTLCAOCpp(b, cells, R_w=Rw, M=M, precision=precision)
+
+
+@pytest.mark.parametrize("M", [32, 64])
+def test_fixed_mu_pass_matches_oracle(sys3, M):
+    cr, b, v = sys3
+    print(f"[gpu] fixed-mu pass n=3 R_w=8 M={M}", flush=True)
+    cells = cell_centres((3, 3, 3), 6.0)
+    C = _plan(b, cells, 8.0, M)
+    H = b.H(v)
+    W, own = build_windows(b, cells, 8.0)
+    lam_min = min(linalg.eigh(H[w][:, w].toarray(), b.S[w][:, w].toarray(), eigvals_only=True)[0] for w in W)
+    e_lo, mu = lam_min - 0.3, lam_min + 0.35
+    from lresdft_amd.lcao3d import csr64
+    C.plan.set_h(csr64(H))
+    nsum, esum = C.plan.run(mu, e_lo, want_density=True)
+    dens = C.plan.dens.cpu().numpy()
+    NB = C.plan.NB.cpu().numpy()
+    EB = C.plan.EB.cpu().numpy()
+    dp, NBo, EBo = pole_window_pass(H, b.S, b.Xr, W, own, cr.dV, mu, e_lo, M, True)
+    de, lam, c, lamoff, _ = eig_window_pass(H, b.S, b.Xr, W, own, cr.dV, mu, e_lo, M, None, True)
+    assert np.max(np.abs(dens - dp)) < 1e-12
+    assert np.max(np.abs(dens - de)) < 1e-12
+    assert np.max(np.abs(NB - NBo)) < 1e-12 and np.max(np.abs(EB - EBo)) < 1e-12
+    occ = r_M(lam, mu, e_lo, M)
+    assert abs(nsum - np.dot(c, occ)) < 1e-11
+    assert abs(esum - np.dot(c, lam * occ)) < 1e-11
+    # bitwise rerun
+    n2, e2 = C.plan.run(mu, e_lo, want_density=True)
+    assert n2 == nsum and e2 == esum
+    assert np.array_equal(C.plan.dens.cpu().numpy(), dens)
+
+
+def test_inertia_counts_match_library_eigenvalues(sys3):
+    cr, b, v = sys3
+    print("[gpu] inertia n=3 R_w=8", flush=True)
+    cells = cell_centres((3, 3, 3), 6.0)
+    C = _plan(b, cells, 8.0, 32)
+    H = b.H(v)
+    from lresdft_amd.lcao3d import csr64
+    C.plan.set_h(csr64(H))
+    lams = [linalg.eigh(H[w][:, w].toarray(), b.S[w][:, w].toarray(), eigvals_only=True) for w in C.windows]
+    allv = np.concatenate(lams)
+    sig = np.array([allv.min() - 0.1, allv.min() + 1e-3, -0.2, 0.0, 0.3, 1.0, allv.max() + 1.0])
+    tot = C.plan.inertia(sig)
+    for s, t in zip(sig, tot):
+        assert t == int(np.sum(allv < s)), s
+    lmin = C.plan.lambda_min(1e-11)
+    assert abs(lmin - allv.min()) < 1e-10
+
+
+def test_density_cold_then_warm_matches_oracle(sys3):
+    cr, b, v = sys3
+    print("[gpu] density cold/warm n=3 R_w=8 M=32", flush=True)
+    cells = cell_centres((3, 3, 3), 6.0)
+    ref = LResEigen(b, cells, R_w=8.0, M=32, full_spectrum=True)
+    C = _plan(b, cells, 8.0, 32)
+    for step in range(2):
+        nr, er, mur = ref.density(v)
+        ng, eg, mug = C.density(v)
+        assert abs(eg - er) < 1e-9, step
+        assert np.max(np.abs(ng - nr)) < 1e-10, step
+        assert abs(mug - mur) < 1e-8, step
+    assert C.history[0] == "cold" and C.history[1].startswith("warm")
+
+
+def test_forced_mu_resolve_matches_oracle(sys3):
+    """tests/test_three_d.py:142-156 scenario: perturb mu, warm step on 0.999 v."""
+    cr, b, v = sys3
+    print("[gpu] forced mu re-solve n=3 R_w=8 M=64", flush=True)
+    cells = cell_centres((3, 3, 3), 6.0)
+    ref = LResEigen(b, cells, R_w=8.0, M=64, full_spectrum=True, reuse_vectors=False)
+    C = _plan(b, cells, 8.0, 64)
+    ref.density(v)
+    C.density(v)
+    ref.mu += 0.02
+    C.mu += 0.02
+    nr, er, _ = ref.density(0.999 * v)
+    ng, eg, _ = C.density(0.999 * v)
+    assert abs(eg - er) < 1e-9
+    assert np.max(np.abs(ng - nr)) < 1e-10
+    assert C.history[-1] == "warm-resolve"
+
+
+def test_one_shot_entry_point_matches_plan(sys3):
+    cr, b, v = sys3
+    print("[gpu] one-shot lres_windows n=3 R_w=8 M=32", flush=True)
+    from lresdft_amd.lcao3d import csr64, lres_windows, build_windows as native_windows
+    cells = cell_centres((3, 3, 3), 6.0)
+    Wf, Wo, Of, Oo = native_windows(b, cells, 8.0)
+    H = b.H(v)
+    W, own = build_windows(b, cells, 8.0)
+    lam_min = min(linalg.eigh(H[w][:, w].toarray(), b.S[w][:, w].toarray(), eigvals_only=True)[0] for w in W)
+    e_lo, mu = lam_min - 0.3, lam_min + 0.4
+    dens, NB, EB = lres_windows(csr64(H), csr64(b.S), csr64(b.Xr), Wf, Wo, Of, Oo, b.nb, cr.ng, cr.dV, mu, e_lo, 32)
+    dp, NBo, EBo = pole_window_pass(H, b.S, b.Xr, W, own, cr.dV, mu, e_lo, 32, True)
+    assert np.max(np.abs(dens - dp)) < 1e-12 and np.max(np.abs(NB - NBo)) < 1e-12
+
+
+def test_lres_equals_ks_when_window_is_whole_cell():
+    """tests/test_three_d.py:51-60 on the GPU path: n=2, R_w 20 (n_W = 96)."""
+    print("[gpu] whole-cell window n=2", flush=True)
+    cr = crystal(2)
+    b = LCAOBasis(cr, **BASIS)
+    v = guess_v(cr)
+    nk, ebk, _ = ks_lcao(b, v)
+    nl, ebl, _ = _plan(b, cell_centres((2, 2, 2), 6.0), 20.0, 64).density(v)
+    assert abs(ebl - ebk) < 1e-10
+    assert np.sum(np.abs(nl - nk)) * cr.dV < 1e-9
+
+
+def test_scf_matches_oracle_scf():
+    """Full SCF on n=2 at R_w 8 (n_W 84), M 32: GPU LRes vs oracle LRes (full spectra)."""
+    print("[gpu] SCF n=2 R_w=8 M=32", flush=True)
+    cr = crystal(2)
+    b = LCAOBasis(cr, **BASIS)
+    cells = cell_centres((2, 2, 2), 6.0)
+    ref = scf_lcao(cr, LResEigen(b, cells, R_w=8.0, M=32, full_spectrum=True, reuse_vectors=False), tol=1e-8)
+    gpu = scf_lcao(cr, _plan(b, cells, 8.0, 32), tol=1e-8)
+    assert ref["converged"] and gpu["converged"]
+    assert abs(gpu["energy"] - ref["energy"]) / cr.n_elec < 1e-10
+    assert gpu["iterations"] == ref["iterations"]
+
+
+def test_mixed_precision_pass_error_is_fp32_level(sys3):
+    """The complex64 sweep without refinement: relative error ~1e-5 (fp32 GJ measured 8e-6 on these pencils)."""
+    cr, b, v = sys3
+    print("[gpu] fp32 pass n=3 R_w=8 M=32", flush=True)
+    cells = cell_centres((3, 3, 3), 6.0)
+    C = _plan(b, cells, 8.0, 32)
+    from lresdft_amd.lcao3d import csr64
+    C.plan.set_h(csr64(b.H(v)))
+    lmin = C.plan.lambda_min()
+    e_lo, mu = lmin - 0.3, lmin + 0.35
+    n64, e64 = C.plan.run(mu, e_lo)
+    d64 = C.plan.dens.cpu().numpy().copy()
+    C.set_precision("mixed")
+    n32, e32 = C.plan.run(mu, e_lo)
+    d32 = C.plan.dens.cpu().numpy()
+    assert abs(n32 - n64) / n64 < 1e-4 and abs(e32 - e64) / abs(e64) < 1e-4
+    assert np.max(np.abs(d32 - d64)) < 1e-4 * np.max(np.abs(d64))
