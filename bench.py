@@ -1,0 +1,176 @@
+#!/usr/bin/env python
+"""LRes SCF wall time at 1024 electrons on one MI355X: mixed precision vs FP64.
+
+Workload: 512 H2 (1024 electrons) on an 8x8x8 simple-cubic lattice, a = 6.0
+bohr, bond 1.4 bohr, random orientations and 0.15 bohr jitter from seed 0, grid
+10 points per cell (80^3), default LCAO basis (rcut 4.5, s 1.2/0.4/0.15, p 0.5),
+local pseudopotential rc 0.6, closed-shell LDA, window radius 8 bohr (512
+windows of 84 functions), M = 32 poles, SCF from the uniform density to
+|dn|/N < 1e-6 (the reference scf_lcao defaults).  Synthetic geometry: no data.
+
+One step = one full SCF (cold start) with the mixed-precision window step;
+the same SCF with the FP64 window step is timed alongside (same iteration
+count) and reported as fp64_scf_s.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "LRes SCF wall time @1024 e⁻: FP32+refine vs FP64 GPU at ΔE ≤2e-7 Ha/e⁻"
+PEAK_FP64 = 78.6e12      # MI355X dense FP64 (vector and matrix), FLOP/s
+PEAK_FP32 = 157.3e12     # MI355X FP32, FLOP/s
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=8, help="lattice cells per axis")
+    ap.add_argument("--rw", type=float, default=8.0)
+    ap.add_argument("--M", type=int, default=32)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def run_scf(C, cr, grid, vion, precision, tol):
+    import torch
+    from lresdft_amd.scf import scf_lcao
+    C.set_precision(precision)
+    C.mu = None
+    p0 = C.passes
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    r = scf_lcao(cr, C, tol=tol, grid=grid, vion=vion)
+    torch.cuda.synchronize()
+    r["wall"] = time.perf_counter() - t
+    r["passes"] = C.passes - p0
+    return r
+
+
+def roofline(stats, flops, peak):
+    passes, ms = stats
+    if passes == 0 or ms <= 0:
+        return None
+    t = ms * 1e-3 / passes
+    achieved = flops / t
+    return {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+            "frac": achieved / peak, "traffic": None, "kernel": "k_pole_gj", "launch_ms": ms / passes,
+            "flops_per_launch": flops}
+
+
+def cpu_baseline(cr_p, v_host, n, rw, M, iters, budget_s=40.0):
+    """Oracle (eigen route, scipy LAPACK) timed on the host on a bounded sample."""
+    from threadpoolctl import threadpool_limits
+    from oracle import lres_oracle as orc
+    cr = orc.h2_crystal((n, n, n), a=6.0, pts_per_cell=10, rc=0.6)
+    b = orc.LCAOBasis(cr, **orc.BASIS)
+    cells = orc.cell_centres((n, n, n), 6.0)
+    # the host share of one GPU is 16 cores, whatever the machine reports
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    out = {}
+    for threads in (1, cores):
+        with threadpool_limits(1):
+            t = time.perf_counter()
+            H = b.H(v_host)
+            t_h = time.perf_counter() - t
+            t = time.perf_counter()
+            dens = np.full(cr.shape, cr.n_elec / (cr.ng * cr.dV))
+            orc.hartree(cr, dens)
+            orc.lda_xc(dens)
+            t_pot = time.perf_counter() - t
+            L = orc.LResEigen(b, cells, R_w=rw, M=M, nthreads=threads)
+            t = time.perf_counter()
+            L.density(v_host)
+            t_cold = time.perf_counter() - t
+            t = time.perf_counter()
+            L.density(v_host)
+            t_warm = time.perf_counter() - t
+            sub = cr.pos[:64]
+            crs = orc.Crystal(sub, np.ones(len(sub)), cr.cell, cr.shape, cr.rc)
+            t = time.perf_counter()
+            orc.ion_potential(crs)
+            t_ion = (time.perf_counter() - t) * len(cr.pos) / len(sub)
+        est = t_ion + t_cold + (iters - 1) * t_warm + iters * (t_h + t_pot)
+        out[threads] = dict(est=est, t_cold=t_cold, t_warm=t_warm, t_H=t_h, t_pot=t_pot, t_ion=t_ion)
+    one, allc = out[1], out[cores]
+    return {"value": allc["est"], "unit": "s", "cores": cores, "kind": "port",
+            "value_1thread": one["est"],
+            "sample": (f"oracle eigen-route LRes (scipy eigh per window, R_w {rw}, M {M}) on the same "
+                       f"{n}^3 crystal: one cold + one warm density build, one H(v), Hartree+LDA, ion "
+                       f"potential on 64 atoms scaled; SCF estimate = ion + cold + (it-1) warm + it (H + pot) "
+                       f"with it = {iters} (the GPU run's iterations)"),
+            "parts_1thread": one, "parts_all_cores": allc}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the window step shards by "
+              "windows across GPUs, which this build does not launch", file=sys.stderr)
+        sys.exit(2)
+    from lresdft_amd import _native
+    _native.lib()
+    import torch
+    from lresdft_amd import system
+    from lresdft_amd.lcao3d import LRFTLCAOCpp
+    from lresdft_amd.scf import GridOps
+
+    n = args.n
+    cr = system.h2_crystal((n, n, n), a=6.0, pts_per_cell=10, rc=0.6)
+    b = system.LCAOBasis(cr, **system.DEFAULT_BASIS)
+    cells = system.cell_centres((n, n, n), 6.0)
+    grid = GridOps(cr)
+    vion = grid.ion_potential()
+    C = LRFTLCAOCpp(b, cells, R_w=args.rw, M=args.M)
+    flops = C.plan.flops_per_pass()
+
+    for _ in range(args.warmup):
+        run_scf(C, cr, grid, vion, "fp64", args.tol)
+        run_scf(C, cr, grid, vion, "mixed", args.tol)
+
+    C.plan.stats(reset=True)
+    fp64 = [run_scf(C, cr, grid, vion, "fp64", args.tol) for _ in range(args.steps)]
+    st64 = C.plan.stats(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mixed = [run_scf(C, cr, grid, vion, "mixed", args.tol) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t_total = time.perf_counter() - t0
+    st32 = C.plan.stats(reset=True)
+
+    t_mixed = t_total / args.steps
+    t_fp64 = sum(r["wall"] for r in fp64) / args.steps
+    dE = (mixed[-1]["energy"] - fp64[-1]["energy"]) / cr.n_elec
+    line = {
+        "metric": METRIC, "value": t_mixed, "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * t_mixed, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64+f64", "data": "synthetic",
+        "config": {"workload": f"h2_crystal {n}x{n}x{n} a=6.0 jitter 0.15 seed 0, {cr.n_elec} e-, "
+                               f"R_w {args.rw}, M {args.M}, SCF tol {args.tol}",
+                   "windows": int(C.plan.nwin), "n_W": int(C.plan.n.max()), "grid": list(cr.shape),
+                   "buckets": C.plan.buckets()},
+        "fp64_scf_s": t_fp64, "speedup_vs_fp64": t_fp64 / t_mixed, "dE_per_e_vs_fp64": dE,
+        "energy_fp64": fp64[-1]["energy"], "energy_mixed": mixed[-1]["energy"],
+        "iterations": {"fp64": fp64[-1]["iterations"], "mixed": mixed[-1]["iterations"]},
+        "passes_per_scf": {"fp64": fp64[-1]["passes"], "mixed": mixed[-1]["passes"]},
+        "roofline": roofline(st32["fp32"], flops, PEAK_FP32) or roofline(st32["fp64"], flops, PEAK_FP64),
+        "roofline_fp64": roofline(st64["fp64"], flops, PEAK_FP64),
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        v_host = (vion + grid.hartree(fp64[-1]["dens_in"]) + grid.lda_xc(fp64[-1]["dens_in"])[1]).cpu().numpy()
+        line["cpu_baseline"] = cpu_baseline(cr, v_host, n, args.rw, args.M, fp64[-1]["iterations"])
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -66,6 +66,12 @@ int lres_plan_destroy(lres_plan* plan);
 /* Window sizes n_W and |T_B| (functions with support on own(B)); h_n, h_k of length nwin. */
 int lres_plan_sizes(const lres_plan* plan, int32_t* h_n, int32_t* h_k);
 
+/* Window buckets of the plan (windows grouped by size, one pole-kernel launch
+ * each): for bucket i < min(cap, *h_nbucket), h_out[6i..6i+5] = {largest n_W,
+ * windows, fp64 tile T, fp64 grid G, fp32 tile T, fp32 grid G} (T = G = 0 until
+ * the first pass in that precision has run the autotuner). */
+int lres_plan_buckets(const lres_plan* plan, int32_t* h_out, int cap, int* h_nbucket);
+
 /* Set precision (LRES_FP64 / LRES_MIXED) for later passes. */
 int lres_plan_set_precision(lres_plan* plan, int precision);
 
@@ -77,6 +83,20 @@ int lres_set_hamiltonian_csr(lres_plan* plan, const int64_t* h_Hp, const int64_t
 /* Same, with the CSR already in device memory (d_Hp has nb+1 entries). */
 int lres_set_hamiltonian_csr_device(lres_plan* plan, const int64_t* d_Hp, const int64_t* d_Hi,
                                     const double* d_Hv);
+
+/* Basis data for assembling H(v) on the device (replaces LCAOBasis.H,
+ * lrft/lcao3d.py:98-100, and the host CSR round trip of lcao3d.py:411):
+ * atom_of_fn (nb), X (grid x nb CSR, as at plan creation), the fixed kinetic
+ * matrix T (nb x nb CSR, LCAOBasis.T, lcao3d.py:92-96) and the window tables of
+ * plan creation.  At most 8 functions per atom. */
+int lres_plan_set_basis(lres_plan* plan, const int64_t* h_atom_of_fn, int64_t natom,
+                        const int64_t* h_Xp, const int64_t* h_Xi, const double* h_Xv,
+                        const int64_t* h_Tp, const int64_t* h_Ti, const double* h_Tv,
+                        const int64_t* h_Wflat, const int64_t* h_Woff);
+
+/* H_W = T_W + [X^T diag(v) X dV]_W for every window from a device potential
+ * d_v (ng doubles, C order of the grid); afterwards passes use this H. */
+int lres_set_potential(lres_plan* plan, const double* d_v);
 
 /* One window pass at (mu, e_lo): the pole form of lrft_windows
  * (src/lrft_cpp.cpp:129-233).  Per window B it forms
@@ -118,6 +138,12 @@ int lres_windows(const int64_t* h_Hp, const int64_t* h_Hi, const double* h_Hv,
                  const int64_t* h_Wflat, const int64_t* h_Woff, int64_t nwin,
                  const int64_t* h_Oflat, const int64_t* h_Ooff, int64_t nb, int64_t ng, double dV,
                  double mu, double e_lo, int M, double* h_dens, double* h_NB, double* h_EB);
+
+/* Cumulative pole-kernel statistics of passes that returned host sums:
+ * h_out4 = {fp64 passes, fp64 kernel ms, fp32 passes, fp32 kernel ms}, measured
+ * with HIP events around the pole-kernel launches on the plan's stream.
+ * reset != 0 zeroes the counters after reading. */
+int lres_plan_stats(lres_plan* plan, double* h_out4, int reset);
 
 /* Timing of the last lres_window_pass, in milliseconds, measured with HIP events
  * on the plan's stream: {gather (set_hamiltonian), pole solves, reduction}. */

@@ -31,9 +31,13 @@ SIGNATURES = {
                                    _i64p, _i64p, _f64p, _vp]),
     "lres_plan_destroy": (C.c_int, [_vp]),
     "lres_plan_sizes": (C.c_int, [_vp, _i32p, _i32p]),
+    "lres_plan_buckets": (C.c_int, [_vp, _i32p, C.c_int, C.POINTER(C.c_int)]),
     "lres_plan_set_precision": (C.c_int, [_vp, C.c_int]),
     "lres_set_hamiltonian_csr": (C.c_int, [_vp, _i64p, _i64p, _f64p, C.c_int64]),
     "lres_set_hamiltonian_csr_device": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "lres_plan_set_basis": (C.c_int, [_vp, _i64p, C.c_int64, _i64p, _i64p, _f64p, _i64p, _i64p, _f64p,
+                                      _i64p, _i64p]),
+    "lres_set_potential": (C.c_int, [_vp, _vp]),
     "lres_window_pass": (C.c_int, [_vp, C.c_double, C.c_double, _vp, _vp, _vp, _f64p]),
     "lres_inertia": (C.c_int, [_vp, _f64p, C.c_int, _i64p, _vp]),
     "lres_lambda_min": (C.c_int, [_vp, C.c_double, _f64p]),
@@ -43,6 +47,7 @@ SIGNATURES = {
                                _i64p, _i64p, C.c_int64, _i64p, _i64p, C.c_int64, C.c_int64, C.c_double,
                                C.c_double, C.c_double, C.c_int, _f64p, _f64p, _f64p]),
     "lres_last_timing": (C.c_int, [_vp, _f64p]),
+    "lres_plan_stats": (C.c_int, [_vp, _f64p, C.c_int]),
 }
 
 _lib = None

@@ -18,7 +18,7 @@ struct WinDesc {
 hipError_t launch_gather_csr(const lres::WinDesc* wins, int nwin, const int64_t* Wflat, const int64_t* Woff,
                              const int32_t* pos, const int64_t* Hp, const int64_t* Hi, const double* Hv,
                              double* out, hipStream_t st);
-hipError_t launch_pole_gj(int ts, int fp32, const lres::WinDesc* wins, const int32_t* ids, int nids,
+hipError_t launch_pole_gj(int T, int G, int fp32, const lres::WinDesc* wins, const int32_t* ids, int nids,
                           const double* SW, const double* HW, int M, int ngroup, double c, double Rr,
                           double* Dpart, double* Fpart, hipStream_t st);
 hipError_t launch_reduce(const lres::WinDesc* wins, int nwin, int ngroup, const double* Dpart, const double* Fpart,
@@ -27,6 +27,14 @@ hipError_t launch_density(const lres::WinDesc* wins, int nwin, const double* D, 
                           const int64_t* xptr, const int32_t* xidx, const double* xval, double* dens,
                           hipStream_t st);
 hipError_t launch_sum2(const double* a, const double* b, int n, double* out, hipStream_t st);
-hipError_t launch_inertia(int ts, const lres::WinDesc* wins, const int32_t* ids, int nids, const double* SW,
+hipError_t launch_inertia(int T, int G, const lres::WinDesc* wins, const int32_t* ids, int nids, const double* SW,
                           const double* HW, const double* sigma, int nshift, int32_t* counts,
                           unsigned long long* total, hipStream_t st);
+hipError_t launch_pair_v(int nf_max, int npair, const int32_t* pairA, const int32_t* pairB, const int64_t* pent_off,
+                         const int32_t* ent_r, const int32_t* ent_a, const int32_t* ent_b, const int64_t* xat_off,
+                         const int32_t* nf_of, const double* xat, const int64_t* vp_off, const double* v, double dV,
+                         double* vpair, hipStream_t st);
+hipError_t launch_window_hv(const lres::WinDesc* wins, int nwin, const int64_t* Woff, const int32_t* lat,
+                            const int32_t* lf, const int64_t* wat_off, const int32_t* wat_g, const int64_t* wpt_off,
+                            const int32_t* pairtab, const int32_t* nf_of, const int64_t* vp_off, const double* vpair,
+                            const double* TW, double* HW, hipStream_t st);

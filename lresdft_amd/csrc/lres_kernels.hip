@@ -89,133 +89,126 @@ __global__ void __launch_bounds__(256) k_gather_csr(const WinDesc* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// pole solves: register-tiled Gauss-Jordan, TS x TS complex tile per thread
+// pole solves: register-tiled Gauss-Jordan on a G x G thread grid, T x T tile
+// per thread (cyclic: thread (ty, tx) owns rows ty + G i, columns tx + G j).
+// G = blockDim.x / G is chosen per bucket so that G T >= n with little padding
+// (n = 84: G = 21, T = 4, no padding, ~110 VGPRs -> 4 waves per SIMD).
 // ---------------------------------------------------------------------------
-template <int TS, typename R>
-__global__ void __launch_bounds__(256) k_pole_gj(const WinDesc* __restrict__ wins, const int32_t* __restrict__ ids,
-                                                 const double* __restrict__ SW, const double* __restrict__ HW,
-                                                 int M, int ngroup, double c, double Rr,
-                                                 double* __restrict__ Dpart, double* __restrict__ Fpart) {
-    constexpr int L = 16 * TS;
-    __shared__ cx<R> rowbuf[2][L];
-    __shared__ cx<R> colbuf[2][L];
+constexpr int kMaxG = 32;
+
+template <int T, typename R>
+__global__ void __launch_bounds__(T <= 4 ? 1024 : 512) k_pole_gj(const WinDesc* __restrict__ wins, const int32_t* __restrict__ ids,
+                                                  const double* __restrict__ SW, const double* __restrict__ HW,
+                                                  int M, int ngroup, int G, double c, double Rr,
+                                                  double* __restrict__ Dpart, double* __restrict__ Fpart) {
+    __shared__ cx<R> rowbuf[2][kMaxG * T];
+    __shared__ cx<R> colbuf[2][kMaxG * T];
     const int w = ids[blockIdx.x];
     const int g = blockIdx.y;
     const WinDesc d = wins[w];
     const int n = d.n, m = d.m, k = d.k;
-    const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
+    const int ty = threadIdx.x / G, tx = threadIdx.x - ty * G;
     const double* S = SW + d.mat_off;
     const double* H = HW + d.mat_off;
+    // this block's slot of the pole-group partial sums (each thread owns its tile entries)
+    double* dp = Dpart + d.kk_off * ngroup + (int64_t)g * k * k;
+    double* fp = Fpart + d.kk_off * ngroup + (int64_t)g * k * k;
 
-    R dacc[TS][TS], facc[TS][TS];
-#pragma unroll
-    for (int i = 0; i < TS; ++i)
-#pragma unroll
-        for (int j = 0; j < TS; ++j) dacc[i][j] = facc[i][j] = R(0);
-
-    cx<R> A[TS][TS];
+    cx<R> A[T][T];
+    bool first = true;
     for (int jp = g; jp < M / 2; jp += ngroup) {
         cx<double> z, wt, wz;
         pole(jp, M, c, Rr, z, wt, wz);
+        // opaque per pole: keeps the compiler from hoisting T*T tile addresses out of the pole loop
+        const double* Sp = S;
+        const double* Hq = H;
+        asm volatile("" : "+s"(Sp), "+s"(Hq));
 #pragma unroll
-        for (int i = 0; i < TS; ++i) {
-            const int r = ty + 16 * i;
+        for (int i = 0; i < T; ++i) {
+            const int r = ty + G * i;
 #pragma unroll
-            for (int j = 0; j < TS; ++j) {
-                const int col = tx + 16 * j;
+            for (int j = 0; j < T; ++j) {
+                const int col = tx + G * j;
                 if (r < n && col < n) {
-                    const double s = S[(int64_t)r * n + col], h = H[(int64_t)r * n + col];
+                    const double s = Sp[(int64_t)r * n + col], h = Hq[(int64_t)r * n + col];
                     A[i][j] = {(R)(z.re * s - h), (R)(z.im * s)};
                 } else {
                     A[i][j] = {(R)(r == col ? 1 : 0), R(0)};
                 }
             }
         }
-        for (int p = 0; p < n; ++p) {
-            const int buf = p & 1, pr = p & 15, pi = p >> 4;
-            if (ty == pr) {
+        // pivot p = G PI + pr: the tile slot PI is a compile-time constant in each copy
 #pragma unroll
-                for (int i = 0; i < TS; ++i)
-                    if (i == pi) {
-#pragma unroll
-                        for (int j = 0; j < TS; ++j) rowbuf[buf][tx + 16 * j] = A[i][j];
-                    }
-            }
-            if (tx == pr) {
-#pragma unroll
-                for (int j = 0; j < TS; ++j)
-                    if (j == pi) {
-#pragma unroll
-                        for (int i = 0; i < TS; ++i) colbuf[buf][ty + 16 * i] = A[i][j];
-                    }
-            }
-            __syncthreads();
-            const cx<R> inv = crecip(rowbuf[buf][p]);
-            cx<R> rs[TS], cc[TS];
-#pragma unroll
-            for (int j = 0; j < TS; ++j) rs[j] = cmul(rowbuf[buf][tx + 16 * j], inv);
-#pragma unroll
-            for (int i = 0; i < TS; ++i) cc[i] = colbuf[buf][ty + 16 * i];
-#pragma unroll
-            for (int i = 0; i < TS; ++i)
-#pragma unroll
-                for (int j = 0; j < TS; ++j) {
-                    A[i][j].re -= cc[i].re * rs[j].re - cc[i].im * rs[j].im;
-                    A[i][j].im -= cc[i].re * rs[j].im + cc[i].im * rs[j].re;
-                }
-            if (p >= m) {   // Gauss-Jordan part: pivot row/column take their final values
+        for (int PI = 0; PI < T; ++PI) {
+            const int pr_end = min(G, n - G * PI);
+            for (int pr = 0; pr < pr_end; ++pr) {
+                const int p = G * PI + pr, buf = p & 1;
                 if (ty == pr) {
 #pragma unroll
-                    for (int i = 0; i < TS; ++i)
-                        if (i == pi) {
-#pragma unroll
-                            for (int j = 0; j < TS; ++j) A[i][j] = rs[j];
-                        }
+                    for (int j = 0; j < T; ++j) rowbuf[buf][tx + G * j] = A[PI][j];
                 }
                 if (tx == pr) {
 #pragma unroll
-                    for (int j = 0; j < TS; ++j)
-                        if (j == pi) {
-#pragma unroll
-                            for (int i = 0; i < TS; ++i) {
-                                const cx<R> t = cmul(cc[i], inv);
-                                A[i][j] = {-t.re, -t.im};
-                            }
-                        }
+                    for (int i = 0; i < T; ++i) colbuf[buf][ty + G * i] = A[i][PI];
                 }
-                if (ty == pr && tx == pr) {
+                __syncthreads();
+                const cx<R> inv = crecip(rowbuf[buf][p]);
+                cx<R> rs[T], cc[T];
 #pragma unroll
-                    for (int i = 0; i < TS; ++i)
-                        if (i == pi) {
+                for (int j = 0; j < T; ++j) rs[j] = cmul(rowbuf[buf][tx + G * j], inv);
 #pragma unroll
-                            for (int j = 0; j < TS; ++j)
-                                if (j == pi) A[i][j] = inv;
+                for (int i = 0; i < T; ++i) cc[i] = colbuf[buf][ty + G * i];
+                // rank-1 update A -= cc rs^T: four fused multiply-adds per complex entry
+#pragma unroll
+                for (int i = 0; i < T; ++i)
+#pragma unroll
+                    for (int j = 0; j < T; ++j) {
+                        R re = A[i][j].re, im = A[i][j].im;
+                        re = fma(-cc[i].re, rs[j].re, re);
+                        re = fma(cc[i].im, rs[j].im, re);
+                        im = fma(-cc[i].re, rs[j].im, im);
+                        im = fma(-cc[i].im, rs[j].re, im);
+                        A[i][j] = {re, im};
+                    }
+                if (p >= m) {   // Gauss-Jordan part: pivot row/column take their final values
+                    if (ty == pr) {
+#pragma unroll
+                        for (int j = 0; j < T; ++j) A[PI][j] = rs[j];
+                    }
+                    if (tx == pr) {
+#pragma unroll
+                        for (int i = 0; i < T; ++i) {
+                            const cx<R> t = cmul(cc[i], inv);
+                            A[i][PI] = {-t.re, -t.im};
                         }
+                    }
+                    if (ty == pr && tx == pr) A[PI][PI] = inv;
                 }
             }
         }
         __syncthreads();   // the next pole's first step rewrites buffer 0
-        const cx<R> w_r = {(R)wt.re, (R)wt.im}, wz_r = {(R)wz.re, (R)wz.im};
+        // accumulate Re(w G) and Re(w z G) on T x T into this block's partial slot
 #pragma unroll
-        for (int i = 0; i < TS; ++i)
+        for (int i = 0; i < T; ++i) {
+            const int r = ty + G * i - m;
 #pragma unroll
-            for (int j = 0; j < TS; ++j) {
-                dacc[i][j] += w_r.re * A[i][j].re - w_r.im * A[i][j].im;
-                facc[i][j] += wz_r.re * A[i][j].re - wz_r.im * A[i][j].im;
-            }
-    }
-    const int64_t base = (d.kk_off * ngroup) + (int64_t)g * k * k;
-#pragma unroll
-    for (int i = 0; i < TS; ++i) {
-        const int r = ty + 16 * i - m;
-#pragma unroll
-        for (int j = 0; j < TS; ++j) {
-            const int col = tx + 16 * j - m;
-            if (r >= 0 && col >= 0 && r < k && col < k) {
-                Dpart[base + (int64_t)r * k + col] = (double)dacc[i][j];
-                Fpart[base + (int64_t)r * k + col] = (double)facc[i][j];
+            for (int j = 0; j < T; ++j) {
+                const int col = tx + G * j - m;
+                if (r >= 0 && col >= 0 && r < k && col < k) {
+                    const double gr = (double)A[i][j].re, gi = (double)A[i][j].im;
+                    const double dv = wt.re * gr - wt.im * gi, fv = wz.re * gr - wz.im * gi;
+                    const int64_t q = (int64_t)r * k + col;
+                    if (first) {
+                        dp[q] = dv;
+                        fp[q] = fv;
+                    } else {
+                        dp[q] += dv;
+                        fp[q] += fv;
+                    }
+                }
             }
         }
+        first = false;
     }
 }
 
@@ -314,65 +307,60 @@ __global__ void k_sum2(const double* __restrict__ a, const double* __restrict__ 
 // inertia: unpivoted LDL^T of sigma S_W - H_W, count of positive pivots
 // = number of window eigenvalues below sigma (Sylvester)
 // ---------------------------------------------------------------------------
-template <int TS>
-__global__ void __launch_bounds__(256) k_inertia(const WinDesc* __restrict__ wins, const int32_t* __restrict__ ids,
-                                                 const double* __restrict__ SW, const double* __restrict__ HW,
-                                                 const double* __restrict__ sigma, int nshift,
-                                                 int32_t* __restrict__ counts, unsigned long long* __restrict__ total) {
-    constexpr int L = 16 * TS;
-    __shared__ double rowbuf[2][L];
-    __shared__ double colbuf[2][L];
+template <int T>
+__global__ void __launch_bounds__(1024) k_inertia(const WinDesc* __restrict__ wins, const int32_t* __restrict__ ids,
+                                                  const double* __restrict__ SW, const double* __restrict__ HW,
+                                                  int G, const double* __restrict__ sigma, int nshift,
+                                                  int32_t* __restrict__ counts, unsigned long long* __restrict__ total) {
+    __shared__ double rowbuf[2][kMaxG * T];
+    __shared__ double colbuf[2][kMaxG * T];
     const int w = ids[blockIdx.x];
     const int s_id = blockIdx.y;
     const WinDesc d = wins[w];
     const int n = d.n;
-    const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
+    const int ty = threadIdx.x / G, tx = threadIdx.x - ty * G;
     const double sg = sigma[s_id];
     const double* S = SW + d.mat_off;
     const double* H = HW + d.mat_off;
-    double A[TS][TS];
+    double A[T][T];
 #pragma unroll
-    for (int i = 0; i < TS; ++i) {
-        const int r = ty + 16 * i;
+    for (int i = 0; i < T; ++i) {
+        const int r = ty + G * i;
 #pragma unroll
-        for (int j = 0; j < TS; ++j) {
-            const int col = tx + 16 * j;
+        for (int j = 0; j < T; ++j) {
+            const int col = tx + G * j;
             A[i][j] = (r < n && col < n) ? sg * S[(int64_t)r * n + col] - H[(int64_t)r * n + col]
                                          : (r == col ? 1.0 : 0.0);
         }
     }
     int npos = 0;
-    for (int p = 0; p < n; ++p) {
-        const int buf = p & 1, pr = p & 15, pi = p >> 4;
-        if (ty == pr) {
 #pragma unroll
-            for (int i = 0; i < TS; ++i)
-                if (i == pi) {
+    for (int PI = 0; PI < T; ++PI) {
+        const int pr_end = min(G, n - G * PI);
+        for (int pr = 0; pr < pr_end; ++pr) {
+            const int p = G * PI + pr, buf = p & 1;
+            if (ty == pr) {
 #pragma unroll
-                    for (int j = 0; j < TS; ++j) rowbuf[buf][tx + 16 * j] = A[i][j];
-                }
+                for (int j = 0; j < T; ++j) rowbuf[buf][tx + G * j] = A[PI][j];
+            }
+            if (tx == pr) {
+#pragma unroll
+                for (int i = 0; i < T; ++i) colbuf[buf][ty + G * i] = A[i][PI];
+            }
+            __syncthreads();
+            const double piv = rowbuf[buf][p];
+            npos += piv > 0.0 ? 1 : 0;
+            const double inv = 1.0 / piv;
+            double rs[T], cc[T];
+#pragma unroll
+            for (int j = 0; j < T; ++j) rs[j] = rowbuf[buf][tx + G * j] * inv;
+#pragma unroll
+            for (int i = 0; i < T; ++i) cc[i] = colbuf[buf][ty + G * i];
+#pragma unroll
+            for (int i = 0; i < T; ++i)
+#pragma unroll
+                for (int j = 0; j < T; ++j) A[i][j] = fma(-cc[i], rs[j], A[i][j]);
         }
-        if (tx == pr) {
-#pragma unroll
-            for (int j = 0; j < TS; ++j)
-                if (j == pi) {
-#pragma unroll
-                    for (int i = 0; i < TS; ++i) colbuf[buf][ty + 16 * i] = A[i][j];
-                }
-        }
-        __syncthreads();
-        const double piv = rowbuf[buf][p];
-        npos += piv > 0.0 ? 1 : 0;
-        const double inv = 1.0 / piv;
-        double rs[TS], cc[TS];
-#pragma unroll
-        for (int j = 0; j < TS; ++j) rs[j] = rowbuf[buf][tx + 16 * j] * inv;
-#pragma unroll
-        for (int i = 0; i < TS; ++i) cc[i] = colbuf[buf][ty + 16 * i];
-#pragma unroll
-        for (int i = 0; i < TS; ++i)
-#pragma unroll
-            for (int j = 0; j < TS; ++j) A[i][j] -= cc[i] * rs[j];
     }
     if (threadIdx.x == 0) {
         if (counts) counts[(int64_t)w * nshift + s_id] = npos;
@@ -395,26 +383,27 @@ hipError_t launch_gather_csr(const WinDesc* wins, int nwin, const int64_t* Wflat
 }
 
 template <typename R>
-static hipError_t launch_pole_ts(int ts, const WinDesc* wins, const int32_t* ids, int nids, const double* SW,
-                                 const double* HW, int M, int ngroup, double c, double Rr, double* Dpart,
-                                 double* Fpart, hipStream_t st) {
-    dim3 grid(nids, ngroup), blk(256);
-    switch (ts) {
-        case 2: hipLaunchKernelGGL((k_pole_gj<2, R>), grid, blk, 0, st, wins, ids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart); break;
-        case 3: hipLaunchKernelGGL((k_pole_gj<3, R>), grid, blk, 0, st, wins, ids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart); break;
-        case 4: hipLaunchKernelGGL((k_pole_gj<4, R>), grid, blk, 0, st, wins, ids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart); break;
-        case 5: hipLaunchKernelGGL((k_pole_gj<5, R>), grid, blk, 0, st, wins, ids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart); break;
-        case 6: hipLaunchKernelGGL((k_pole_gj<6, R>), grid, blk, 0, st, wins, ids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart); break;
+static hipError_t launch_pole_t(int T, int G, const WinDesc* wins, const int32_t* ids, int nids, const double* SW,
+                                const double* HW, int M, int ngroup, double c, double Rr, double* Dpart,
+                                double* Fpart, hipStream_t st) {
+    if (G < 1 || G > kMaxG) return hipErrorInvalidValue;
+    dim3 grid(nids, ngroup), blk(G * G);
+    switch (T) {
+        case 2: hipLaunchKernelGGL((k_pole_gj<2, R>), grid, blk, 0, st, wins, ids, SW, HW, M, ngroup, G, c, Rr, Dpart, Fpart); break;
+        case 3: hipLaunchKernelGGL((k_pole_gj<3, R>), grid, blk, 0, st, wins, ids, SW, HW, M, ngroup, G, c, Rr, Dpart, Fpart); break;
+        case 4: hipLaunchKernelGGL((k_pole_gj<4, R>), grid, blk, 0, st, wins, ids, SW, HW, M, ngroup, G, c, Rr, Dpart, Fpart); break;
+        case 5: hipLaunchKernelGGL((k_pole_gj<5, R>), grid, blk, 0, st, wins, ids, SW, HW, M, ngroup, G, c, Rr, Dpart, Fpart); break;
+        case 6: hipLaunchKernelGGL((k_pole_gj<6, R>), grid, blk, 0, st, wins, ids, SW, HW, M, ngroup, G, c, Rr, Dpart, Fpart); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_pole_gj(int ts, int fp32, const WinDesc* wins, const int32_t* ids, int nids, const double* SW,
-                          const double* HW, int M, int ngroup, double c, double Rr, double* Dpart, double* Fpart,
-                          hipStream_t st) {
-    if (fp32) return launch_pole_ts<float>(ts, wins, ids, nids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart, st);
-    return launch_pole_ts<double>(ts, wins, ids, nids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart, st);
+hipError_t launch_pole_gj(int T, int G, int fp32, const WinDesc* wins, const int32_t* ids, int nids,
+                          const double* SW, const double* HW, int M, int ngroup, double c, double Rr,
+                          double* Dpart, double* Fpart, hipStream_t st) {
+    if (fp32) return launch_pole_t<float>(T, G, wins, ids, nids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart, st);
+    return launch_pole_t<double>(T, G, wins, ids, nids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart, st);
 }
 
 hipError_t launch_reduce(const WinDesc* wins, int nwin, int ngroup, const double* Dpart, const double* Fpart,
@@ -435,19 +424,136 @@ hipError_t launch_sum2(const double* a, const double* b, int n, double* out, hip
     return hipGetLastError();
 }
 
-hipError_t launch_inertia(int ts, const WinDesc* wins, const int32_t* ids, int nids, const double* SW,
+hipError_t launch_inertia(int T, int G, const WinDesc* wins, const int32_t* ids, int nids, const double* SW,
                           const double* HW, const double* sigma, int nshift, int32_t* counts,
                           unsigned long long* total, hipStream_t st) {
-    dim3 grid(nids, nshift), blk(256);
-    switch (ts) {
-        case 2: hipLaunchKernelGGL((k_inertia<2>), grid, blk, 0, st, wins, ids, SW, HW, sigma, nshift, counts, total); break;
-        case 3: hipLaunchKernelGGL((k_inertia<3>), grid, blk, 0, st, wins, ids, SW, HW, sigma, nshift, counts, total); break;
-        case 4: hipLaunchKernelGGL((k_inertia<4>), grid, blk, 0, st, wins, ids, SW, HW, sigma, nshift, counts, total); break;
-        case 5: hipLaunchKernelGGL((k_inertia<5>), grid, blk, 0, st, wins, ids, SW, HW, sigma, nshift, counts, total); break;
-        case 6: hipLaunchKernelGGL((k_inertia<6>), grid, blk, 0, st, wins, ids, SW, HW, sigma, nshift, counts, total); break;
-        case 7: hipLaunchKernelGGL((k_inertia<7>), grid, blk, 0, st, wins, ids, SW, HW, sigma, nshift, counts, total); break;
-        case 8: hipLaunchKernelGGL((k_inertia<8>), grid, blk, 0, st, wins, ids, SW, HW, sigma, nshift, counts, total); break;
+    if (G < 1 || G > kMaxG) return hipErrorInvalidValue;
+    dim3 grid(nids, nshift), blk(G * G);
+    switch (T) {
+        case 2: hipLaunchKernelGGL((k_inertia<2>), grid, blk, 0, st, wins, ids, SW, HW, G, sigma, nshift, counts, total); break;
+        case 3: hipLaunchKernelGGL((k_inertia<3>), grid, blk, 0, st, wins, ids, SW, HW, G, sigma, nshift, counts, total); break;
+        case 4: hipLaunchKernelGGL((k_inertia<4>), grid, blk, 0, st, wins, ids, SW, HW, G, sigma, nshift, counts, total); break;
+        case 5: hipLaunchKernelGGL((k_inertia<5>), grid, blk, 0, st, wins, ids, SW, HW, G, sigma, nshift, counts, total); break;
+        case 6: hipLaunchKernelGGL((k_inertia<6>), grid, blk, 0, st, wins, ids, SW, HW, G, sigma, nshift, counts, total); break;
+        case 7: hipLaunchKernelGGL((k_inertia<7>), grid, blk, 0, st, wins, ids, SW, HW, G, sigma, nshift, counts, total); break;
+        case 8: hipLaunchKernelGGL((k_inertia<8>), grid, blk, 0, st, wins, ids, SW, HW, G, sigma, nshift, counts, total); break;
         default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// H(v) on the device (replaces LCAOBasis.H, lrft/lcao3d.py:98-100):
+//   V_AB = dV sum_{r in supp A and supp B} v(r) x_A(r) x_B(r)^T   per atom pair A <= B,
+// then H_W = T_W + V gathered into the window blocks.  The pair lists hold the
+// common grid points of two atoms' supports in ascending order, so every V
+// entry is summed in a fixed order (bitwise reproducible).
+// ---------------------------------------------------------------------------
+template <int NF>
+__global__ void __launch_bounds__(64) k_pair_v(const int32_t* __restrict__ pairA, const int32_t* __restrict__ pairB,
+                                               const int64_t* __restrict__ pent_off,
+                                               const int32_t* __restrict__ ent_r, const int32_t* __restrict__ ent_a,
+                                               const int32_t* __restrict__ ent_b,
+                                               const int64_t* __restrict__ xat_off, const int32_t* __restrict__ nf_of,
+                                               const double* __restrict__ xat, const int64_t* __restrict__ vp_off,
+                                               const double* __restrict__ v, double dV, double* __restrict__ vpair) {
+    const int p = blockIdx.x;
+    const int A = pairA[p], B = pairB[p];
+    const int nfa = nf_of[A], nfb = nf_of[B];
+    const double* XA = xat + xat_off[A];
+    const double* XB = xat + xat_off[B];
+    double acc[NF][NF];
+#pragma unroll
+    for (int i = 0; i < NF; ++i)
+#pragma unroll
+        for (int j = 0; j < NF; ++j) acc[i][j] = 0.0;
+    for (int64_t e = pent_off[p] + threadIdx.x; e < pent_off[p + 1]; e += 64) {
+        const double vr = v[ent_r[e]];
+        const double* xa = XA + (int64_t)ent_a[e] * nfa;
+        const double* xb = XB + (int64_t)ent_b[e] * nfb;
+        double bv[NF];
+#pragma unroll
+        for (int j = 0; j < NF; ++j) bv[j] = j < nfb ? xb[j] : 0.0;
+#pragma unroll
+        for (int i = 0; i < NF; ++i) {
+            const double av = i < nfa ? xa[i] * vr : 0.0;
+#pragma unroll
+            for (int j = 0; j < NF; ++j) acc[i][j] = fma(av, bv[j], acc[i][j]);
+        }
+    }
+    double* out = vpair + vp_off[p];
+#pragma unroll
+    for (int i = 0; i < NF; ++i)
+#pragma unroll
+        for (int j = 0; j < NF; ++j) {
+            double t = acc[i][j];
+            for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o, 64);
+            if (threadIdx.x == 0 && i < nfa && j < nfb) out[i * nfb + j] = t * dV;
+        }
+}
+
+// H_W[pa][pb] = T_W[pa][pb] + V(atom(a), atom(b))[fa][fb] in the T-last order.
+// Per window: lat/lf give, per T-last position, the window-local atom and the
+// function's index within that atom; the window's atom table wat_g (global atom
+// ids) and its natw x natw pair table (global pair id or -1) are at wat_off /
+// wpt_off.  Pairs are stored once with A <= B (global ids); the other
+// orientation reads the transposed block.
+__global__ void __launch_bounds__(256) k_window_hv(const WinDesc* __restrict__ wins, const int64_t* __restrict__ Woff,
+                                                   const int32_t* __restrict__ lat_of_pos,
+                                                   const int32_t* __restrict__ lf_of_pos,
+                                                   const int64_t* __restrict__ wat_off,
+                                                   const int32_t* __restrict__ wat_g,
+                                                   const int64_t* __restrict__ wpt_off,
+                                                   const int32_t* __restrict__ pairtab,
+                                                   const int32_t* __restrict__ nf_of,
+                                                   const int64_t* __restrict__ vp_off,
+                                                   const double* __restrict__ vpair, const double* __restrict__ TW,
+                                                   double* __restrict__ HW) {
+    const int w = blockIdx.x;
+    const WinDesc d = wins[w];
+    const int n = d.n;
+    const int32_t* lat = lat_of_pos + Woff[w];
+    const int32_t* lf = lf_of_pos + Woff[w];
+    const int32_t* gat = wat_g + wat_off[w];
+    const int natw = (int)(wat_off[w + 1] - wat_off[w]);
+    const int32_t* ptab = pairtab + wpt_off[w];
+    const double* T = TW + d.mat_off;
+    double* Hout = HW + d.mat_off;
+    for (int64_t q = threadIdx.x; q < (int64_t)n * n; q += blockDim.x) {
+        const int pa = (int)(q / n), pb = (int)(q - (int64_t)pa * n);
+        const int la = lat[pa], lb = lat[pb];
+        double val = T[q];
+        const int pid = ptab[la * natw + lb];
+        if (pid >= 0) {
+            const int ga = gat[la], gb = gat[lb];
+            const double* blk = vpair + vp_off[pid];
+            val += ga <= gb ? blk[lf[pa] * nf_of[gb] + lf[pb]] : blk[lf[pb] * nf_of[ga] + lf[pa]];
+        }
+        Hout[q] = val;
+    }
+}
+
+hipError_t launch_pair_v(int nf_max, int npair, const int32_t* pairA, const int32_t* pairB, const int64_t* pent_off,
+                         const int32_t* ent_r, const int32_t* ent_a, const int32_t* ent_b, const int64_t* xat_off,
+                         const int32_t* nf_of, const double* xat, const int64_t* vp_off, const double* v, double dV,
+                         double* vpair, hipStream_t st) {
+    if (npair == 0) return hipSuccess;
+    if (nf_max <= 6)
+        hipLaunchKernelGGL(k_pair_v<6>, dim3(npair), dim3(64), 0, st, pairA, pairB, pent_off, ent_r, ent_a, ent_b,
+                           xat_off, nf_of, xat, vp_off, v, dV, vpair);
+    else if (nf_max <= 8)
+        hipLaunchKernelGGL(k_pair_v<8>, dim3(npair), dim3(64), 0, st, pairA, pairB, pent_off, ent_r, ent_a, ent_b,
+                           xat_off, nf_of, xat, vp_off, v, dV, vpair);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_window_hv(const WinDesc* wins, int nwin, const int64_t* Woff, const int32_t* lat, const int32_t* lf,
+                            const int64_t* wat_off, const int32_t* wat_g, const int64_t* wpt_off,
+                            const int32_t* pairtab, const int32_t* nf_of, const int64_t* vp_off, const double* vpair,
+                            const double* TW, double* HW, hipStream_t st) {
+    hipLaunchKernelGGL(k_window_hv, dim3(nwin), dim3(256), 0, st, wins, Woff, lat, lf, wat_off, wat_g, wpt_off,
+                       pairtab, nf_of, vp_off, vpair, TW, HW);
     return hipGetLastError();
 }

@@ -5,9 +5,11 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/lres.h"
@@ -54,7 +56,38 @@ struct DevBuf {
     ~DevBuf() { release(); }
 };
 
-constexpr int kMaxTileRegister = 6;   // register-resident Gauss-Jordan up to n = 96
+constexpr int kMaxRegisterN = 128;    // register-resident Gauss-Jordan up to n = 128
+
+// Kernel shape of one bucket: T x T tile per thread on a G x G thread grid.
+struct Shape {
+    int T = 0, G = 0;
+};
+
+// Valid pole-kernel shapes for windows of at most nmax functions.
+std::vector<Shape> pole_shapes(int nmax) {
+    std::vector<Shape> out;
+    for (int T = 2; T <= 6; ++T) {
+        const int G = (nmax + T - 1) / T;
+        const int maxthreads = T <= 4 ? 1024 : 512;
+        if (G >= 2 && G <= 32 && G * G <= maxthreads) out.push_back({T, G});
+    }
+    return out;
+}
+
+Shape default_pole_shape(int nmax) {
+    const auto v = pole_shapes(nmax);
+    for (const auto& s : v)
+        if (s.T == 4) return s;
+    return v.empty() ? Shape{} : v.front();
+}
+
+Shape inertia_shape(int nmax) {
+    for (int T = 2; T <= 8; ++T) {
+        const int G = (nmax + T - 1) / T;
+        if (G <= 32) return {T, std::max(G, 2)};
+    }
+    return {};
+}
 
 }  // namespace
 
@@ -67,12 +100,20 @@ struct lres_plan {
     hipStream_t st = nullptr;
     std::vector<WinDesc> wins;
     std::vector<int32_t> n_of, k_of;
-    std::vector<std::vector<int32_t>> bucket;   // window ids per register tile size (index = TS)
+    struct Bucket {
+        int nmax = 0;
+        std::vector<int32_t> ids;
+        int64_t id_off = 0;
+        Shape shape[2];        // pole kernel shape per precision (0 = fp64, 1 = fp32)
+        bool tuned[2] = {false, false};
+        Shape ishape;          // inertia kernel shape
+    };
+    std::vector<Bucket> buckets;
+    bool autotune = true;
     DevBuf<WinDesc> d_wins;
     DevBuf<int64_t> d_Wflat, d_Woff;
     DevBuf<int32_t> d_pos;                      // local window index -> T-last position
     DevBuf<int32_t> d_bucket_ids;
-    std::vector<int64_t> bucket_off;            // offsets into d_bucket_ids per TS
     DevBuf<double> d_SW, d_HW, d_Q, d_D, d_F, d_Dpart, d_Fpart, d_NB, d_EB, d_sums;
     DevBuf<int64_t> d_own_g, d_xptr;
     DevBuf<int32_t> d_xidx;
@@ -81,9 +122,19 @@ struct lres_plan {
     DevBuf<double> d_Hv;
     DevBuf<double> d_sigma;
     DevBuf<unsigned long long> d_itot;
+    // H(v) assembly on the device (lres_plan_set_basis / lres_set_potential)
+    bool have_basis = false;
+    int nf_max = 0;
+    int64_t npair = 0;
+    DevBuf<int32_t> d_pairA, d_pairB, d_ent_r, d_ent_a, d_ent_b, d_nf_of, d_lat, d_lf, d_wat_g, d_pairtab;
+    DevBuf<int64_t> d_pent_off, d_xat_off, d_vp_off, d_wat_off, d_wpt_off;
+    DevBuf<double> d_xat, d_vpair, d_TW;
     bool have_H = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double timing[3] = {0, 0, 0};
+    // cumulative pole-kernel statistics per precision: passes and milliseconds (HIP events)
+    int64_t stat_passes[2] = {0, 0};
+    double stat_ms[2] = {0.0, 0.0};
     ~lres_plan() {
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -120,6 +171,43 @@ void gather_h(lres_plan* P, const int64_t* Hp, const int64_t* Hi, const double* 
     P->have_H = true;
 }
 
+// Autotuner: time every valid shape of a bucket on the loaded H (two launches
+// after one warm-up, HIP events) and keep the fastest.  All shapes apply the
+// same operations to every matrix entry, so the choice never changes results.
+void tune_bucket(lres_plan* P, lres_plan::Bucket& b, int fp32, double c, double R) {
+    const auto cands = pole_shapes(b.nmax);
+    require(!cands.empty(), "no kernel shape for windows of " + std::to_string(b.nmax) + " functions");
+    b.shape[fp32] = default_pole_shape(b.nmax);
+    b.tuned[fp32] = true;
+    if (!P->autotune || cands.size() == 1) return;
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    float best = 1e30f;
+    for (const auto& sh : cands) {
+        auto launch = [&] {
+            hip_check(launch_pole_gj(sh.T, sh.G, fp32, P->d_wins.p, P->d_bucket_ids.p + b.id_off, (int)b.ids.size(),
+                                     P->d_SW.p, P->d_HW.p, P->M, P->ngroup, c, R, P->d_Dpart.p, P->d_Fpart.p,
+                                     P->st),
+                      "pole kernel (tuning)");
+        };
+        launch();
+        hip_check(hipEventRecord(e0, P->st), "event");
+        launch();
+        launch();
+        hip_check(hipEventRecord(e1, P->st), "event");
+        hip_check(hipEventSynchronize(e1), "event sync");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "event time");
+        if (ms < best) {
+            best = ms;
+            b.shape[fp32] = sh;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
+
 // pole solves + reduction (+ density) at (mu, e_lo); sums to host if requested
 void run_pass(lres_plan* P, double mu, double e_lo, double* d_dens, double* d_NB, double* d_EB, double* h_sums) {
     require(P->have_H, "no Hamiltonian loaded: call lres_set_hamiltonian_csr first");
@@ -127,11 +215,11 @@ void run_pass(lres_plan* P, double mu, double e_lo, double* d_dens, double* d_NB
     const double c = 0.5 * (mu + e_lo), R = 0.5 * (mu - e_lo);
     hip_check(hipEventRecord(P->ev[2], P->st), "event");
     const int fp32 = P->precision == LRES_MIXED ? 1 : 0;
-    for (int ts = 1; ts <= kMaxTileRegister; ++ts) {
-        const int cnt = (int)P->bucket[ts].size();
-        if (!cnt) continue;
-        hip_check(launch_pole_gj(ts, fp32, P->d_wins.p, P->d_bucket_ids.p + P->bucket_off[ts], cnt, P->d_SW.p,
-                                 P->d_HW.p, P->M, P->ngroup, c, R, P->d_Dpart.p, P->d_Fpart.p, P->st),
+    for (auto& b : P->buckets) {
+        if (!b.tuned[fp32]) tune_bucket(P, b, fp32, c, R);
+        hip_check(launch_pole_gj(b.shape[fp32].T, b.shape[fp32].G, fp32, P->d_wins.p, P->d_bucket_ids.p + b.id_off,
+                                 (int)b.ids.size(), P->d_SW.p, P->d_HW.p, P->M, P->ngroup, c, R, P->d_Dpart.p,
+                                 P->d_Fpart.p, P->st),
                   "pole kernel");
     }
     hip_check(hipEventRecord(P->ev[3], P->st), "event");
@@ -150,7 +238,11 @@ void run_pass(lres_plan* P, double mu, double e_lo, double* d_dens, double* d_NB
         hip_check(hipStreamSynchronize(P->st), "sync");
         float t0 = 0, t1 = 0;
         if (hipEventElapsedTime(&t0, P->ev[0], P->ev[1]) == hipSuccess) P->timing[0] = t0;
-        if (hipEventElapsedTime(&t1, P->ev[2], P->ev[3]) == hipSuccess) P->timing[1] = t1;
+        if (hipEventElapsedTime(&t1, P->ev[2], P->ev[3]) == hipSuccess) {
+            P->timing[1] = t1;
+            P->stat_passes[fp32] += 1;
+            P->stat_ms[fp32] += t1;
+        }
     }
 }
 
@@ -160,13 +252,10 @@ void inertia(lres_plan* P, const double* sig, int ns, int64_t* h_total, int32_t*
     P->d_sigma.upload(sig, ns, P->st);
     if ((int)P->d_itot.n < ns) P->d_itot.alloc(ns);
     hip_check(hipMemsetAsync(P->d_itot.p, 0, ns * sizeof(unsigned long long), P->st), "memset");
-    for (int ts = 1; ts <= 8; ++ts) {
-        const int cnt = (int)P->bucket[ts].size();
-        if (!cnt) continue;
-        hip_check(launch_inertia(std::max(ts, 2), P->d_wins.p, P->d_bucket_ids.p + P->bucket_off[ts], cnt,
+    for (const auto& b : P->buckets)
+        hip_check(launch_inertia(b.ishape.T, b.ishape.G, P->d_wins.p, P->d_bucket_ids.p + b.id_off, (int)b.ids.size(),
                                  P->d_SW.p, P->d_HW.p, P->d_sigma.p, ns, d_counts, P->d_itot.p, P->st),
                   "inertia kernel");
-    }
     std::vector<unsigned long long> tot(ns);
     hip_check(hipMemcpyAsync(tot.data(), P->d_itot.p, ns * sizeof(unsigned long long), hipMemcpyDeviceToHost, P->st),
               "copy");
@@ -346,7 +435,6 @@ int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, 
         P->wins.resize(nwin);
         P->n_of.resize(nwin);
         P->k_of.resize(nwin);
-        P->bucket.assign(9, {});
         std::vector<int32_t> pos_all(Woff[nwin]);
         std::vector<int32_t> loc(nb, -1);
         std::vector<double> SW_all, Q_all;
@@ -418,20 +506,35 @@ int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, 
             P->k_of[w] = k;
             mat_off += (int64_t)n * n;
             kk_off += (int64_t)k * k;
-            const int ts = std::max(2, (n + 15) / 16);
-            require(ts <= kMaxTileRegister,
-                    "window of " + std::to_string(n) + " functions exceeds the register-resident solver (n <= 96)");
-            P->bucket[ts].push_back((int32_t)w);
+            require(n <= kMaxRegisterN, "window of " + std::to_string(n) +
+                                            " functions exceeds the register-resident solver (n <= 128)");
             for (int a = 0; a < n; ++a) loc[W[a]] = -1;
         }
         // pole groups: enough (window, group) blocks to fill 256 CUs a few times over
         P->ngroup = (int)std::min<int64_t>(M / 2, std::max<int64_t>(1, (2048 + nwin - 1) / nwin));
-        std::vector<int32_t> ids;
-        P->bucket_off.assign(9, 0);
-        for (int ts = 0; ts <= 8; ++ts) {
-            P->bucket_off[ts] = (int64_t)ids.size();
-            ids.insert(ids.end(), P->bucket[ts].begin(), P->bucket[ts].end());
+        // buckets: windows grouped by size in steps of 12 functions (one H2 of the default basis)
+        {
+            std::vector<int32_t> order(nwin);
+            for (int64_t w = 0; w < nwin; ++w) order[w] = (int32_t)w;
+            std::stable_sort(order.begin(), order.end(),
+                             [&](int32_t a, int32_t b) { return P->n_of[a] < P->n_of[b]; });
+            for (int32_t w : order) {
+                const int key = (P->n_of[w] + 11) / 12;
+                if (P->buckets.empty() || (P->buckets.back().nmax + 11) / 12 != key) P->buckets.emplace_back();
+                auto& b = P->buckets.back();
+                b.ids.push_back(w);
+                b.nmax = std::max(b.nmax, P->n_of[w]);
+            }
         }
+        std::vector<int32_t> ids;
+        for (auto& b : P->buckets) {
+            b.id_off = (int64_t)ids.size();
+            ids.insert(ids.end(), b.ids.begin(), b.ids.end());
+            b.ishape = inertia_shape(b.nmax);
+            require(!pole_shapes(b.nmax).empty(), "no kernel shape for windows of " + std::to_string(b.nmax));
+        }
+        const char* at = std::getenv("LRES_AUTOTUNE");
+        P->autotune = !(at && at[0] == '0');
         hipStream_t st = P->st;
         P->d_wins.upload(P->wins.data(), nwin, st);
         P->d_Wflat.upload(Wflat, Woff[nwin], st);
@@ -461,6 +564,183 @@ int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, 
     return rc;
 }
 
+int lres_plan_set_basis(lres_plan* P, const int64_t* aof, int64_t natom, const int64_t* Xp, const int64_t* Xi,
+                        const double* Xv, const int64_t* Tp, const int64_t* Ti, const double* Tv, const int64_t* Wflat,
+                        const int64_t* Woff) {
+    return guarded([&] {
+        require(P != nullptr, "null plan");
+        const int64_t nb = P->nb, ng = P->ng, nwin = P->nwin;
+        require(natom > 0, "no atoms");
+        std::vector<int32_t> nf(natom, 0), fidx(nb);
+        for (int64_t f = 0; f < nb; ++f) {
+            require(aof[f] >= 0 && aof[f] < natom, "atom_of_fn out of range");
+            fidx[f] = nf[aof[f]]++;
+        }
+        int nf_max = 0;
+        for (auto x : nf) nf_max = std::max(nf_max, (int)x);
+        require(nf_max <= 8, "at most 8 basis functions per atom are supported by the H(v) kernel");
+        // pass 1: support size per atom (grid points where any of its functions is stored)
+        std::vector<int64_t> last(natom, -1), supp(natom, 0);
+        for (int64_t r = 0; r < ng; ++r)
+            for (int64_t p = Xp[r]; p < Xp[r + 1]; ++p) {
+                const int64_t A = aof[Xi[p]];
+                if (last[A] != r) {
+                    last[A] = r;
+                    supp[A]++;
+                }
+            }
+        std::vector<int64_t> xoff(natom + 1, 0);
+        for (int64_t A = 0; A < natom; ++A) xoff[A + 1] = xoff[A] + supp[A] * nf[A];
+        std::vector<double> xat(xoff[natom], 0.0);
+        // pass 2: atom-blocked values and the common-support lists of atom pairs
+        std::fill(last.begin(), last.end(), -1);
+        std::vector<int64_t> cur(natom, -1);   // support index of r for atom A (valid when last[A] == r)
+        std::vector<int64_t> fillc(natom, 0);
+        std::vector<int32_t> at_r;
+        struct PairList {
+            std::vector<int32_t> r, a, b;
+        };
+        std::vector<PairList> lists;
+        std::vector<std::pair<int32_t, int32_t>> pair_atoms;
+        std::unordered_map<int64_t, int32_t> pid;
+        for (int64_t r = 0; r < ng; ++r) {
+            at_r.clear();
+            for (int64_t p = Xp[r]; p < Xp[r + 1]; ++p) {
+                const int64_t f = Xi[p];
+                const int64_t A = aof[f];
+                if (last[A] != r) {
+                    last[A] = r;
+                    cur[A] = fillc[A]++;
+                    at_r.push_back((int32_t)A);
+                }
+                xat[xoff[A] + cur[A] * nf[A] + fidx[f]] = Xv[p];
+            }
+            std::sort(at_r.begin(), at_r.end());
+            for (size_t i = 0; i < at_r.size(); ++i)
+                for (size_t j = i; j < at_r.size(); ++j) {
+                    const int32_t A = at_r[i], B = at_r[j];
+                    const int64_t key = (int64_t)A * natom + B;
+                    auto it = pid.find(key);
+                    int32_t id;
+                    if (it == pid.end()) {
+                        id = (int32_t)lists.size();
+                        pid.emplace(key, id);
+                        lists.emplace_back();
+                        pair_atoms.emplace_back(A, B);
+                    } else {
+                        id = it->second;
+                    }
+                    lists[id].r.push_back((int32_t)r);
+                    lists[id].a.push_back((int32_t)cur[A]);
+                    lists[id].b.push_back((int32_t)cur[B]);
+                }
+        }
+        const int64_t npair = (int64_t)lists.size();
+        std::vector<int32_t> pA(npair), pB(npair), er, ea, eb;
+        std::vector<int64_t> poff(npair + 1, 0), vpoff(npair + 1, 0);
+        for (int64_t q = 0; q < npair; ++q) {
+            pA[q] = pair_atoms[q].first;
+            pB[q] = pair_atoms[q].second;
+            poff[q + 1] = poff[q] + (int64_t)lists[q].r.size();
+            vpoff[q + 1] = vpoff[q] + (int64_t)nf[pA[q]] * nf[pB[q]];
+        }
+        er.reserve(poff[npair]);
+        ea.reserve(poff[npair]);
+        eb.reserve(poff[npair]);
+        for (auto& L : lists) {
+            er.insert(er.end(), L.r.begin(), L.r.end());
+            ea.insert(ea.end(), L.a.begin(), L.a.end());
+            eb.insert(eb.end(), L.b.begin(), L.b.end());
+        }
+        // per-window atom tables, pair tables and dense T_W (T-last order)
+        std::vector<int32_t> lat(Woff[nwin]), lf(Woff[nwin]), wat_g, pairtab;
+        std::vector<int64_t> wat_off(nwin + 1, 0), wpt_off(nwin + 1, 0);
+        std::vector<double> TW(P->d_SW.n, 0.0);
+        std::vector<int32_t> pos_all(Woff[nwin]);
+        hip_check(hipMemcpy(pos_all.data(), P->d_pos.p, pos_all.size() * sizeof(int32_t), hipMemcpyDeviceToHost),
+                  "copy");
+        std::vector<int32_t> loc_atom(natom, -1), loc(nb, -1);
+        for (int64_t w = 0; w < nwin; ++w) {
+            const int64_t* W = Wflat + Woff[w];
+            const int n = (int)(Woff[w + 1] - Woff[w]);
+            require(n == P->n_of[w], "window tables differ from the plan's");
+            const int32_t* pos = pos_all.data() + Woff[w];
+            std::vector<int32_t> atoms;
+            for (int a = 0; a < n; ++a) {
+                const int32_t A = (int32_t)aof[W[a]];
+                if (loc_atom[A] < 0) {
+                    loc_atom[A] = (int32_t)atoms.size();
+                    atoms.push_back(A);
+                }
+                lat[Woff[w] + pos[a]] = loc_atom[A];
+                lf[Woff[w] + pos[a]] = fidx[W[a]];
+                loc[W[a]] = a;
+            }
+            const int natw = (int)atoms.size();
+            wat_off[w + 1] = wat_off[w] + natw;
+            wpt_off[w + 1] = wpt_off[w] + (int64_t)natw * natw;
+            wat_g.insert(wat_g.end(), atoms.begin(), atoms.end());
+            for (int i = 0; i < natw; ++i)
+                for (int j = 0; j < natw; ++j) {
+                    const int32_t A = std::min(atoms[i], atoms[j]), B = std::max(atoms[i], atoms[j]);
+                    auto it = pid.find((int64_t)A * natom + B);
+                    pairtab.push_back(it == pid.end() ? -1 : it->second);
+                }
+            const int64_t m0 = P->wins[w].mat_off;
+            for (int a = 0; a < n; ++a)
+                for (int64_t p = Tp[W[a]]; p < Tp[W[a] + 1]; ++p) {
+                    const int b = loc[Ti[p]];
+                    if (b >= 0) TW[m0 + (int64_t)pos[a] * n + pos[b]] = Tv[p];
+                }
+            for (int32_t A : atoms) loc_atom[A] = -1;
+            for (int a = 0; a < n; ++a) loc[W[a]] = -1;
+        }
+        std::vector<int64_t> woff_rel(nwin + 1);
+        for (int64_t w = 0; w <= nwin; ++w) woff_rel[w] = Woff[w] - Woff[0];
+        hipStream_t st = P->st;
+        P->d_pairA.upload(pA.data(), npair, st);
+        P->d_pairB.upload(pB.data(), npair, st);
+        P->d_pent_off.upload(poff.data(), npair + 1, st);
+        P->d_ent_r.upload(er.data(), er.size(), st);
+        P->d_ent_a.upload(ea.data(), ea.size(), st);
+        P->d_ent_b.upload(eb.data(), eb.size(), st);
+        P->d_xat_off.upload(xoff.data(), natom + 1, st);
+        P->d_nf_of.upload(nf.data(), natom, st);
+        P->d_xat.upload(xat.data(), xat.size(), st);
+        P->d_vp_off.upload(vpoff.data(), npair + 1, st);
+        P->d_vpair.alloc(std::max<int64_t>(1, vpoff[npair]));
+        P->d_lat.upload(lat.data(), lat.size(), st);
+        P->d_lf.upload(lf.data(), lf.size(), st);
+        P->d_wat_off.upload(wat_off.data(), nwin + 1, st);
+        P->d_wat_g.upload(wat_g.data(), wat_g.size(), st);
+        P->d_wpt_off.upload(wpt_off.data(), nwin + 1, st);
+        P->d_pairtab.upload(pairtab.data(), pairtab.size(), st);
+        P->d_TW.upload(TW.data(), TW.size(), st);
+        hip_check(hipStreamSynchronize(st), "sync");
+        P->nf_max = nf_max;
+        P->npair = npair;
+        P->have_basis = true;
+    });
+}
+
+int lres_set_potential(lres_plan* P, const double* d_v) {
+    return guarded([&] {
+        require(P != nullptr, "null plan");
+        require(P->have_basis, "no basis: call lres_plan_set_basis first");
+        hip_check(hipEventRecord(P->ev[0], P->st), "event");
+        hip_check(launch_pair_v(P->nf_max, (int)P->npair, P->d_pairA.p, P->d_pairB.p, P->d_pent_off.p, P->d_ent_r.p,
+                                P->d_ent_a.p, P->d_ent_b.p, P->d_xat_off.p, P->d_nf_of.p, P->d_xat.p, P->d_vp_off.p,
+                                d_v, P->dV, P->d_vpair.p, P->st),
+                  "pair kernel");
+        hip_check(launch_window_hv(P->d_wins.p, (int)P->nwin, P->d_Woff.p, P->d_lat.p, P->d_lf.p, P->d_wat_off.p,
+                                   P->d_wat_g.p, P->d_wpt_off.p, P->d_pairtab.p, P->d_nf_of.p, P->d_vp_off.p,
+                                   P->d_vpair.p, P->d_TW.p, P->d_HW.p, P->st),
+                  "window H kernel");
+        hip_check(hipEventRecord(P->ev[1], P->st), "event");
+        P->have_H = true;
+    });
+}
+
 int lres_plan_destroy(lres_plan* P) {
     return guarded([&] {
         if (!P) return;
@@ -474,6 +754,23 @@ int lres_plan_sizes(const lres_plan* P, int32_t* n, int32_t* k) {
         require(P != nullptr, "null plan");
         std::copy(P->n_of.begin(), P->n_of.end(), n);
         std::copy(P->k_of.begin(), P->k_of.end(), k);
+    });
+}
+
+int lres_plan_buckets(const lres_plan* P, int32_t* out, int cap, int* nbucket) {
+    return guarded([&] {
+        require(P != nullptr, "null plan");
+        *nbucket = (int)P->buckets.size();
+        for (int i = 0; i < (int)P->buckets.size() && i < cap; ++i) {
+            const auto& b = P->buckets[i];
+            int32_t* o = out + 6 * i;
+            o[0] = b.nmax;
+            o[1] = (int32_t)b.ids.size();
+            o[2] = b.shape[0].T;
+            o[3] = b.shape[0].G;
+            o[4] = b.shape[1].T;
+            o[5] = b.shape[1].G;
+        }
     });
 }
 
@@ -615,6 +912,20 @@ int lres_windows(const int64_t* Hp, const int64_t* Hi, const double* Hv, const i
     lres_plan_destroy(P);
     g_err = keep;
     return rc;
+}
+
+int lres_plan_stats(lres_plan* P, double* out4, int reset) {
+    return guarded([&] {
+        require(P != nullptr, "null plan");
+        out4[0] = (double)P->stat_passes[0];
+        out4[1] = P->stat_ms[0];
+        out4[2] = (double)P->stat_passes[1];
+        out4[3] = P->stat_ms[1];
+        if (reset) {
+            P->stat_passes[0] = P->stat_passes[1] = 0;
+            P->stat_ms[0] = P->stat_ms[1] = 0.0;
+        }
+    });
 }
 
 int lres_last_timing(const lres_plan* P, double* ms3) {

@@ -109,6 +109,13 @@ class WindowPlan:
                 pass
             self.h = None
 
+    def buckets(self):
+        """[(nmax, nwin, T64, G64, T32, G32)] per window bucket (tile shapes chosen by the autotuner)."""
+        out = np.zeros(6 * 64, np.int32)
+        nb = nat.C.c_int()
+        nat.check(nat.lib().lres_plan_buckets(self.h, nat.pi32(out), 64, nat.C.byref(nb)))
+        return [tuple(int(x) for x in out[6 * i:6 * i + 6]) for i in range(min(nb.value, 64))]
+
     def set_precision(self, precision):
         nat.check(nat.lib().lres_plan_set_precision(self.h, nat.LRES_MIXED if precision == "mixed"
                                                     else nat.LRES_FP64))
@@ -116,6 +123,25 @@ class WindowPlan:
     def set_h(self, Hc):
         nat.check(nat.lib().lres_set_hamiltonian_csr(self.h, nat.p64(Hc[0]), nat.p64(Hc[1]), nat.pf(Hc[2]),
                                                      int(Hc[0][-1])))
+
+    def set_basis(self, basis, Wflat, Woff):
+        """Upload what H(v) assembly on the device needs (atom blocks of X, pair lists, T_W)."""
+        aof = nat.i64(basis.atom_of_fn)
+        X = self._keep[1]
+        T = csr64(basis.T)
+        self._keep_T = T
+        nat.check(nat.lib().lres_plan_set_basis(self.h, nat.p64(aof), int(aof.max()) + 1,
+                                                nat.p64(X[0]), nat.p64(X[1]), nat.pf(X[2]),
+                                                nat.p64(T[0]), nat.p64(T[1]), nat.pf(T[2]),
+                                                nat.p64(Wflat), nat.p64(Woff)))
+
+    def set_potential(self, v_dev):
+        """H_W from a device potential (float64 tensor of ng entries)."""
+        v_dev = v_dev.reshape(-1)
+        if v_dev.dtype != self.torch.float64 or not v_dev.is_contiguous() or v_dev.device.type != "cuda":
+            raise ValueError("potential must be a contiguous float64 CUDA tensor")
+        self._v_keep = v_dev
+        nat.check(nat.lib().lres_set_potential(self.h, v_dev.data_ptr()))
 
     def set_h_device(self, Hp, Hi, Hv):
         nat.check(nat.lib().lres_set_hamiltonian_csr_device(self.h, Hp.data_ptr(), Hi.data_ptr(), Hv.data_ptr()))
@@ -148,6 +174,19 @@ class WindowPlan:
         self.passes += int(info[0])
         return float(mu[0]), info
 
+    def stats(self, reset=False):
+        """{fp64: (passes, kernel ms), fp32: (...)} of the pole kernel since the last reset."""
+        t = np.zeros(4)
+        nat.check(nat.lib().lres_plan_stats(self.h, nat.pf(t), int(reset)))
+        return {"fp64": (int(t[0]), float(t[1])), "fp32": (int(t[2]), float(t[3]))}
+
+    def flops_per_pass(self):
+        """Algorithmic flops of one pass: sum_B (M/2) [(8/3) n^3 + 8 n^2 k] (complex LU + two
+        triangular solves with k right-hand sides, a complex multiply-add counted as 8)."""
+        n = self.n.astype(np.float64)
+        k = self.k.astype(np.float64)
+        return float(np.sum((self.M // 2) * ((8.0 / 3.0) * n**3 + 8.0 * n**2 * k)))
+
     def timing(self):
         t = np.zeros(3)
         nat.check(nat.lib().lres_last_timing(self.h, nat.pf(t)))
@@ -162,7 +201,7 @@ class LRFTLCAOCpp:
     (one GPU); ``lag_mu`` keeps the reference's meaning (lcao3d.py:423-427)."""
 
     def __init__(self, basis, cells, R_w=12.1, M=64, margin=0.3, nthreads=1, lag_mu=False, reuse_vectors=True,
-                 precision="fp64", mu_xtol=1e-12):
+                 precision="fp64", mu_xtol=1e-12, device_h=True):
         if M % 2:
             raise ValueError("M must be even")
         self.b, self.M, self.margin = basis, M, margin
@@ -177,6 +216,11 @@ class LRFTLCAOCpp:
         self.X = csr64(basis.Xr)
         self.plan = WindowPlan(self.S, self.X, self.Wflat, self.Woff, self.Oflat, self.Ooff, basis.nb, cr.ng,
                                cr.dV, M, precision)
+        # H(v) assembled on the device from v (needs the kinetic matrix T of the basis);
+        # device_h=False keeps the reference's route: scipy H(v) -> CSR -> gather
+        self.device_h = bool(device_h) and hasattr(basis, "T")
+        if self.device_h:
+            self.plan.set_basis(basis, self.Wflat, self.Woff)
         self.mu = None
         self.e_lo = None
         self.history = []
@@ -238,14 +282,33 @@ class LRFTLCAOCpp:
 
     def _finish(self, mu, e_lo):
         n, e = self.plan.run(mu, e_lo, want_density=True)
-        dens = self.plan.dens.cpu().numpy()
-        return 2.0 * dens.reshape(self.b.cr.shape), 2.0 * e, n
+        return 2.0 * self.plan.dens, 2.0 * e, n
 
     def _density(self, v, N=None):
         cr = self.b.cr
         N = cr.n_elec if N is None else N
-        self.plan.set_h(csr64(self.b.H(v)))
-        return self._density_loaded(N)
+        if self.device_h:
+            torch = self.plan.torch
+            self.plan.set_potential(torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64)).cuda().reshape(-1))
+        else:
+            self.plan.set_h(csr64(self.b.H(v)))
+        dens, e_band, mu = self._density_loaded(N)
+        return dens.cpu().numpy().reshape(cr.shape), e_band, mu
+
+    def density_device(self, v_dev, N=None):
+        """density() for a device potential: returns (dens tensor [ng] incl. spin 2, e_band, mu).
+
+        Needs device_h (H(v) assembled on the GPU); the density tensor is the
+        plan's buffer scaled by 2, valid until the next call."""
+        if not self.device_h:
+            raise RuntimeError("density_device needs device_h=True")
+        N = self.b.cr.n_elec if N is None else N
+        self.plan.set_potential(v_dev)
+        try:
+            return self._density_loaded(N)
+        except StaleMu:
+            self.mu = None
+            return self._density_loaded(N)
 
     def _density_loaded(self, N):
         tol = 1e-8 * N

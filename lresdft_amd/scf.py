@@ -1,0 +1,128 @@
+"""SCF driver on the GPU: the caller of the window step.
+
+Same iteration as the reference's scf_lcao (lrft/lcao3d.py:237-284): LDA + FFT
+Hartree potential of the input density, one density build, Pulay/DIIS over the
+last ``ndiis`` (density, residual) pairs with mixing ``mix``, clamp at 1e-12
+and renormalise to N, Harris-Foulkes energy at the last input density.  The grid
+arrays stay on the device (torch); only DIIS's (k+1) x (k+1) system and scalars
+go to the host.  Potentials follow lrft/three_d.py:102-148."""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+
+
+class GridOps:
+    """FFT Hartree, ion potential and PZ81 LDA on the device for one crystal."""
+
+    def __init__(self, cr, device="cuda"):
+        import torch
+        self.torch, self.cr, self.device = torch, cr, device
+        ks = [2 * np.pi * np.fft.fftfreq(m, d=L / m) for m, L in zip(cr.shape, cr.cell)]
+        self.k1 = [torch.tensor(k, dtype=torch.float64, device=device) for k in ks]
+        kx, ky, kz = torch.meshgrid(*self.k1, indexing="ij")
+        G2 = kx * kx + ky * ky + kz * kz
+        G2[0, 0, 0] = 1.0
+        self.inv_G2 = 1.0 / G2
+        self.inv_G2[0, 0, 0] = 0.0
+        self.G2 = G2
+        self.G2[0, 0, 0] = 0.0
+
+    def ion_potential(self, chunk=256):
+        """Periodic sum of -Z erf(r/rc)/r, G = 0 dropped (three_d.py:102-119).
+        The structure factor sum_a Z_a e^{-i G.R_a} is separable per axis and is
+        formed as one batched product per chunk of atoms."""
+        torch, cr = self.torch, self.cr
+        pos = torch.tensor(cr.pos, dtype=torch.float64, device=self.device)
+        Z = torch.tensor(cr.Z, dtype=torch.complex128, device=self.device)
+        n1, n2, n3 = cr.shape
+        Sg = torch.zeros((n1 * n2, n3), dtype=torch.complex128, device=self.device)
+        for a0 in range(0, len(cr.pos), chunk):
+            p = pos[a0:a0 + chunk]
+            ex = torch.exp(-1j * p[:, 0:1] * self.k1[0][None, :]) * Z[a0:a0 + chunk, None]
+            ey = torch.exp(-1j * p[:, 1:2] * self.k1[1][None, :])
+            ez = torch.exp(-1j * p[:, 2:3] * self.k1[2][None, :])
+            P = (ex[:, :, None] * ey[:, None, :]).reshape(len(p), n1 * n2)
+            Sg += P.transpose(0, 1) @ ez
+        vol = float(np.prod(cr.cell))
+        vG = -4 * math.pi * Sg.reshape(cr.shape) * torch.exp(-self.G2 * cr.rc**2 / 4) * self.inv_G2 / vol
+        return torch.fft.ifftn(vG).real * cr.ng
+
+    def hartree(self, n):
+        torch = self.torch
+        nG = torch.fft.fftn(n)
+        return torch.fft.ifftn(4 * math.pi * nG * self.inv_G2).real
+
+    def lda_xc(self, n):
+        """Slater exchange + Perdew-Zunger 1981 correlation (three_d.py:132-148)."""
+        torch = self.torch
+        n = torch.clamp(n, min=1e-14)
+        rs = (3.0 / (4.0 * math.pi * n)) ** (1.0 / 3.0)
+        ex = -0.75 * (3.0 / math.pi) ** (1.0 / 3.0) * n ** (1.0 / 3.0)
+        vx = 4.0 / 3.0 * ex
+        gam, b1, b2 = -0.1423, 1.0529, 0.3334
+        A, B, C, D = 0.0311, -0.048, 0.0020, -0.0116
+        srs = torch.sqrt(rs)
+        den = 1.0 + b1 * srs + b2 * rs
+        ec_h = gam / den
+        vc_h = ec_h * (1.0 + 7.0 / 6.0 * b1 * srs + 4.0 / 3.0 * b2 * rs) / den
+        lnr = torch.log(rs)
+        ec_l = A * lnr + B + C * rs * lnr + D * rs
+        vc_l = A * lnr + (B - A / 3.0) + 2.0 / 3.0 * C * rs * lnr + (2.0 * D - C) / 3.0 * rs
+        high = rs >= 1
+        return ex + torch.where(high, ec_h, ec_l), vx + torch.where(high, vc_h, vc_l)
+
+
+def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion=None, dens0=None):
+    """Device SCF; ``solver.density_device(v)`` returns (density tensor [ng], e_band, mu)."""
+    grid = GridOps(cr) if grid is None else grid
+    torch = grid.torch
+    t0 = time.perf_counter()
+    vion = grid.ion_potential() if vion is None else vion
+    dV, N = cr.dV, cr.n_elec
+    dens = (torch.full(cr.shape, N / (cr.ng * dV), dtype=torch.float64, device=grid.device) if dens0 is None
+            else dens0.clone())
+    hist_n, hist_r = [], []
+    converged = False
+    t_solver = 0.0
+    for it in range(1, maxiter + 1):
+        v = vion + grid.hartree(dens) + grid.lda_xc(dens)[1]
+        ts = time.perf_counter()
+        nout, eband, mu = solver.density_device(v.reshape(-1).contiguous())
+        nout = nout.reshape(cr.shape)
+        t_solver += time.perf_counter() - ts
+        resid = nout - dens
+        err = float(torch.sum(torch.abs(resid))) * dV / N
+        if err < tol:
+            converged = True
+            break
+        hist_n.append(dens.clone())
+        hist_r.append(resid.clone())
+        if len(hist_n) > ndiis:
+            hist_n.pop(0)
+            hist_r.pop(0)
+        k = len(hist_n)
+        if k > 1:
+            R = torch.stack([r.reshape(-1) for r in hist_r])
+            Bm = np.zeros((k + 1, k + 1))
+            Bm[:k, :k] = (R @ R.T).cpu().numpy()
+            Bm[k, :k] = Bm[:k, k] = 1.0
+            rhs = np.zeros(k + 1)
+            rhs[k] = 1.0
+            coef = np.linalg.lstsq(Bm, rhs, rcond=None)[0][:k]
+            dens = sum(float(c) * h for c, h in zip(coef, hist_n)) + mix * sum(float(c) * r
+                                                                             for c, r in zip(coef, hist_r))
+        else:
+            dens = dens + mix * resid
+        dens = torch.clamp(dens, min=1e-12)
+        dens = dens * (N / (float(dens.sum()) * dV))
+    vh = grid.hartree(dens)
+    exc, vxc = grid.lda_xc(dens)
+    E_H = 0.5 * float(torch.sum(dens * vh)) * dV
+    E_xc = float(torch.sum(dens * exc)) * dV
+    E = eband - float(torch.sum((vh + vxc) * dens)) * dV + E_H + E_xc
+    torch.cuda.synchronize()
+    return dict(energy=E, density=nout, iterations=it, converged=converged, dens_in=dens, eband=eband, mu=mu,
+                time=time.perf_counter() - t0, t_solver=t_solver, E_H=E_H, E_xc=E_xc)
