@@ -123,11 +123,12 @@ int lres_lambda_min(lres_plan* plan, double tol, double* h_lambda_min);
 
 /* Chemical potential: root of 2 sum_B N_B(mu) - N by Brent's method on device
  * passes, bracket [lo, hi] with count(lo) <= 0 <= count(hi); f_lo / f_hi are the
- * counts already known at lo / hi (NaN: evaluated here); xtol as brentq's.
- * Replaces _solve_mu (lcao3d.py:387-398), which the eigen route evaluates in
- * closed form.  h_info (may be NULL) receives {passes, final count}. */
+ * counts already known at lo / hi (NaN: evaluated here).  Stops when the
+ * bracket is below xtol (brentq's criterion) or |count| <= ftol.  Replaces
+ * _solve_mu (lcao3d.py:387-398), which the eigen route evaluates in closed
+ * form.  h_info (may be NULL) receives {passes, count at the returned mu}. */
 int lres_mu_solve(lres_plan* plan, double N, double e_lo, double lo, double hi, double f_lo, double f_hi,
-                  double xtol, double* h_mu, double* h_info);
+                  double xtol, double ftol, double* h_mu, double* h_info);
 
 /* One-shot form with lrft_windows' argument list (src/lrft_cpp.cpp:129-132):
  * host CSR H, S, X, window tables, nb, dV, mu, e_lo, M; outputs h_dens (ng, may be

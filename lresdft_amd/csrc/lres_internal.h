@@ -23,7 +23,7 @@ hipError_t launch_pole_gj(int T, int G, int fp32, const lres::WinDesc* wins, con
                           double* Dpart, double* Fpart, hipStream_t st);
 hipError_t launch_reduce(const lres::WinDesc* wins, int nwin, int ngroup, const double* Dpart, const double* Fpart,
                          const double* Q, double* D, double* F, double* NB, double* EB, hipStream_t st);
-hipError_t launch_density(const lres::WinDesc* wins, int nwin, const double* D, const int64_t* own_g,
+hipError_t launch_density(const lres::WinDesc* wins, int nwin, int kmax, const double* D, const int64_t* own_g,
                           const int64_t* xptr, const int32_t* xidx, const double* xval, double* dens,
                           hipStream_t st);
 hipError_t launch_sum2(const double* a, const double* b, int n, double* out, hipStream_t st);

@@ -262,26 +262,33 @@ __global__ void __launch_bounds__(256) k_reduce(const WinDesc* __restrict__ wins
     }
 }
 
-// n(g) = x_g^T D_B x_g on the owned points (src/lrft_cpp.cpp:183-191 in D form)
+// n(g) = x_g^T D_B x_g on the owned points (src/lrft_cpp.cpp:183-191 in D form).
+// D_B is staged in LDS when it fits (stage != 0, dynamic shared size k*k doubles).
 __global__ void __launch_bounds__(256) k_density(const WinDesc* __restrict__ wins, const double* __restrict__ D,
                                                  const int64_t* __restrict__ own_g,
                                                  const int64_t* __restrict__ xptr,
                                                  const int32_t* __restrict__ xidx,
-                                                 const double* __restrict__ xval, double* __restrict__ dens) {
+                                                 const double* __restrict__ xval, double* __restrict__ dens,
+                                                 int stage) {
+    extern __shared__ double Dl[];
     const int w = blockIdx.x;
     const WinDesc d = wins[w];
-    const double* Dw = D + d.kk_off;
     const int k = d.k;
+    const double* Dw = D + d.kk_off;
+    if (stage) {
+        for (int q = threadIdx.x; q < k * k; q += blockDim.x) Dl[q] = Dw[q];
+        __syncthreads();
+        Dw = Dl;
+    }
     for (int r = threadIdx.x; r < d.nown; r += blockDim.x) {
         const int64_t o = d.own_off + r;
         const int64_t p0 = xptr[o], p1 = xptr[o + 1];
         double s = 0.0;
         for (int64_t a = p0; a < p1; ++a) {
-            const double xa = xval[a];
             const double* row = Dw + (int64_t)xidx[a] * k;
             double t = 0.0;
             for (int64_t b = p0; b < p1; ++b) t += row[xidx[b]] * xval[b];
-            s += xa * t;
+            s += xval[a] * t;
         }
         dens[own_g[o]] = s;
     }
@@ -412,10 +419,13 @@ hipError_t launch_reduce(const WinDesc* wins, int nwin, int ngroup, const double
     return hipGetLastError();
 }
 
-hipError_t launch_density(const WinDesc* wins, int nwin, const double* D, const int64_t* own_g,
+hipError_t launch_density(const WinDesc* wins, int nwin, int kmax, const double* D, const int64_t* own_g,
                           const int64_t* xptr, const int32_t* xidx, const double* xval, double* dens,
                           hipStream_t st) {
-    hipLaunchKernelGGL(k_density, dim3(nwin), dim3(256), 0, st, wins, D, own_g, xptr, xidx, xval, dens);
+    const size_t lds = (size_t)kmax * kmax * sizeof(double);
+    const int stage = lds <= 64 * 1024 ? 1 : 0;   // default dynamic LDS limit per block
+    hipLaunchKernelGGL(k_density, dim3(nwin), dim3(256), stage ? lds : 0, st, wins, D, own_g, xptr, xidx, xval, dens,
+                       stage);
     return hipGetLastError();
 }
 

@@ -97,6 +97,7 @@ struct lres_plan {
     int M = 0;
     int precision = LRES_FP64;
     int ngroup = 1;
+    int kmax = 0;
     hipStream_t st = nullptr;
     std::vector<WinDesc> wins;
     std::vector<int32_t> n_of, k_of;
@@ -229,7 +230,7 @@ void run_pass(lres_plan* P, double mu, double e_lo, double* d_dens, double* d_NB
                             P->d_F.p, NB, EB, P->st),
               "reduce kernel");
     if (d_dens)
-        hip_check(launch_density(P->d_wins.p, (int)P->nwin, P->d_D.p, P->d_own_g.p, P->d_xptr.p, P->d_xidx.p,
+        hip_check(launch_density(P->d_wins.p, (int)P->nwin, P->kmax, P->d_D.p, P->d_own_g.p, P->d_xptr.p, P->d_xidx.p,
                                  P->d_xval.p, d_dens, P->st),
                   "density kernel");
     hip_check(launch_sum2(NB, EB, (int)P->nwin, P->d_sums.p, P->st), "sum kernel");
@@ -267,11 +268,11 @@ void inertia(lres_plan* P, const double* sig, int ns, int64_t* h_total, int32_t*
 // Derivatives, 1973, ch. 4): inverse quadratic interpolation / secant steps
 // safeguarded by bisection; tolerance 2 eps |b| + xtol/2 as in the textbook.
 template <typename Fn>
-double brent_root(Fn f, double a, double b, double fa, double fb, double xtol, int maxit, int* evals) {
+double brent_root(Fn f, double a, double b, double fa, double fb, double xtol, double ftol, int maxit, int* evals) {
     const double eps = std::numeric_limits<double>::epsilon();
     require(fa * fb <= 0.0, "mu bracket does not change sign");
-    if (fa == 0.0) return a;
-    if (fb == 0.0) return b;
+    if (std::fabs(fa) <= ftol) return a;
+    if (std::fabs(fb) <= ftol) return b;
     double c = a, fc = fa, d = b - a, e = d;
     for (int it = 0; it < maxit; ++it) {
         if ((fb > 0 && fc > 0) || (fb < 0 && fc < 0)) {
@@ -285,7 +286,7 @@ double brent_root(Fn f, double a, double b, double fa, double fb, double xtol, i
         }
         const double tol = 2.0 * eps * std::fabs(b) + 0.5 * xtol;
         const double mid = 0.5 * (c - b);
-        if (std::fabs(mid) <= tol || fb == 0.0) return b;
+        if (std::fabs(mid) <= tol || std::fabs(fb) <= ftol) return b;
         if (std::fabs(e) >= tol && std::fabs(fa) > std::fabs(fb)) {
             double p, q, r;
             const double s = fb / fa;
@@ -504,6 +505,7 @@ int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, 
             P->wins[w] = d;
             P->n_of[w] = n;
             P->k_of[w] = k;
+            P->kmax = std::max(P->kmax, k);
             mat_off += (int64_t)n * n;
             kk_off += (int64_t)k * k;
             require(n <= kMaxRegisterN, "window of " + std::to_string(n) +
@@ -864,24 +866,27 @@ int lres_lambda_min(lres_plan* P, double tol, double* out) {
 }
 
 int lres_mu_solve(lres_plan* P, double N, double e_lo, double lo, double hi, double f_lo, double f_hi,
-                  double xtol, double* h_mu, double* h_info) {
+                  double xtol, double ftol, double* h_mu, double* h_info) {
     return guarded([&] {
         require(P != nullptr, "null plan");
         require(lo < hi, "mu bracket must satisfy lo < hi");
         int evals = 0;
         double sums[2] = {0.0, 0.0};
+        double last_mu = NAN, last_f = NAN;
         auto f = [&](double mu) {
             run_pass(P, mu, e_lo, nullptr, nullptr, nullptr, sums);
-            return 2.0 * sums[0] - N;
+            last_mu = mu;
+            last_f = 2.0 * sums[0] - N;
+            return last_f;
         };
         double flo = f_lo, fhi = f_hi;
         if (std::isnan(flo)) { flo = f(lo); ++evals; }
         if (std::isnan(fhi)) { fhi = f(hi); ++evals; }
-        const double mu = brent_root(f, lo, hi, flo, fhi, xtol, 200, &evals);
+        const double mu = brent_root(f, lo, hi, flo, fhi, xtol, ftol, 200, &evals);
         *h_mu = mu;
         if (h_info) {
             h_info[0] = evals;
-            h_info[1] = 2.0 * sums[0] - N;
+            h_info[1] = mu == last_mu ? last_f : (mu == lo ? flo : (mu == hi ? fhi : NAN));
         }
     });
 }

@@ -166,11 +166,12 @@ class WindowPlan:
         nat.check(nat.lib().lres_lambda_min(self.h, float(tol), nat.pf(out)))
         return float(out[0])
 
-    def mu_solve(self, N, e_lo, lo, hi, f_lo=math.nan, f_hi=math.nan, xtol=1e-12):
+    def mu_solve(self, N, e_lo, lo, hi, f_lo=math.nan, f_hi=math.nan, xtol=1e-12, ftol=0.0):
+        """Brent root of the count on [lo, hi]; returns (mu, [passes, count at mu])."""
         mu = np.zeros(1)
         info = np.zeros(2)
         nat.check(nat.lib().lres_mu_solve(self.h, float(N), float(e_lo), float(lo), float(hi), float(f_lo),
-                                          float(f_hi), float(xtol), nat.pf(mu), nat.pf(info)))
+                                          float(f_hi), float(xtol), float(ftol), nat.pf(mu), nat.pf(info)))
         self.passes += int(info[0])
         return float(mu[0]), info
 
@@ -224,18 +225,38 @@ class LRFTLCAOCpp:
         self.mu = None
         self.e_lo = None
         self.history = []
+        self.precision = precision
+        self._slope = None
 
     @property
     def passes(self):
         return self.plan.passes
 
     def set_precision(self, precision):
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.precision = precision
         self.plan.set_precision(precision)
 
     # -- mu search -----------------------------------------------------------
+    # The root of count(mu) = 2 sum_B N_B(mu) - N is accepted at |count| <= ftol:
+    # fp64 passes resolve counts to ~1e-13 N; complex64 passes carry ~1e-8 N of
+    # rounding noise, so their search stops at 1e-7 N and refine() finishes the
+    # root in fp64.  Brent's xtol is the reference's (brentq xtol=1e-12).
+    def _ftol(self, N):
+        return (1e-11 if self.precision == "fp64" else 1e-7) * N
+
     def _count(self, mu, e_lo, N):
         n, _ = self.plan.run(mu, e_lo, want_density=False)
         return 2.0 * n - N
+
+    def _note_slope(self, mu0, f0, mu1, f1):
+        if mu1 != mu0 and np.isfinite(f1) and (f1 - f0) / (mu1 - mu0) > 0:
+            self._slope = (f1 - f0) / (mu1 - mu0)
+
+    def _brent(self, N, e_lo, lo, hi, f_lo, f_hi):
+        mu, info = self.plan.mu_solve(N, e_lo, lo, hi, f_lo, f_hi, self.mu_xtol, self._ftol(N))
+        return mu, info[1]
 
     def _solve_mu_cold(self, lam_min, e_lo, N):
         """Bracket as _solve_mu (lcao3d.py:390-397), then Brent on device counts."""
@@ -251,26 +272,57 @@ class LRFTLCAOCpp:
             if hi > lam_min + 50.0:
                 raise StaleMu("no electron-count bracket within 50 Ha of the window spectra")
             f_hi = self._count(hi, e_lo, N)
-        mu, _ = self.plan.mu_solve(N, e_lo, lo, hi, f_lo, f_hi, self.mu_xtol)
+        mu, f = self._brent(N, e_lo, lo, hi, f_lo, f_hi)
+        self._note_slope(lo, f_lo, hi, f_hi)
         return mu
 
     def _solve_mu_warm(self, mu0, f0, e_lo, N):
-        """Bracket outward from the previous mu (count f0 there), then Brent."""
-        step = 1e-3
-        direction = 1.0 if f0 < 0 else -1.0
+        """Secant steps from the previous mu (count f0 there) with the last known
+        slope dN/dmu until the root is bracketed, then Brent."""
+        ftol = self._ftol(N)
         a, fa = mu0, f0
-        for _ in range(40):
-            b = a + direction * step
-            if b <= e_lo:
-                b = 0.5 * (a + e_lo)
+        slope = self._slope
+        step = -fa / slope if slope else (1e-3 if fa < 0 else -1e-3)
+        for _ in range(60):
+            step = float(np.clip(step, -0.25, 0.25))
+            if abs(step) < 1e-12:
+                step = 1e-12 if fa < 0 else -1e-12
+            b = a + step
+            if b <= e_lo + 1e-6:
+                b = 0.5 * (a + e_lo + 1e-6)
             fb = self._count(b, e_lo, N)
-            if (fa < 0) != (fb < 0) or fb == 0:
+            self._note_slope(a, fa, b, fb)
+            if abs(fb) <= ftol:
+                return b
+            if (fa < 0) != (fb < 0):
                 lo, hi, flo, fhi = (a, b, fa, fb) if a < b else (b, a, fb, fa)
-                mu, _ = self.plan.mu_solve(N, e_lo, lo, hi, flo, fhi, self.mu_xtol)
+                mu, _ = self._brent(N, e_lo, lo, hi, flo, fhi)
                 return mu
+            s = (fb - fa) / (b - a)
+            # same side of the root: secant extrapolation with 20% overshoot to bracket it
+            step = -1.2 * fb / s if s > 0 else 4.0 * step
             a, fa = b, fb
-            step *= 4.0
         raise StaleMu("no electron-count bracket around the previous mu")
+
+    def refine(self, N=None):
+        """fp64 finish of a mixed-precision step: an fp64 pass at the current
+        (mu, e_lo); if its count misses N by more than the reference's keep
+        tolerance 1e-8 N, secant/Brent steps in fp64 move mu.  Returns the fp64
+        (density tensor, e_band, mu) and leaves the precision as it was."""
+        N = self.b.cr.n_elec if N is None else N
+        prec = self.precision
+        self.set_precision("fp64")
+        try:
+            mu, e_lo = self.mu, self.e_lo
+            dens, e_band, nsum = self._finish(mu, e_lo)
+            cnt = 2.0 * nsum - N
+            if abs(cnt) > 1e-8 * N:
+                mu = self._solve_mu_warm(mu, cnt, e_lo, N)
+                dens, e_band, _ = self._finish(mu, e_lo)
+                self.mu = mu
+            return dens, e_band, mu
+        finally:
+            self.set_precision(prec)
 
     # -- density (lcao3d.py:400-456) ------------------------------------------
     def density(self, v, N=None):
@@ -311,7 +363,7 @@ class LRFTLCAOCpp:
             return self._density_loaded(N)
 
     def _density_loaded(self, N):
-        tol = 1e-8 * N
+        tol = max(1e-8 * N, self._ftol(N))   # the reference's keep-mu rule, floored at the pass precision
         m = self.margin
         if self.mu is not None:
             mu, e_lo = self.mu, self.e_lo
@@ -332,10 +384,17 @@ class LRFTLCAOCpp:
                 return dens, e_band, mu
         lam_min = self.plan.lambda_min()
         e_lo = lam_min - m
-        mu = self._solve_mu_cold(lam_min, e_lo, N)
+        prev = self.mu if self.mu is not None and self.mu > e_lo + 1e-6 else None
+        if prev is not None:
+            # new filter edge, previous mu as the start of the search: the root of
+            # count(mu) at this e_lo is unique, so this only shortens the search
+            mu = self._solve_mu_warm(prev, self._count(prev, e_lo, N), e_lo, N)
+            self.history.append("cold-edge")
+        else:
+            mu = self._solve_mu_cold(lam_min, e_lo, N)
+            self.history.append("cold")
         dens, e_band, _ = self._finish(mu, e_lo)
         self.mu, self.e_lo = mu, e_lo
-        self.history.append("cold")
         return dens, e_band, mu
 
 

@@ -76,7 +76,11 @@ class GridOps:
 
 
 def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion=None, dens0=None):
-    """Device SCF; ``solver.density_device(v)`` returns (density tensor [ng], e_band, mu)."""
+    """Device SCF; ``solver.density_device(v)`` returns (density tensor [ng], e_band, mu).
+
+    With a mixed-precision solver the iterations run on complex64 passes and
+    the converged step is finished in fp64 (``solver.refine()``): the energy
+    and the returned density are fp64 results at the converged input density."""
     grid = GridOps(cr) if grid is None else grid
     torch = grid.torch
     t0 = time.perf_counter()
@@ -84,7 +88,7 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
     dV, N = cr.dV, cr.n_elec
     dens = (torch.full(cr.shape, N / (cr.ng * dV), dtype=torch.float64, device=grid.device) if dens0 is None
             else dens0.clone())
-    hist_n, hist_r = [], []
+    hist_n, hist_r, hist_g = [], [], []
     converged = False
     t_solver = 0.0
     for it in range(1, maxiter + 1):
@@ -100,14 +104,19 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
             break
         hist_n.append(dens.clone())
         hist_r.append(resid.clone())
+        # Gram matrix of the residuals, one new row of dot products per iteration
+        gram = [[float(torch.sum(ri * resid)) for ri in hist_r]] if not hist_g else \
+            [row + [float(torch.sum(ri * resid))] for ri, row in zip(hist_r[:-1], hist_g)] + \
+            [[float(torch.sum(ri * resid)) for ri in hist_r]]
+        hist_g = [row[:] for row in gram]
         if len(hist_n) > ndiis:
             hist_n.pop(0)
             hist_r.pop(0)
+            hist_g = [row[1:] for row in hist_g[1:]]
         k = len(hist_n)
         if k > 1:
-            R = torch.stack([r.reshape(-1) for r in hist_r])
             Bm = np.zeros((k + 1, k + 1))
-            Bm[:k, :k] = (R @ R.T).cpu().numpy()
+            Bm[:k, :k] = np.array(hist_g)
             Bm[k, :k] = Bm[:k, k] = 1.0
             rhs = np.zeros(k + 1)
             rhs[k] = 1.0
@@ -118,6 +127,12 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
             dens = dens + mix * resid
         dens = torch.clamp(dens, min=1e-12)
         dens = dens * (N / (float(dens.sum()) * dV))
+    if getattr(solver, "precision", "fp64") == "mixed" and hasattr(solver, "refine"):
+        # fp64 finish: mu, density and band energy of the converged input potential in fp64
+        ts = time.perf_counter()
+        nout, eband, mu = solver.refine()
+        nout = nout.reshape(cr.shape)
+        t_solver += time.perf_counter() - ts
     vh = grid.hartree(dens)
     exc, vxc = grid.lda_xc(dens)
     E_H = 0.5 * float(torch.sum(dens * vh)) * dV

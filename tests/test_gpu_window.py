@@ -211,3 +211,30 @@ def test_gpu_scf_driver_matches_oracle_scf():
     gpu = scf_gpu(cr, LRFTLCAOCpp(b, cells, R_w=8.0, M=32), tol=1e-8)
     assert gpu["converged"] and gpu["iterations"] == ref["iterations"]
     assert abs(gpu["energy"] - ref["energy"]) / cr.n_elec < 1e-10
+
+
+def test_mixed_precision_scf_meets_energy_target():
+    """complex64 passes + fp64 finish vs the fp64 window step, same SCF (n=3, R_w 8, M 32):
+    |dE| <= 2e-7 Ha per electron (the target of the mixed path); the finish makes the
+    final count satisfy the reference's keep-mu rule |count| <= 1e-8 N in fp64."""
+    print("[gpu] mixed vs fp64 SCF n=3 R_w=8 M=32", flush=True)
+    from lresdft_amd import system
+    from lresdft_amd.lcao3d import LRFTLCAOCpp
+    from lresdft_amd.scf import GridOps, scf_lcao as scf_gpu
+    cr = system.h2_crystal((3, 3, 3), a=6.0, pts_per_cell=10, rc=0.6)
+    b = system.LCAOBasis(cr, **system.DEFAULT_BASIS)
+    cells = system.cell_centres((3, 3, 3), 6.0)
+    g = GridOps(cr)
+    vion = g.ion_potential()
+    C = LRFTLCAOCpp(b, cells, R_w=8.0, M=32)
+    r64 = scf_gpu(cr, C, tol=1e-6, grid=g, vion=vion)
+    C.set_precision("mixed")
+    C.mu = None
+    r32 = scf_gpu(cr, C, tol=1e-6, grid=g, vion=vion)
+    assert r64["converged"] and r32["converged"]
+    assert abs(r32["energy"] - r64["energy"]) / cr.n_elec < 2e-7
+    # the fp64 finish leaves a count within the keep tolerance
+    n, _ = C.plan.run(C.mu, C.e_lo, want_density=False)
+    C.set_precision("fp64")
+    n64, _ = C.plan.run(C.mu, C.e_lo, want_density=False)
+    assert abs(2 * n64 - cr.n_elec) <= 1e-8 * cr.n_elec
