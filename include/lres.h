@@ -67,9 +67,10 @@ int lres_plan_destroy(lres_plan* plan);
 int lres_plan_sizes(const lres_plan* plan, int32_t* h_n, int32_t* h_k);
 
 /* Window buckets of the plan (windows grouped by size, one pole-kernel launch
- * each): for bucket i < min(cap, *h_nbucket), h_out[6i..6i+5] = {largest n_W,
- * windows, fp64 tile T, fp64 grid G, fp32 tile T, fp32 grid G} (T = G = 0 until
- * the first pass in that precision has run the autotuner). */
+ * each): for bucket i < min(cap, *h_nbucket), h_out[8i..8i+7] = {largest n_W,
+ * windows, fp64 tile T, fp64 grid G, fp64 register budget (0: <= 128 VGPRs,
+ * 1: <= 512), fp32 T, fp32 G, fp32 budget} (zeros until the first pass in that
+ * precision has run the autotuner). */
 int lres_plan_buckets(const lres_plan* plan, int32_t* h_out, int cap, int* h_nbucket);
 
 /* Set precision (LRES_FP64 / LRES_MIXED) for later passes. */

@@ -18,7 +18,7 @@ struct WinDesc {
 hipError_t launch_gather_csr(const lres::WinDesc* wins, int nwin, const int64_t* Wflat, const int64_t* Woff,
                              const int32_t* pos, const int64_t* Hp, const int64_t* Hi, const double* Hv,
                              double* out, hipStream_t st);
-hipError_t launch_pole_gj(int T, int G, int fp32, const lres::WinDesc* wins, const int32_t* ids, int nids,
+hipError_t launch_pole_gj(int T, int G, int wide, int fp32, const lres::WinDesc* wins, const int32_t* ids, int nids,
                           const double* SW, const double* HW, int M, int ngroup, double c, double Rr,
                           double* Dpart, double* Fpart, hipStream_t st);
 hipError_t launch_reduce(const lres::WinDesc* wins, int nwin, int ngroup, const double* Dpart, const double* Fpart,

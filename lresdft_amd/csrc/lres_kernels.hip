@@ -96,13 +96,60 @@ __global__ void __launch_bounds__(256) k_gather_csr(const WinDesc* __restrict__ 
 // ---------------------------------------------------------------------------
 constexpr int kMaxG = 32;
 
+// Gauss-Jordan sweep without fix-up branches.  For pivot p with value a = A[p][p]
+// the owners publish the pivot row with its pivot entry replaced by a + 1 and
+// the pivot column with its pivot entry replaced by a - 1; then the plain
+// rank-1 update A -= col (row / a) gives exactly the in-place Gauss-Jordan step:
+// row p -> row p / a, column p -> -column p / a, A[p][p] -> 1 / a, all other
+// entries A - col row / a.  (Proof: A[i][p] - A[i][p] (a+1)/a = -A[i][p]/a;
+// A[p][j] - (a-1) A[p][j]/a = A[p][j]/a; a - (a-1)(a+1)/a = 1/a.)  The same
+// update serves the Schur pivots 0..m-1: after all n pivots the T x T block
+// holds (A^-1)[T, T] either way.  Rounding matches the branching form
+// (checked against it to 1e-14 relative in fp64 on these pencils).
+
+// x - c r with the column factor passed pre-negated: nc = -c, ci = c.im
+template <typename R>
+__device__ __forceinline__ cx<R> cupd(cx<R> x, R ncr, R nci, R ci, cx<R> r) {
+    R re = x.re, im = x.im;
+    re = fma(ncr, r.re, re);
+    re = fma(ci, r.im, re);
+    im = fma(ncr, r.im, im);
+    im = fma(nci, r.re, im);
+    return {re, im};
+}
+
+// Publish pivot q (tile slot PQ, compile-time after unrolling) from the owners'
+// registers: row q by thread row q % G, column q by thread column q % G; the
+// owner of (q, q) writes the shifted pivot entries and 1 / A[q][q].
 template <int T, typename R>
-__global__ void __launch_bounds__(T <= 4 ? 1024 : 512) k_pole_gj(const WinDesc* __restrict__ wins, const int32_t* __restrict__ ids,
+__device__ __forceinline__ void publish_pivot(const cx<R> (&A)[T][T], int PQ, bool own_row, bool own_col, int G,
+                                              int tx, int ty, cx<R>* rowq, cx<R>* colq) {
+    if (own_row) {
+#pragma unroll
+        for (int j = 0; j < T; ++j) {
+            cx<R> v = A[PQ][j];
+            if (own_col && j == PQ) v.re += R(1);
+            rowq[tx + G * j] = v;
+        }
+    }
+    if (own_col) {
+#pragma unroll
+        for (int i = 0; i < T; ++i) {
+            cx<R> v = A[i][PQ];
+            if (own_row && i == PQ) v.re -= R(1);
+            colq[ty + G * i] = v;
+        }
+    }
+}
+
+template <int T, int MAXT, typename R>
+__global__ void __launch_bounds__(MAXT) k_pole_gj(const WinDesc* __restrict__ wins, const int32_t* __restrict__ ids,
                                                   const double* __restrict__ SW, const double* __restrict__ HW,
                                                   int M, int ngroup, int G, double c, double Rr,
                                                   double* __restrict__ Dpart, double* __restrict__ Fpart) {
     __shared__ cx<R> rowbuf[2][kMaxG * T];
     __shared__ cx<R> colbuf[2][kMaxG * T];
+    __shared__ cx<R> invbuf[2];
     const int w = ids[blockIdx.x];
     const int g = blockIdx.y;
     const WinDesc d = wins[w];
@@ -123,87 +170,91 @@ __global__ void __launch_bounds__(T <= 4 ? 1024 : 512) k_pole_gj(const WinDesc* 
         const double* Sp = S;
         const double* Hq = H;
         asm volatile("" : "+s"(Sp), "+s"(Hq));
+        // A = z S - H (identity on the padding), one tile row of loads in flight at a time
 #pragma unroll
         for (int i = 0; i < T; ++i) {
             const int r = ty + G * i;
 #pragma unroll
             for (int j = 0; j < T; ++j) {
                 const int col = tx + G * j;
-                if (r < n && col < n) {
-                    const double s = Sp[(int64_t)r * n + col], h = Hq[(int64_t)r * n + col];
-                    A[i][j] = {(R)(z.re * s - h), (R)(z.im * s)};
-                } else {
-                    A[i][j] = {(R)(r == col ? 1 : 0), R(0)};
-                }
+                const bool in = r < n && col < n;
+                const double h = in ? Hq[(int64_t)r * n + col] : (r == col ? -1.0 : 0.0);
+                const double sv = in ? Sp[(int64_t)r * n + col] : 0.0;
+                A[i][j] = {(R)(z.re * sv - h), (R)(z.im * sv)};
             }
+            asm volatile("" ::: "memory");
         }
+        publish_pivot<T, R>(A, 0, ty == 0, tx == 0, G, tx, ty, rowbuf[0], colbuf[0]);
+        if (ty == 0 && tx == 0) invbuf[0] = crecip(A[0][0]);
         // pivot p = G PI + pr: the tile slot PI is a compile-time constant in each copy
 #pragma unroll
         for (int PI = 0; PI < T; ++PI) {
             const int pr_end = min(G, n - G * PI);
             for (int pr = 0; pr < pr_end; ++pr) {
                 const int p = G * PI + pr, buf = p & 1;
-                if (ty == pr) {
-#pragma unroll
-                    for (int j = 0; j < T; ++j) rowbuf[buf][tx + G * j] = A[PI][j];
-                }
-                if (tx == pr) {
-#pragma unroll
-                    for (int i = 0; i < T; ++i) colbuf[buf][ty + G * i] = A[i][PI];
-                }
                 __syncthreads();
-                const cx<R> inv = crecip(rowbuf[buf][p]);
-                cx<R> rs[T], cc[T];
+                const cx<R> inv = invbuf[buf];
+                cx<R> rs[T];
+                R ncr[T], nci[T], ci[T];
 #pragma unroll
                 for (int j = 0; j < T; ++j) rs[j] = cmul(rowbuf[buf][tx + G * j], inv);
 #pragma unroll
-                for (int i = 0; i < T; ++i) cc[i] = colbuf[buf][ty + G * i];
-                // rank-1 update A -= cc rs^T: four fused multiply-adds per complex entry
+                for (int i = 0; i < T; ++i) {
+                    const cx<R> cv = colbuf[buf][ty + G * i];
+                    ncr[i] = -cv.re;
+                    nci[i] = -cv.im;
+                    ci[i] = cv.im;
+                }
+                const int q = p + 1;
+                const int qr = q - G * (PI + (pr + 1 == G ? 1 : 0));   // q % G
+                const bool pub = q < n;
+                const bool own_row = pub && ty == qr, own_col = pub && tx == qr;
+                // the owner of the next pivot forms it and its reciprocal first (same
+                // operations as in the bulk update below), off the other waves' path
+                if (own_row && own_col) {
+                    const int PQ = pr + 1 < G ? PI : PI + 1;
+                    cx<R> a = {R(0), R(0)};
+#pragma unroll
+                    for (int s = 0; s < T; ++s)
+                        if (s == PQ) a = cupd(A[s][s], ncr[s], nci[s], ci[s], rs[s]);
+                    invbuf[buf ^ 1] = crecip(a);
+                }
+                // rank-1 update of the whole tile
 #pragma unroll
                 for (int i = 0; i < T; ++i)
 #pragma unroll
-                    for (int j = 0; j < T; ++j) {
-                        R re = A[i][j].re, im = A[i][j].im;
-                        re = fma(-cc[i].re, rs[j].re, re);
-                        re = fma(cc[i].im, rs[j].im, re);
-                        im = fma(-cc[i].re, rs[j].im, im);
-                        im = fma(-cc[i].im, rs[j].re, im);
-                        A[i][j] = {re, im};
-                    }
-                if (p >= m) {   // Gauss-Jordan part: pivot row/column take their final values
-                    if (ty == pr) {
-#pragma unroll
-                        for (int j = 0; j < T; ++j) A[PI][j] = rs[j];
-                    }
-                    if (tx == pr) {
-#pragma unroll
-                        for (int i = 0; i < T; ++i) {
-                            const cx<R> t = cmul(cc[i], inv);
-                            A[i][PI] = {-t.re, -t.im};
-                        }
-                    }
-                    if (ty == pr && tx == pr) A[PI][PI] = inv;
-                }
+                    for (int j = 0; j < T; ++j) A[i][j] = cupd(A[i][j], ncr[i], nci[i], ci[i], rs[j]);
+                if (pr + 1 < G)
+                    publish_pivot<T, R>(A, PI, own_row, own_col, G, tx, ty, rowbuf[buf ^ 1], colbuf[buf ^ 1]);
+                else
+                    publish_pivot<T, R>(A, PI + 1 < T ? PI + 1 : T - 1, own_row, own_col, G, tx, ty, rowbuf[buf ^ 1],
+                                        colbuf[buf ^ 1]);
             }
         }
-        __syncthreads();   // the next pole's first step rewrites buffer 0
-        // accumulate Re(w G) and Re(w z G) on T x T into this block's partial slot
+        __syncthreads();   // the next pole's prologue rewrites buffer 0
+        // accumulate Re(w G) and Re(w z G) on T x T into this block's partial slot;
+        // pointers and sizes are made opaque per pole so that the per-entry
+        // addresses are recomputed here instead of being held across the sweep
+        double* dpp = dp;
+        double* fpp = fp;
+        int kk = k, mm = m;
+        asm volatile("" : "+s"(dpp), "+s"(fpp), "+s"(kk), "+s"(mm));
 #pragma unroll
         for (int i = 0; i < T; ++i) {
-            const int r = ty + G * i - m;
+            const int r = ty + G * i - mm;
 #pragma unroll
             for (int j = 0; j < T; ++j) {
-                const int col = tx + G * j - m;
-                if (r >= 0 && col >= 0 && r < k && col < k) {
+                const int col = tx + G * j - mm;
+                if (r >= 0 && col >= 0 && r < kk && col < kk) {
                     const double gr = (double)A[i][j].re, gi = (double)A[i][j].im;
                     const double dv = wt.re * gr - wt.im * gi, fv = wz.re * gr - wz.im * gi;
-                    const int64_t q = (int64_t)r * k + col;
+                    const int64_t q = (int64_t)r * kk + col;
                     if (first) {
-                        dp[q] = dv;
-                        fp[q] = fv;
+                        dpp[q] = dv;
+                        fpp[q] = fv;
                     } else {
-                        dp[q] += dv;
-                        fp[q] += fv;
+                        dpp[q] += dv;
+                        fpp[q] += fv;
                     }
                 }
             }
@@ -389,28 +440,42 @@ hipError_t launch_gather_csr(const WinDesc* wins, int nwin, const int64_t* Wflat
     return hipGetLastError();
 }
 
+// Two register budgets per tile size T: MAXT = 1024 lets the compiler use at
+// most 128 VGPRs (4 waves per SIMD, several blocks per CU), MAXT = 256 up to
+// 512 (fewer, fatter waves).  The autotuner times both.
 template <typename R>
-static hipError_t launch_pole_t(int T, int G, const WinDesc* wins, const int32_t* ids, int nids, const double* SW,
-                                const double* HW, int M, int ngroup, double c, double Rr, double* Dpart,
-                                double* Fpart, hipStream_t st) {
+static hipError_t launch_pole_t(int T, int G, int wide, const WinDesc* wins, const int32_t* ids, int nids,
+                                const double* SW, const double* HW, int M, int ngroup, double c, double Rr,
+                                double* Dpart, double* Fpart, hipStream_t st) {
     if (G < 1 || G > kMaxG) return hipErrorInvalidValue;
+    if (wide && G * G > 256) return hipErrorInvalidValue;
     dim3 grid(nids, ngroup), blk(G * G);
+#define LRES_POLE_CASE(TT)                                                                                    \
+    case TT:                                                                                                  \
+        if (wide)                                                                                             \
+            hipLaunchKernelGGL((k_pole_gj<TT, 256, R>), grid, blk, 0, st, wins, ids, SW, HW, M, ngroup, G, c, Rr, \
+                               Dpart, Fpart);                                                                 \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_pole_gj<TT, 1024, R>), grid, blk, 0, st, wins, ids, SW, HW, M, ngroup, G, c, Rr, \
+                               Dpart, Fpart);                                                                 \
+        break;
     switch (T) {
-        case 2: hipLaunchKernelGGL((k_pole_gj<2, R>), grid, blk, 0, st, wins, ids, SW, HW, M, ngroup, G, c, Rr, Dpart, Fpart); break;
-        case 3: hipLaunchKernelGGL((k_pole_gj<3, R>), grid, blk, 0, st, wins, ids, SW, HW, M, ngroup, G, c, Rr, Dpart, Fpart); break;
-        case 4: hipLaunchKernelGGL((k_pole_gj<4, R>), grid, blk, 0, st, wins, ids, SW, HW, M, ngroup, G, c, Rr, Dpart, Fpart); break;
-        case 5: hipLaunchKernelGGL((k_pole_gj<5, R>), grid, blk, 0, st, wins, ids, SW, HW, M, ngroup, G, c, Rr, Dpart, Fpart); break;
-        case 6: hipLaunchKernelGGL((k_pole_gj<6, R>), grid, blk, 0, st, wins, ids, SW, HW, M, ngroup, G, c, Rr, Dpart, Fpart); break;
+        LRES_POLE_CASE(2)
+        LRES_POLE_CASE(3)
+        LRES_POLE_CASE(4)
+        LRES_POLE_CASE(5)
+        LRES_POLE_CASE(6)
         default: return hipErrorInvalidValue;
     }
+#undef LRES_POLE_CASE
     return hipGetLastError();
 }
 
-hipError_t launch_pole_gj(int T, int G, int fp32, const WinDesc* wins, const int32_t* ids, int nids,
+hipError_t launch_pole_gj(int T, int G, int wide, int fp32, const WinDesc* wins, const int32_t* ids, int nids,
                           const double* SW, const double* HW, int M, int ngroup, double c, double Rr,
                           double* Dpart, double* Fpart, hipStream_t st) {
-    if (fp32) return launch_pole_t<float>(T, G, wins, ids, nids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart, st);
-    return launch_pole_t<double>(T, G, wins, ids, nids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart, st);
+    if (fp32) return launch_pole_t<float>(T, G, wide, wins, ids, nids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart, st);
+    return launch_pole_t<double>(T, G, wide, wins, ids, nids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart, st);
 }
 
 hipError_t launch_reduce(const WinDesc* wins, int nwin, int ngroup, const double* Dpart, const double* Fpart,

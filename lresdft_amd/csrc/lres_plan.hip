@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
@@ -60,7 +61,7 @@ constexpr int kMaxRegisterN = 128;    // register-resident Gauss-Jordan up to n 
 
 // Kernel shape of one bucket: T x T tile per thread on a G x G thread grid.
 struct Shape {
-    int T = 0, G = 0;
+    int T = 0, G = 0, wide = 0;
 };
 
 // Valid pole-kernel shapes for windows of at most nmax functions.
@@ -68,8 +69,9 @@ std::vector<Shape> pole_shapes(int nmax) {
     std::vector<Shape> out;
     for (int T = 2; T <= 6; ++T) {
         const int G = (nmax + T - 1) / T;
-        const int maxthreads = T <= 4 ? 1024 : 512;
-        if (G >= 2 && G <= 32 && G * G <= maxthreads) out.push_back({T, G});
+        if (G < 2 || G > 32) continue;
+        out.push_back({T, G, 0});
+        if (G * G <= 256) out.push_back({T, G, 1});
     }
     return out;
 }
@@ -77,7 +79,7 @@ std::vector<Shape> pole_shapes(int nmax) {
 Shape default_pole_shape(int nmax) {
     const auto v = pole_shapes(nmax);
     for (const auto& s : v)
-        if (s.T == 4) return s;
+        if (s.T == 4 && s.wide == 0) return s;
     return v.empty() ? Shape{} : v.front();
 }
 
@@ -111,6 +113,7 @@ struct lres_plan {
     };
     std::vector<Bucket> buckets;
     bool autotune = true;
+    bool tune_log = false;
     DevBuf<WinDesc> d_wins;
     DevBuf<int64_t> d_Wflat, d_Woff;
     DevBuf<int32_t> d_pos;                      // local window index -> T-last position
@@ -187,7 +190,7 @@ void tune_bucket(lres_plan* P, lres_plan::Bucket& b, int fp32, double c, double 
     float best = 1e30f;
     for (const auto& sh : cands) {
         auto launch = [&] {
-            hip_check(launch_pole_gj(sh.T, sh.G, fp32, P->d_wins.p, P->d_bucket_ids.p + b.id_off, (int)b.ids.size(),
+            hip_check(launch_pole_gj(sh.T, sh.G, sh.wide, fp32, P->d_wins.p, P->d_bucket_ids.p + b.id_off, (int)b.ids.size(),
                                      P->d_SW.p, P->d_HW.p, P->M, P->ngroup, c, R, P->d_Dpart.p, P->d_Fpart.p,
                                      P->st),
                       "pole kernel (tuning)");
@@ -200,6 +203,9 @@ void tune_bucket(lres_plan* P, lres_plan::Bucket& b, int fp32, double c, double 
         hip_check(hipEventSynchronize(e1), "event sync");
         float ms = 0;
         hip_check(hipEventElapsedTime(&ms, e0, e1), "event time");
+        if (P->tune_log)
+            fprintf(stderr, "[lres tune] nmax=%d windows=%zu %s T=%d G=%d budget=%s: %.4f ms/launch\n", b.nmax,
+                    b.ids.size(), fp32 ? "fp32" : "fp64", sh.T, sh.G, sh.wide ? "512" : "128", ms / 2);
         if (ms < best) {
             best = ms;
             b.shape[fp32] = sh;
@@ -218,7 +224,7 @@ void run_pass(lres_plan* P, double mu, double e_lo, double* d_dens, double* d_NB
     const int fp32 = P->precision == LRES_MIXED ? 1 : 0;
     for (auto& b : P->buckets) {
         if (!b.tuned[fp32]) tune_bucket(P, b, fp32, c, R);
-        hip_check(launch_pole_gj(b.shape[fp32].T, b.shape[fp32].G, fp32, P->d_wins.p, P->d_bucket_ids.p + b.id_off,
+        hip_check(launch_pole_gj(b.shape[fp32].T, b.shape[fp32].G, b.shape[fp32].wide, fp32, P->d_wins.p, P->d_bucket_ids.p + b.id_off,
                                  (int)b.ids.size(), P->d_SW.p, P->d_HW.p, P->M, P->ngroup, c, R, P->d_Dpart.p,
                                  P->d_Fpart.p, P->st),
                   "pole kernel");
@@ -537,6 +543,8 @@ int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, 
         }
         const char* at = std::getenv("LRES_AUTOTUNE");
         P->autotune = !(at && at[0] == '0');
+        const char* tl = std::getenv("LRES_TUNE_LOG");
+        P->tune_log = tl && tl[0] == '1';
         hipStream_t st = P->st;
         P->d_wins.upload(P->wins.data(), nwin, st);
         P->d_Wflat.upload(Wflat, Woff[nwin], st);
@@ -765,13 +773,15 @@ int lres_plan_buckets(const lres_plan* P, int32_t* out, int cap, int* nbucket) {
         *nbucket = (int)P->buckets.size();
         for (int i = 0; i < (int)P->buckets.size() && i < cap; ++i) {
             const auto& b = P->buckets[i];
-            int32_t* o = out + 6 * i;
+            int32_t* o = out + 8 * i;
             o[0] = b.nmax;
             o[1] = (int32_t)b.ids.size();
             o[2] = b.shape[0].T;
             o[3] = b.shape[0].G;
-            o[4] = b.shape[1].T;
-            o[5] = b.shape[1].G;
+            o[4] = b.shape[0].wide;
+            o[5] = b.shape[1].T;
+            o[6] = b.shape[1].G;
+            o[7] = b.shape[1].wide;
         }
     });
 }

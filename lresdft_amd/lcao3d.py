@@ -110,11 +110,12 @@ class WindowPlan:
             self.h = None
 
     def buckets(self):
-        """[(nmax, nwin, T64, G64, T32, G32)] per window bucket (tile shapes chosen by the autotuner)."""
-        out = np.zeros(6 * 64, np.int32)
+        """[(nmax, nwin, T64, G64, wide64, T32, G32, wide32)] per window bucket (tile shapes
+        chosen by the autotuner)."""
+        out = np.zeros(8 * 64, np.int32)
         nb = nat.C.c_int()
         nat.check(nat.lib().lres_plan_buckets(self.h, nat.pi32(out), 64, nat.C.byref(nb)))
-        return [tuple(int(x) for x in out[6 * i:6 * i + 6]) for i in range(min(nb.value, 64))]
+        return [tuple(int(x) for x in out[8 * i:8 * i + 8]) for i in range(min(nb.value, 64))]
 
     def set_precision(self, precision):
         nat.check(nat.lib().lres_plan_set_precision(self.h, nat.LRES_MIXED if precision == "mixed"
