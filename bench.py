@@ -46,6 +46,7 @@ def run_scf(C, cr, grid, vion, precision, tol):
     from lresdft_amd.scf import scf_lcao
     C.set_precision(precision)
     C.mu = None
+    C.history = []
     p0 = C.passes
     torch.cuda.synchronize()
     t = time.perf_counter()
@@ -53,6 +54,7 @@ def run_scf(C, cr, grid, vion, precision, tol):
     torch.cuda.synchronize()
     r["wall"] = time.perf_counter() - t
     r["passes"] = C.passes - p0
+    r["history"] = list(C.history)
     return r
 
 
@@ -162,6 +164,7 @@ def main():
         "energy_fp64": fp64[-1]["energy"], "energy_mixed": mixed[-1]["energy"],
         "iterations": {"fp64": fp64[-1]["iterations"], "mixed": mixed[-1]["iterations"]},
         "passes_per_scf": {"fp64": fp64[-1]["passes"], "mixed": mixed[-1]["passes"]},
+        "history": {"fp64": fp64[-1]["history"], "mixed": mixed[-1]["history"]},
         "roofline": roofline(st32["fp32"], flops, PEAK_FP32) or roofline(st32["fp64"], flops, PEAK_FP64),
         "roofline_fp64": roofline(st64["fp64"], flops, PEAK_FP64),
         "cpu_baseline": None,

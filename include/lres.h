@@ -127,9 +127,12 @@ int lres_lambda_min(lres_plan* plan, double tol, double* h_lambda_min);
  * counts already known at lo / hi (NaN: evaluated here).  Stops when the
  * bracket is below xtol (brentq's criterion) or |count| <= ftol.  Replaces
  * _solve_mu (lcao3d.py:387-398), which the eigen route evaluates in closed
- * form.  h_info (may be NULL) receives {passes, count at the returned mu}. */
+ * form.  Each trial pass also writes its density to d_dens (may be NULL).
+ * h_info (4 doubles, may be NULL) receives {passes, count at the returned mu,
+ * 1 if the returned mu was the last pass (d_dens and sum E_B then belong to
+ * it), sum_B E_B of that pass}. */
 int lres_mu_solve(lres_plan* plan, double N, double e_lo, double lo, double hi, double f_lo, double f_hi,
-                  double xtol, double ftol, double* h_mu, double* h_info);
+                  double xtol, double ftol, double* d_dens, double* h_mu, double* h_info);
 
 /* One-shot form with lrft_windows' argument list (src/lrft_cpp.cpp:129-132):
  * host CSR H, S, X, window tables, nb, dV, mu, e_lo, M; outputs h_dens (ng, may be

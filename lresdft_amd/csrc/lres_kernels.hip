@@ -314,14 +314,24 @@ __global__ void __launch_bounds__(256) k_reduce(const WinDesc* __restrict__ wins
 }
 
 // n(g) = x_g^T D_B x_g on the owned points (src/lrft_cpp.cpp:183-191 in D form).
-// D_B is staged in LDS when it fits (stage != 0, dynamic shared size k*k doubles).
-__global__ void __launch_bounds__(256) k_density(const WinDesc* __restrict__ wins, const double* __restrict__ D,
-                                                 const int64_t* __restrict__ own_g,
-                                                 const int64_t* __restrict__ xptr,
-                                                 const int32_t* __restrict__ xidx,
-                                                 const double* __restrict__ xval, double* __restrict__ dens,
-                                                 int stage) {
+// One wave per owned point, lanes over the nnz^2 entry pairs of the point's row
+// of X (restricted to T); the row is staged in a per-wave LDS slot and D_B in
+// LDS when it fits (stage != 0, dynamic shared size k*k doubles).  Pair q maps
+// to (q / nnz, q % nnz) by a multiply-shift that is exact for nnz <= 128 and
+// q < nnz^2.  Fixed-shape wave reduction: bitwise reproducible.
+constexpr int kDensWaves = 8;
+constexpr int kDensStage = 64;
+
+__global__ void __launch_bounds__(64 * kDensWaves) k_density(const WinDesc* __restrict__ wins,
+                                                             const double* __restrict__ D,
+                                                             const int64_t* __restrict__ own_g,
+                                                             const int64_t* __restrict__ xptr,
+                                                             const int32_t* __restrict__ xidx,
+                                                             const double* __restrict__ xval,
+                                                             double* __restrict__ dens, int stage) {
     extern __shared__ double Dl[];
+    __shared__ int32_t sidx[kDensWaves][kDensStage];
+    __shared__ double sval[kDensWaves][kDensStage];
     const int w = blockIdx.x;
     const WinDesc d = wins[w];
     const int k = d.k;
@@ -329,19 +339,40 @@ __global__ void __launch_bounds__(256) k_density(const WinDesc* __restrict__ win
     if (stage) {
         for (int q = threadIdx.x; q < k * k; q += blockDim.x) Dl[q] = Dw[q];
         __syncthreads();
-        Dw = Dl;
     }
-    for (int r = threadIdx.x; r < d.nown; r += blockDim.x) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    for (int r = wave; r < d.nown; r += nwave) {
         const int64_t o = d.own_off + r;
-        const int64_t p0 = xptr[o], p1 = xptr[o + 1];
-        double s = 0.0;
-        for (int64_t a = p0; a < p1; ++a) {
-            const double* row = Dw + (int64_t)xidx[a] * k;
-            double t = 0.0;
-            for (int64_t b = p0; b < p1; ++b) t += row[xidx[b]] * xval[b];
-            s += xval[a] * t;
+        const int64_t p0 = xptr[o];
+        const int nnz = (int)(xptr[o + 1] - p0);
+        double acc = 0.0;
+        if (nnz <= kDensStage && nnz > 0) {
+            if (lane < nnz) {
+                sidx[wave][lane] = xidx[p0 + lane];
+                sval[wave][lane] = xval[p0 + lane];
+            }
+            // LDS operations of one wave execute in issue order; the wave barrier keeps
+            // the compiler from moving the reads above the writes
+            __builtin_amdgcn_wave_barrier();
+            const uint32_t magic = ((1u << 22) + (uint32_t)nnz - 1u) / (uint32_t)nnz;
+            const int npair = nnz * nnz;
+            for (int q = lane; q < npair; q += 64) {
+                const int ai = (int)(((uint32_t)q * magic) >> 22);
+                const int bi = q - ai * nnz;
+                const int ia = sidx[wave][ai], ib = sidx[wave][bi];
+                const double dv = stage ? Dl[ia * k + ib] : Dw[(int64_t)ia * k + ib];
+                acc = fma(sval[wave][ai] * dv, sval[wave][bi], acc);
+            }
+            __builtin_amdgcn_wave_barrier();     // the slot is rewritten for the next point
+        } else {
+            for (int q = lane; q < nnz * nnz; q += 64) {
+                const int ai = q / nnz, bi = q - (q / nnz) * nnz;
+                const int ia = xidx[p0 + ai], ib = xidx[p0 + bi];
+                acc = fma(xval[p0 + ai] * Dw[(int64_t)ia * k + ib], xval[p0 + bi], acc);
+            }
         }
-        dens[own_g[o]] = s;
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+        if (lane == 0) dens[own_g[o]] = acc;
     }
 }
 
@@ -488,9 +519,10 @@ hipError_t launch_density(const WinDesc* wins, int nwin, int kmax, const double*
                           const int64_t* xptr, const int32_t* xidx, const double* xval, double* dens,
                           hipStream_t st) {
     const size_t lds = (size_t)kmax * kmax * sizeof(double);
-    const int stage = lds <= 64 * 1024 ? 1 : 0;   // default dynamic LDS limit per block
-    hipLaunchKernelGGL(k_density, dim3(nwin), dim3(256), stage ? lds : 0, st, wins, D, own_g, xptr, xidx, xval, dens,
-                       stage);
+    // static staging uses 6 KB; the dynamic D block keeps the total within the 64 KB default
+    const int stage = lds <= 56 * 1024 ? 1 : 0;
+    hipLaunchKernelGGL(k_density, dim3(nwin), dim3(64 * kDensWaves), stage ? lds : 0, st, wins, D, own_g, xptr, xidx,
+                       xval, dens, stage);
     return hipGetLastError();
 }
 

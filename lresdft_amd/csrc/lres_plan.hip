@@ -876,17 +876,18 @@ int lres_lambda_min(lres_plan* P, double tol, double* out) {
 }
 
 int lres_mu_solve(lres_plan* P, double N, double e_lo, double lo, double hi, double f_lo, double f_hi,
-                  double xtol, double ftol, double* h_mu, double* h_info) {
+                  double xtol, double ftol, double* d_dens, double* h_mu, double* h_info) {
     return guarded([&] {
         require(P != nullptr, "null plan");
         require(lo < hi, "mu bracket must satisfy lo < hi");
         int evals = 0;
         double sums[2] = {0.0, 0.0};
-        double last_mu = NAN, last_f = NAN;
+        double last_mu = NAN, last_f = NAN, last_e = NAN;
         auto f = [&](double mu) {
-            run_pass(P, mu, e_lo, nullptr, nullptr, nullptr, sums);
+            run_pass(P, mu, e_lo, d_dens, nullptr, nullptr, sums);
             last_mu = mu;
             last_f = 2.0 * sums[0] - N;
+            last_e = sums[1];
             return last_f;
         };
         double flo = f_lo, fhi = f_hi;
@@ -895,8 +896,11 @@ int lres_mu_solve(lres_plan* P, double N, double e_lo, double lo, double hi, dou
         const double mu = brent_root(f, lo, hi, flo, fhi, xtol, ftol, 200, &evals);
         *h_mu = mu;
         if (h_info) {
+            const bool last = mu == last_mu;
             h_info[0] = evals;
-            h_info[1] = mu == last_mu ? last_f : (mu == lo ? flo : (mu == hi ? fhi : NAN));
+            h_info[1] = last ? last_f : (mu == lo ? flo : (mu == hi ? fhi : NAN));
+            h_info[2] = last ? 1.0 : 0.0;
+            h_info[3] = last ? last_e : NAN;
         }
     });
 }

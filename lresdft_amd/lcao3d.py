@@ -96,6 +96,7 @@ class WindowPlan:
         self.k = np.zeros(self.nwin, np.int32)
         nat.check(L.lres_plan_sizes(h, nat.pi32(self.n), nat.pi32(self.k)))
         self.dens = torch.zeros(self.ng, dtype=torch.float64, device="cuda")
+        self.last = None     # (mu, e_lo, sum N_B, sum E_B) of the pass whose density self.dens holds
         self.NB = torch.zeros(self.nwin, dtype=torch.float64, device="cuda")
         self.EB = torch.zeros(self.nwin, dtype=torch.float64, device="cuda")
         self.passes = 0
@@ -118,10 +119,12 @@ class WindowPlan:
         return [tuple(int(x) for x in out[8 * i:8 * i + 8]) for i in range(min(nb.value, 64))]
 
     def set_precision(self, precision):
+        self.last = None
         nat.check(nat.lib().lres_plan_set_precision(self.h, nat.LRES_MIXED if precision == "mixed"
                                                     else nat.LRES_FP64))
 
     def set_h(self, Hc):
+        self.last = None
         nat.check(nat.lib().lres_set_hamiltonian_csr(self.h, nat.p64(Hc[0]), nat.p64(Hc[1]), nat.pf(Hc[2]),
                                                      int(Hc[0][-1])))
 
@@ -142,18 +145,22 @@ class WindowPlan:
         if v_dev.dtype != self.torch.float64 or not v_dev.is_contiguous() or v_dev.device.type != "cuda":
             raise ValueError("potential must be a contiguous float64 CUDA tensor")
         self._v_keep = v_dev
+        self.last = None
         nat.check(nat.lib().lres_set_potential(self.h, v_dev.data_ptr()))
 
     def set_h_device(self, Hp, Hi, Hv):
+        self.last = None
         nat.check(nat.lib().lres_set_hamiltonian_csr_device(self.h, Hp.data_ptr(), Hi.data_ptr(), Hv.data_ptr()))
 
     def run(self, mu, e_lo, want_density=True):
-        """One pass; returns (sum_B N_B, sum_B E_B) (no spin factor)."""
+        """One pass; returns (sum_B N_B, sum_B E_B) (no spin factor).  With the
+        density, self.last records (mu, e_lo, sums) of the pass self.dens holds."""
         sums = np.zeros(2)
         self.passes += 1
         nat.check(nat.lib().lres_window_pass(self.h, float(mu), float(e_lo),
                                              self.dens.data_ptr() if want_density else None,
                                              self.NB.data_ptr(), self.EB.data_ptr(), nat.pf(sums)))
+        self.last = (float(mu), float(e_lo), float(sums[0]), float(sums[1])) if want_density else None
         return float(sums[0]), float(sums[1])
 
     def inertia(self, sigmas):
@@ -168,12 +175,15 @@ class WindowPlan:
         return float(out[0])
 
     def mu_solve(self, N, e_lo, lo, hi, f_lo=math.nan, f_hi=math.nan, xtol=1e-12, ftol=0.0):
-        """Brent root of the count on [lo, hi]; returns (mu, [passes, count at mu])."""
+        """Brent root of the count on [lo, hi]; every trial pass writes self.dens.
+        Returns (mu, [passes, count at mu, 1 if mu was the last pass, sum E_B there])."""
         mu = np.zeros(1)
-        info = np.zeros(2)
+        info = np.zeros(4)
         nat.check(nat.lib().lres_mu_solve(self.h, float(N), float(e_lo), float(lo), float(hi), float(f_lo),
-                                          float(f_hi), float(xtol), float(ftol), nat.pf(mu), nat.pf(info)))
+                                          float(f_hi), float(xtol), float(ftol), self.dens.data_ptr(), nat.pf(mu),
+                                          nat.pf(info)))
         self.passes += int(info[0])
+        self.last = None
         return float(mu[0]), info
 
     def stats(self, reset=False):
@@ -248,7 +258,9 @@ class LRFTLCAOCpp:
         return (1e-11 if self.precision == "fp64" else 1e-7) * N
 
     def _count(self, mu, e_lo, N):
-        n, _ = self.plan.run(mu, e_lo, want_density=False)
+        # trial passes also form the density (a small part of a pass), so that
+        # the pass at the accepted mu need not be repeated
+        n, _ = self.plan.run(mu, e_lo, want_density=True)
         return 2.0 * n - N
 
     def _note_slope(self, mu0, f0, mu1, f1):
@@ -257,6 +269,8 @@ class LRFTLCAOCpp:
 
     def _brent(self, N, e_lo, lo, hi, f_lo, f_hi):
         mu, info = self.plan.mu_solve(N, e_lo, lo, hi, f_lo, f_hi, self.mu_xtol, self._ftol(N))
+        if info[2] == 1.0:      # the density buffer holds the pass at the returned mu
+            self.plan.last = (mu, float(e_lo), 0.5 * (info[1] + N), float(info[3]))
         return mu, info[1]
 
     def _solve_mu_cold(self, lam_min, e_lo, N):
@@ -334,7 +348,11 @@ class LRFTLCAOCpp:
             return self._density(v, N)
 
     def _finish(self, mu, e_lo):
-        n, e = self.plan.run(mu, e_lo, want_density=True)
+        last = self.plan.last
+        if last is not None and last[0] == mu and last[1] == e_lo:
+            n, e = last[2], last[3]
+        else:
+            n, e = self.plan.run(mu, e_lo, want_density=True)
         return 2.0 * self.plan.dens, 2.0 * e, n
 
     def _density(self, v, N=None):
