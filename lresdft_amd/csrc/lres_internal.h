@@ -38,3 +38,12 @@ hipError_t launch_window_hv(const lres::WinDesc* wins, int nwin, const int64_t* 
                             const int32_t* lf, const int64_t* wat_off, const int32_t* wat_g, const int64_t* wpt_off,
                             const int32_t* pairtab, const int32_t* nf_of, const int64_t* vp_off, const double* vpair,
                             const double* TW, double* HW, hipStream_t st);
+// Windows above the register-resident limit (blocked sweep in a global workspace).
+size_t big_lds_bytes(int nmax, int b, int elem_bytes);
+hipError_t launch_pole_big(int fp32, int bmax, int nmax, const lres::WinDesc* wins, const int32_t* ids, int nids,
+                           const double* SW, const double* HW, int M, int ngroup, double c, double Rr, void* work,
+                           const int64_t* ws_off, double* Dpart, double* Fpart, hipStream_t st);
+hipError_t launch_inertia_big(int bmax, int nmax, const lres::WinDesc* wins, const int32_t* ids, int nids,
+                              const double* SW, const double* HW, const double* sigma, int nshift, void* work,
+                              const int64_t* ws_off, int ngroup, int32_t* counts, unsigned long long* total,
+                              hipStream_t st);

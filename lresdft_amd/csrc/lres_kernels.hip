@@ -21,6 +21,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "lres_internal.h"
 
 namespace lres {
@@ -662,5 +664,220 @@ hipError_t launch_window_hv(const WinDesc* wins, int nwin, const int64_t* Woff, 
                             const double* TW, double* HW, hipStream_t st) {
     hipLaunchKernelGGL(k_window_hv, dim3(nwin), dim3(256), 0, st, wins, Woff, lat, lf, wat_off, wat_g, wpt_off,
                        pairtab, nf_of, vp_off, vpair, TW, HW);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Windows larger than the register-resident sweep: blocked Gauss-Jordan with the
+// matrix in a global workspace (one slot per block, reused across its poles).
+// Block step with pivot block P = [p0, p0 + b): column panel C = A[:, P] and row
+// panel R = A[P, :] go to LDS; Pi = A_PP^-1 is formed by one wave (scalar
+// shifted-pivot sweep); then with C~ = C - [I on P] and R~ = Pi (R + [I on P])
+// the uniform rank-b update A -= C~ R~ is the blocked Gauss-Jordan step (row
+// block -> Pi R, column block -> -C Pi, pivot block -> Pi, rest A - C Pi R).
+// ---------------------------------------------------------------------------
+template <typename E> struct Fld;
+template <> struct Fld<double> {
+    __device__ static double zero() { return 0.0; }
+    __device__ static double one() { return 1.0; }
+    __device__ static double add_one(double a, double s) { return a + s; }
+    __device__ static double mul(double a, double b) { return a * b; }
+    __device__ static double fms(double acc, double a, double b) { return fma(-a, b, acc); }
+    __device__ static double fma_(double acc, double a, double b) { return fma(a, b, acc); }
+    __device__ static double recip(double a) { return 1.0 / a; }
+    __device__ static bool positive(double a) { return a > 0.0; }
+};
+template <typename R> struct Fld<cx<R>> {
+    __device__ static cx<R> zero() { return {R(0), R(0)}; }
+    __device__ static cx<R> one() { return {R(1), R(0)}; }
+    __device__ static cx<R> add_one(cx<R> a, R s) { return {a.re + s, a.im}; }
+    __device__ static cx<R> mul(cx<R> a, cx<R> b) { return cmul(a, b); }
+    __device__ static cx<R> fms(cx<R> acc, cx<R> a, cx<R> b) { return cupd(acc, -a.re, -a.im, a.im, b); }
+    __device__ static cx<R> fma_(cx<R> acc, cx<R> a, cx<R> b) {
+        return {fma(a.re, b.re, fma(-a.im, b.im, acc.re)), fma(a.re, b.im, fma(a.im, b.re, acc.im))};
+    }
+    __device__ static cx<R> recip(cx<R> a) { return crecip(a); }
+    __device__ static bool positive(cx<R> a) { return a.re > R(0); }
+};
+
+// One blocked Gauss-Jordan step on the n x n matrix A (global, row-major).
+// LDS: Cl (n x b), Rl (b x n), Pl (b x b).  Returns (in thread 0) the number of
+// positive scalar pivots met inside the pivot block (used by the inertia count).
+template <typename E>
+__device__ int big_block_step(E* __restrict__ A, int n, int p0, int b, E* Cl, E* Rl, E* Pl) {
+    using Fo = Fld<E>;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int idx = tid; idx < n * b; idx += nt) {
+        const int i = idx / b, q = idx - (idx / b) * b;
+        Cl[idx] = A[(int64_t)i * n + p0 + q];
+    }
+    for (int idx = tid; idx < b * n; idx += nt) Rl[idx] = A[(int64_t)p0 * n + idx];
+    __syncthreads();
+    int npos = 0;
+    if (tid < 64) {   // wave 0: Pi = A_PP^-1 by the scalar shifted-pivot sweep
+        for (int e = tid; e < b * b; e += 64) Pl[e] = Rl[(e / b) * n + p0 + (e - (e / b) * b)];
+        __builtin_amdgcn_wave_barrier();
+        for (int p = 0; p < b; ++p) {
+            const E piv = Pl[p * b + p];
+            npos += Fo::positive(piv) ? 1 : 0;
+            const E inv = Fo::recip(piv);
+            E colv[4], rowv[4];
+            int cnt = 0;
+            for (int e = tid; e < b * b; e += 64, ++cnt) {
+                const int i = e / b, j = e - (e / b) * b;
+                colv[cnt] = (i == p) ? Fo::add_one(piv, -1) : Pl[i * b + p];
+                rowv[cnt] = Fo::mul((j == p) ? Fo::add_one(piv, 1) : Pl[p * b + j], inv);
+            }
+            __builtin_amdgcn_wave_barrier();
+            cnt = 0;
+            for (int e = tid; e < b * b; e += 64, ++cnt) Pl[e] = Fo::fms(Pl[e], colv[cnt], rowv[cnt]);
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    __syncthreads();
+    // R~ = Pi (R + [I on P]) in place, one column per thread; C~ = C - [I on P]
+    for (int j = tid; j < n; j += nt) {
+        E v[32];
+        for (int q = 0; q < b; ++q) v[q] = (j - p0 == q) ? Fo::add_one(Rl[q * n + j], 1) : Rl[q * n + j];
+        for (int q = 0; q < b; ++q) {
+            E acc = Fo::zero();
+            for (int r = 0; r < b; ++r) acc = Fo::fma_(acc, Pl[q * b + r], v[r]);
+            Rl[q * n + j] = acc;
+        }
+    }
+    for (int q = tid; q < b; q += nt) Cl[(p0 + q) * b + q] = Fo::add_one(Cl[(p0 + q) * b + q], -1);
+    __syncthreads();
+    // A -= C~ R~ : each thread four rows of one column per round (row-major, coalesced in j)
+    const int rows4 = (n + 3) / 4;
+    for (int idx = tid; idx < rows4 * n; idx += nt) {
+        const int i0 = (idx / n) * 4, j = idx - (idx / n) * n;
+        E a[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) a[s] = (i0 + s < n) ? A[(int64_t)(i0 + s) * n + j] : Fo::zero();
+        for (int q = 0; q < b; ++q) {
+            const E r = Rl[q * n + j];
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+                if (i0 + s < n) a[s] = Fo::fms(a[s], Cl[(i0 + s) * b + q], r);
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+            if (i0 + s < n) A[(int64_t)(i0 + s) * n + j] = a[s];
+    }
+    __syncthreads();
+    return npos;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_pole_big(const WinDesc* __restrict__ wins, const int32_t* __restrict__ ids,
+                                                  const double* __restrict__ SW, const double* __restrict__ HW,
+                                                  int M, int ngroup, double c, double Rr, int bmax,
+                                                  cx<double>* __restrict__ work, const int64_t* __restrict__ ws_off,
+                                                  double* __restrict__ Dpart, double* __restrict__ Fpart) {
+    extern __shared__ double lds_raw[];
+    using E = cx<R>;
+    const int w = ids[blockIdx.x];
+    const int g = blockIdx.y;
+    const WinDesc d = wins[w];
+    const int n = d.n, m = d.m, k = d.k;
+    E* A = reinterpret_cast<E*>(work + ws_off[w] + (int64_t)g * n * n);
+    E* Cl = reinterpret_cast<E*>(lds_raw);
+    E* Rl = Cl + (size_t)n * bmax;
+    E* Pl = Rl + (size_t)n * bmax;
+    const double* S = SW + d.mat_off;
+    const double* H = HW + d.mat_off;
+    double* dp = Dpart + d.kk_off * ngroup + (int64_t)g * k * k;
+    double* fp = Fpart + d.kk_off * ngroup + (int64_t)g * k * k;
+    bool first = true;
+    for (int jp = g; jp < M / 2; jp += ngroup) {
+        cx<double> z, wt, wz;
+        pole(jp, M, c, Rr, z, wt, wz);
+        for (int64_t q = threadIdx.x; q < (int64_t)n * n; q += blockDim.x)
+            A[q] = {(R)(z.re * S[q] - H[q]), (R)(z.im * S[q])};
+        __syncthreads();
+        for (int p0 = 0; p0 < n; p0 += bmax) big_block_step<E>(A, n, p0, min(bmax, n - p0), Cl, Rl, Pl);
+        for (int q = threadIdx.x; q < k * k; q += blockDim.x) {
+            const int r = q / k, col = q - (q / k) * k;
+            const E a = A[(int64_t)(r + m) * n + col + m];
+            const double gr = (double)a.re, gi = (double)a.im;
+            const double dv = wt.re * gr - wt.im * gi, fv = wz.re * gr - wz.im * gi;
+            if (first) {
+                dp[q] = dv;
+                fp[q] = fv;
+            } else {
+                dp[q] += dv;
+                fp[q] += fv;
+            }
+        }
+        __syncthreads();
+        first = false;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_inertia_big(const WinDesc* __restrict__ wins, const int32_t* __restrict__ ids,
+                                                     const double* __restrict__ SW, const double* __restrict__ HW,
+                                                     const double* __restrict__ sigma, int nshift, int bmax,
+                                                     cx<double>* __restrict__ work, const int64_t* __restrict__ ws_off,
+                                                     int ngroup, int32_t* __restrict__ counts,
+                                                     unsigned long long* __restrict__ total) {
+    extern __shared__ double lds_raw[];
+    const int w = ids[blockIdx.x];
+    const int slot = blockIdx.y % ngroup;   // workspace slots are shared with the pole kernel
+    const WinDesc d = wins[w];
+    const int n = d.n;
+    double* A = reinterpret_cast<double*>(work + ws_off[w] + (int64_t)slot * n * n);
+    double* Cl = lds_raw;
+    double* Rl = Cl + (size_t)n * bmax;
+    double* Pl = Rl + (size_t)n * bmax;
+    const double* S = SW + d.mat_off;
+    const double* H = HW + d.mat_off;
+    for (int s_id = blockIdx.y; s_id < nshift; s_id += ngroup) {
+        const double sg = sigma[s_id];
+        for (int64_t q = threadIdx.x; q < (int64_t)n * n; q += blockDim.x) A[q] = sg * S[q] - H[q];
+        __syncthreads();
+        int npos = 0;
+        for (int p0 = 0; p0 < n; p0 += bmax) npos += big_block_step<double>(A, n, p0, min(bmax, n - p0), Cl, Rl, Pl);
+        if (threadIdx.x == 0) {
+            if (counts) counts[(int64_t)w * nshift + s_id] = npos;
+            atomicAdd(&total[s_id], (unsigned long long)npos);
+        }
+        __syncthreads();
+    }
+}
+
+// dynamic LDS of the big kernels: two panels of n x b and the b x b pivot block
+size_t big_lds_bytes(int nmax, int b, int elem_bytes) {
+    return ((size_t)2 * nmax * b + (size_t)b * b) * elem_bytes;
+}
+
+hipError_t launch_pole_big(int fp32, int bmax, int nmax, const WinDesc* wins, const int32_t* ids, int nids,
+                           const double* SW, const double* HW, int M, int ngroup, double c, double Rr, void* work,
+                           const int64_t* ws_off, double* Dpart, double* Fpart, hipStream_t st) {
+    const size_t lds = big_lds_bytes(nmax, bmax, fp32 ? 8 : 16);
+    dim3 grid(nids, ngroup), blk(256);
+    hipError_t e;
+    if (fp32) {
+        e = hipFuncSetAttribute((const void*)k_pole_big<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_pole_big<float>, grid, blk, lds, st, wins, ids, SW, HW, M, ngroup, c, Rr, bmax,
+                           (cx<double>*)work, ws_off, Dpart, Fpart);
+    } else {
+        e = hipFuncSetAttribute((const void*)k_pole_big<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_pole_big<double>, grid, blk, lds, st, wins, ids, SW, HW, M, ngroup, c, Rr, bmax,
+                           (cx<double>*)work, ws_off, Dpart, Fpart);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_inertia_big(int bmax, int nmax, const WinDesc* wins, const int32_t* ids, int nids, const double* SW,
+                              const double* HW, const double* sigma, int nshift, void* work, const int64_t* ws_off,
+                              int ngroup, int32_t* counts, unsigned long long* total, hipStream_t st) {
+    const size_t lds = big_lds_bytes(nmax, bmax, 8);
+    hipError_t e = hipFuncSetAttribute((const void*)k_inertia_big, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    dim3 grid(nids, std::min(ngroup, nshift)), blk(256);
+    hipLaunchKernelGGL(k_inertia_big, grid, blk, lds, st, wins, ids, SW, HW, sigma, nshift, bmax, (cx<double>*)work,
+                       ws_off, std::min(ngroup, nshift), counts, total);
     return hipGetLastError();
 }

@@ -110,6 +110,8 @@ struct lres_plan {
         Shape shape[2];        // pole kernel shape per precision (0 = fp64, 1 = fp32)
         bool tuned[2] = {false, false};
         Shape ishape;          // inertia kernel shape
+        bool big = false;      // n above the register-resident limit: blocked sweep in a workspace
+        int bmax = 0;          // panel width of the blocked sweep
     };
     std::vector<Bucket> buckets;
     bool autotune = true;
@@ -118,6 +120,8 @@ struct lres_plan {
     DevBuf<int64_t> d_Wflat, d_Woff;
     DevBuf<int32_t> d_pos;                      // local window index -> T-last position
     DevBuf<int32_t> d_bucket_ids;
+    DevBuf<int64_t> d_ws_off;                   // workspace offset per window (big windows), cx<double> units
+    DevBuf<double> d_work;                      // workspace of the blocked sweep (2 doubles per entry)
     DevBuf<double> d_SW, d_HW, d_Q, d_D, d_F, d_Dpart, d_Fpart, d_NB, d_EB, d_sums;
     DevBuf<int64_t> d_own_g, d_xptr;
     DevBuf<int32_t> d_xidx;
@@ -223,6 +227,13 @@ void run_pass(lres_plan* P, double mu, double e_lo, double* d_dens, double* d_NB
     hip_check(hipEventRecord(P->ev[2], P->st), "event");
     const int fp32 = P->precision == LRES_MIXED ? 1 : 0;
     for (auto& b : P->buckets) {
+        if (b.big) {
+            hip_check(launch_pole_big(fp32, b.bmax, b.nmax, P->d_wins.p, P->d_bucket_ids.p + b.id_off,
+                                      (int)b.ids.size(), P->d_SW.p, P->d_HW.p, P->M, P->ngroup, c, R, P->d_work.p,
+                                      P->d_ws_off.p, P->d_Dpart.p, P->d_Fpart.p, P->st),
+                      "blocked pole kernel");
+            continue;
+        }
         if (!b.tuned[fp32]) tune_bucket(P, b, fp32, c, R);
         hip_check(launch_pole_gj(b.shape[fp32].T, b.shape[fp32].G, b.shape[fp32].wide, fp32, P->d_wins.p, P->d_bucket_ids.p + b.id_off,
                                  (int)b.ids.size(), P->d_SW.p, P->d_HW.p, P->M, P->ngroup, c, R, P->d_Dpart.p,
@@ -259,10 +270,18 @@ void inertia(lres_plan* P, const double* sig, int ns, int64_t* h_total, int32_t*
     P->d_sigma.upload(sig, ns, P->st);
     if ((int)P->d_itot.n < ns) P->d_itot.alloc(ns);
     hip_check(hipMemsetAsync(P->d_itot.p, 0, ns * sizeof(unsigned long long), P->st), "memset");
-    for (const auto& b : P->buckets)
-        hip_check(launch_inertia(b.ishape.T, b.ishape.G, P->d_wins.p, P->d_bucket_ids.p + b.id_off, (int)b.ids.size(),
-                                 P->d_SW.p, P->d_HW.p, P->d_sigma.p, ns, d_counts, P->d_itot.p, P->st),
-                  "inertia kernel");
+    for (const auto& b : P->buckets) {
+        if (b.big)
+            hip_check(launch_inertia_big(b.bmax, b.nmax, P->d_wins.p, P->d_bucket_ids.p + b.id_off, (int)b.ids.size(),
+                                         P->d_SW.p, P->d_HW.p, P->d_sigma.p, ns, P->d_work.p, P->d_ws_off.p,
+                                         P->ngroup, d_counts, P->d_itot.p, P->st),
+                      "blocked inertia kernel");
+        else
+            hip_check(launch_inertia(b.ishape.T, b.ishape.G, P->d_wins.p, P->d_bucket_ids.p + b.id_off,
+                                     (int)b.ids.size(), P->d_SW.p, P->d_HW.p, P->d_sigma.p, ns, d_counts,
+                                     P->d_itot.p, P->st),
+                      "inertia kernel");
+    }
     std::vector<unsigned long long> tot(ns);
     hip_check(hipMemcpyAsync(tot.data(), P->d_itot.p, ns * sizeof(unsigned long long), hipMemcpyDeviceToHost, P->st),
               "copy");
@@ -514,8 +533,6 @@ int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, 
             P->kmax = std::max(P->kmax, k);
             mat_off += (int64_t)n * n;
             kk_off += (int64_t)k * k;
-            require(n <= kMaxRegisterN, "window of " + std::to_string(n) +
-                                            " functions exceeds the register-resident solver (n <= 128)");
             for (int a = 0; a < n; ++a) loc[W[a]] = -1;
         }
         // pole groups: enough (window, group) blocks to fill 256 CUs a few times over
@@ -535,12 +552,29 @@ int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, 
             }
         }
         std::vector<int32_t> ids;
+        std::vector<int64_t> ws_off(nwin, 0);
+        int64_t ws_total = 0;
         for (auto& b : P->buckets) {
             b.id_off = (int64_t)ids.size();
             ids.insert(ids.end(), b.ids.begin(), b.ids.end());
-            b.ishape = inertia_shape(b.nmax);
-            require(!pole_shapes(b.nmax).empty(), "no kernel shape for windows of " + std::to_string(b.nmax));
+            b.big = b.nmax > kMaxRegisterN;
+            if (b.big) {
+                // panel width: the widest of 16 / 8 / 4 whose two n x b complex panels fit the 160 KB of LDS
+                b.bmax = 16;
+                while (b.bmax > 4 && big_lds_bytes(b.nmax, b.bmax, 16) > 160 * 1024) b.bmax /= 2;
+                require(big_lds_bytes(b.nmax, b.bmax, 16) <= 160 * 1024,
+                        "window of " + std::to_string(b.nmax) + " functions exceeds the blocked solver's LDS");
+                for (int32_t w : b.ids) {
+                    ws_off[w] = ws_total;
+                    ws_total += (int64_t)P->ngroup * P->n_of[w] * P->n_of[w];
+                }
+            } else {
+                b.ishape = inertia_shape(b.nmax);
+                require(!pole_shapes(b.nmax).empty(), "no kernel shape for windows of " + std::to_string(b.nmax));
+            }
         }
+        P->d_ws_off.upload(ws_off.data(), nwin, P->st);
+        if (ws_total) P->d_work.alloc((size_t)ws_total * 2);
         const char* at = std::getenv("LRES_AUTOTUNE");
         P->autotune = !(at && at[0] == '0');
         const char* tl = std::getenv("LRES_TUNE_LOG");

@@ -238,3 +238,47 @@ def test_mixed_precision_scf_meets_energy_target():
     C.set_precision("fp64")
     n64, _ = C.plan.run(C.mu, C.e_lo, want_density=False)
     assert abs(2 * n64 - cr.n_elec) <= 1e-8 * cr.n_elec
+
+
+@pytest.mark.parametrize("M", [32, 64])
+def test_big_window_pass_matches_oracle(sys3, M):
+    """Windows above the register-resident limit (n=3, R_w 8.5: 19 cells, n_W 228):
+    blocked sweep vs the oracle pole pass and full-spectrum eigen pass."""
+    cr, b, v = sys3
+    print(f"[gpu] blocked sweep n_W=228 M={M}", flush=True)
+    from lresdft_amd.lcao3d import csr64
+    cells = cell_centres((3, 3, 3), 6.0)
+    C = _plan(b, cells, 8.5, M)
+    assert int(C.plan.n.max()) == 228
+    H = b.H(v)
+    W, own = build_windows(b, cells, 8.5)
+    C.plan.set_h(csr64(H))
+    lmin = C.plan.lambda_min(1e-11)
+    lam0 = min(linalg.eigh(H[w][:, w].toarray(), b.S[w][:, w].toarray(), eigvals_only=True)[0] for w in W)
+    assert abs(lmin - lam0) < 1e-10
+    e_lo, mu = lmin - 0.3, lmin + 0.35
+    nsum, esum = C.plan.run(mu, e_lo)
+    dens = C.plan.dens.cpu().numpy()
+    de, lam, c, lamoff, _ = eig_window_pass(H, b.S, b.Xr, W, own, cr.dV, mu, e_lo, M, None, True)
+    assert np.max(np.abs(dens - de)) < 1e-12
+    occ = r_M(lam, mu, e_lo, M)
+    assert abs(nsum - np.dot(c, occ)) < 1e-11 and abs(esum - np.dot(c, lam * occ)) < 1e-11
+    C.set_precision("mixed")
+    n32, e32 = C.plan.run(mu, e_lo)
+    assert abs(n32 - nsum) / nsum < 1e-5 and abs(e32 - esum) / abs(esum) < 1e-5
+
+
+def test_cpp_kernels_reproduce_python_scenario():
+    """tests/test_three_d.py:86-100 on the GPU path: n=3, R_w 8.5, default M 64, cold then
+    warm density vs the Python eigen route; e_band 1e-9, density 1e-10 (full spectra)."""
+    print("[gpu] reference C++-vs-Python scenario n=3 R_w=8.5 M=64", flush=True)
+    cr = crystal(3)
+    b = LCAOBasis(cr, **BASIS)
+    v = guess_v(cr)
+    cells = cell_centres((3, 3, 3), 6.0)
+    ref = LResEigen(b, cells, R_w=8.5, M=64, full_spectrum=True)
+    npy, epy, _ = ref.density(v)
+    C = _plan(b, cells, 8.5, 64)
+    for _ in range(2):
+        ncp, ecp, _ = C.density(v)
+        assert abs(ecp - epy) < 1e-9 and np.max(np.abs(ncp - npy)) < 1e-10
