@@ -768,8 +768,10 @@ __device__ int big_block_step(E* __restrict__ A, int n, int p0, int b, E* Cl, E*
     return npos;
 }
 
+constexpr int kBigThreads = 1024;
+
 template <typename R>
-__global__ void __launch_bounds__(256) k_pole_big(const WinDesc* __restrict__ wins, const int32_t* __restrict__ ids,
+__global__ void __launch_bounds__(kBigThreads) k_pole_big(const WinDesc* __restrict__ wins, const int32_t* __restrict__ ids,
                                                   const double* __restrict__ SW, const double* __restrict__ HW,
                                                   int M, int ngroup, double c, double Rr, int bmax,
                                                   cx<double>* __restrict__ work, const int64_t* __restrict__ ws_off,
@@ -814,7 +816,7 @@ __global__ void __launch_bounds__(256) k_pole_big(const WinDesc* __restrict__ wi
     }
 }
 
-__global__ void __launch_bounds__(256) k_inertia_big(const WinDesc* __restrict__ wins, const int32_t* __restrict__ ids,
+__global__ void __launch_bounds__(kBigThreads) k_inertia_big(const WinDesc* __restrict__ wins, const int32_t* __restrict__ ids,
                                                      const double* __restrict__ SW, const double* __restrict__ HW,
                                                      const double* __restrict__ sigma, int nshift, int bmax,
                                                      cx<double>* __restrict__ work, const int64_t* __restrict__ ws_off,
@@ -854,7 +856,7 @@ hipError_t launch_pole_big(int fp32, int bmax, int nmax, const WinDesc* wins, co
                            const double* SW, const double* HW, int M, int ngroup, double c, double Rr, void* work,
                            const int64_t* ws_off, double* Dpart, double* Fpart, hipStream_t st) {
     const size_t lds = big_lds_bytes(nmax, bmax, fp32 ? 8 : 16);
-    dim3 grid(nids, ngroup), blk(256);
+    dim3 grid(nids, ngroup), blk(kBigThreads);
     hipError_t e;
     if (fp32) {
         e = hipFuncSetAttribute((const void*)k_pole_big<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -876,7 +878,7 @@ hipError_t launch_inertia_big(int bmax, int nmax, const WinDesc* wins, const int
     const size_t lds = big_lds_bytes(nmax, bmax, 8);
     hipError_t e = hipFuncSetAttribute((const void*)k_inertia_big, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    dim3 grid(nids, std::min(ngroup, nshift)), blk(256);
+    dim3 grid(nids, std::min(ngroup, nshift)), blk(kBigThreads);
     hipLaunchKernelGGL(k_inertia_big, grid, blk, lds, st, wins, ids, SW, HW, sigma, nshift, bmax, (cx<double>*)work,
                        ws_off, std::min(ngroup, nshift), counts, total);
     return hipGetLastError();
