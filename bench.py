@@ -37,6 +37,10 @@ def parse():
     ap.add_argument("--rw", type=float, default=8.0)
     ap.add_argument("--M", type=int, default=32)
     ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--random", type=int, default=0,
+                    help="instead of the lattice: this many H2 at random centres (min separation 3.0 bohr) in a "
+                         "periodic cube at the density of the a = 6.0 lattice (uneven windows)")
+    ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     return ap.parse_args()
 
@@ -127,9 +131,17 @@ def main():
     from lresdft_amd.scf import GridOps
 
     n = args.n
-    cr = system.h2_crystal((n, n, n), a=6.0, pts_per_cell=10, rc=0.6)
+    if args.random:
+        box = 6.0 * args.random ** (1.0 / 3.0)
+        cr, cells = system.random_h2_box(args.random, box, dmin=3.0, seed=args.seed, dx=0.6, rc=0.6)
+        workload = (f"{args.random} H2 at random centres (d_min 3.0, seed {args.seed}) in a {box:.2f} bohr cube, "
+                    f"{cr.n_elec} e-, R_w {args.rw}, M {args.M}, SCF tol {args.tol}")
+    else:
+        cr = system.h2_crystal((n, n, n), a=6.0, pts_per_cell=10, rc=0.6)
+        cells = system.cell_centres((n, n, n), 6.0)
+        workload = (f"h2_crystal {n}x{n}x{n} a=6.0 jitter 0.15 seed 0, {cr.n_elec} e-, "
+                    f"R_w {args.rw}, M {args.M}, SCF tol {args.tol}")
     b = system.LCAOBasis(cr, **system.DEFAULT_BASIS)
-    cells = system.cell_centres((n, n, n), 6.0)
     grid = GridOps(cr)
     vion = grid.ion_potential()
     C = LRFTLCAOCpp(b, cells, R_w=args.rw, M=args.M)
@@ -156,9 +168,9 @@ def main():
         "metric": METRIC, "value": t_mixed, "unit": "s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * t_mixed, "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "c64+f64", "data": "synthetic",
-        "config": {"workload": f"h2_crystal {n}x{n}x{n} a=6.0 jitter 0.15 seed 0, {cr.n_elec} e-, "
-                               f"R_w {args.rw}, M {args.M}, SCF tol {args.tol}",
-                   "windows": int(C.plan.nwin), "n_W": int(C.plan.n.max()), "grid": list(cr.shape),
+        "config": {"workload": workload,
+                   "windows": int(C.plan.nwin), "n_W": int(C.plan.n.max()), "n_W_min": int(C.plan.n.min()),
+                   "n_W_mean": float(C.plan.n.mean()), "grid": list(cr.shape),
                    "buckets": C.plan.buckets()},
         "fp64_scf_s": t_fp64, "speedup_vs_fp64": t_fp64 / t_mixed, "dE_per_e_vs_fp64": dE,
         "energy_fp64": fp64[-1]["energy"], "energy_mixed": mixed[-1]["energy"],
@@ -169,7 +181,7 @@ def main():
         "roofline_fp64": roofline(st64["fp64"], flops, PEAK_FP64),
         "cpu_baseline": None,
     }
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not args.random:
         v_host = (vion + grid.hartree(fp64[-1]["dens_in"]) + grid.lda_xc(fp64[-1]["dens_in"])[1]).cpu().numpy()
         line["cpu_baseline"] = cpu_baseline(cr, v_host, n, args.rw, args.M, fp64[-1]["iterations"])
     print(json.dumps(line), flush=True)

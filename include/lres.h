@@ -111,6 +111,10 @@ int lres_set_potential(lres_plan* plan, const double* d_v);
 int lres_window_pass(lres_plan* plan, double mu, double e_lo, double* d_dens, double* d_NB, double* d_EB,
                      double* h_sums);
 
+/* Owned-point density of the last pass (its D_B blocks are kept on the
+ * device): the density output of lres_window_pass without repeating the solves. */
+int lres_density_last(lres_plan* plan, double* d_dens);
+
 /* Inertia counts: for each shift sigma_s (s < nshift, host array), the number of
  * window eigenvalues (H_W, S_W) below sigma_s summed over windows (h_total[s])
  * and per window (d_counts, nwin x nshift int32, may be NULL).  Sylvester's law

@@ -39,6 +39,7 @@ SIGNATURES = {
                                       _i64p, _i64p]),
     "lres_set_potential": (C.c_int, [_vp, _vp]),
     "lres_window_pass": (C.c_int, [_vp, C.c_double, C.c_double, _vp, _vp, _vp, _f64p]),
+    "lres_density_last": (C.c_int, [_vp, _vp]),
     "lres_inertia": (C.c_int, [_vp, _f64p, C.c_int, _i64p, _vp]),
     "lres_lambda_min": (C.c_int, [_vp, C.c_double, _f64p]),
     "lres_mu_solve": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,

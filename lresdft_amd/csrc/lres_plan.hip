@@ -854,6 +854,16 @@ int lres_window_pass(lres_plan* P, double mu, double e_lo, double* d_dens, doubl
     });
 }
 
+int lres_density_last(lres_plan* P, double* d_dens) {
+    return guarded([&] {
+        require(P != nullptr, "null plan");
+        require(d_dens != nullptr, "null density buffer");
+        hip_check(launch_density(P->d_wins.p, (int)P->nwin, P->kmax, P->d_D.p, P->d_own_g.p, P->d_xptr.p, P->d_xidx.p,
+                                 P->d_xval.p, d_dens, P->st),
+                  "density kernel");
+    });
+}
+
 int lres_inertia(lres_plan* P, const double* sig, int ns, int64_t* h_total, int32_t* d_counts) {
     return guarded([&] {
         require(P != nullptr, "null plan");

@@ -153,15 +153,22 @@ class WindowPlan:
         nat.check(nat.lib().lres_set_hamiltonian_csr_device(self.h, Hp.data_ptr(), Hi.data_ptr(), Hv.data_ptr()))
 
     def run(self, mu, e_lo, want_density=True):
-        """One pass; returns (sum_B N_B, sum_B E_B) (no spin factor).  With the
-        density, self.last records (mu, e_lo, sums) of the pass self.dens holds."""
+        """One pass; returns (sum_B N_B, sum_B E_B) (no spin factor).  self.last
+        records (mu, e_lo, sums, density formed) of the latest pass, whose D_B
+        blocks stay on the device (density_last() forms its density later)."""
         sums = np.zeros(2)
         self.passes += 1
         nat.check(nat.lib().lres_window_pass(self.h, float(mu), float(e_lo),
                                              self.dens.data_ptr() if want_density else None,
                                              self.NB.data_ptr(), self.EB.data_ptr(), nat.pf(sums)))
-        self.last = (float(mu), float(e_lo), float(sums[0]), float(sums[1])) if want_density else None
+        self.last = (float(mu), float(e_lo), float(sums[0]), float(sums[1]), bool(want_density))
         return float(sums[0]), float(sums[1])
+
+    def density_last(self):
+        """Density of the latest pass into self.dens (no new solves)."""
+        nat.check(nat.lib().lres_density_last(self.h, self.dens.data_ptr()))
+        if self.last is not None:
+            self.last = self.last[:4] + (True,)
 
     def inertia(self, sigmas):
         s = nat.f64(np.atleast_1d(sigmas))
@@ -175,12 +182,12 @@ class WindowPlan:
         return float(out[0])
 
     def mu_solve(self, N, e_lo, lo, hi, f_lo=math.nan, f_hi=math.nan, xtol=1e-12, ftol=0.0):
-        """Brent root of the count on [lo, hi]; every trial pass writes self.dens.
+        """Brent root of the count on [lo, hi] (trial passes without density).
         Returns (mu, [passes, count at mu, 1 if mu was the last pass, sum E_B there])."""
         mu = np.zeros(1)
         info = np.zeros(4)
         nat.check(nat.lib().lres_mu_solve(self.h, float(N), float(e_lo), float(lo), float(hi), float(f_lo),
-                                          float(f_hi), float(xtol), float(ftol), self.dens.data_ptr(), nat.pf(mu),
+                                          float(f_hi), float(xtol), float(ftol), None, nat.pf(mu),
                                           nat.pf(info)))
         self.passes += int(info[0])
         self.last = None
@@ -213,12 +220,13 @@ class LRFTLCAOCpp:
     (one GPU); ``lag_mu`` keeps the reference's meaning (lcao3d.py:423-427)."""
 
     def __init__(self, basis, cells, R_w=12.1, M=64, margin=0.3, nthreads=1, lag_mu=False, reuse_vectors=True,
-                 precision="fp64", mu_xtol=1e-12, device_h=True):
+                 precision="fp64", mu_xtol=1e-12, device_h=True, mu_ftol=1e-11):
         if M % 2:
             raise ValueError("M must be even")
         self.b, self.M, self.margin = basis, M, margin
         self.lag_mu, self.reuse_vectors, self.nthreads = lag_mu, reuse_vectors, nthreads
         self.mu_xtol = mu_xtol
+        self.mu_ftol = mu_ftol
         cr = basis.cr
         self.Wflat, self.Woff, self.Oflat, self.Ooff = build_windows(basis, cells, R_w)
         self.windows = [self.Wflat[self.Woff[i]:self.Woff[i + 1]] for i in range(len(self.Woff) - 1)]
@@ -251,16 +259,19 @@ class LRFTLCAOCpp:
 
     # -- mu search -----------------------------------------------------------
     # The root of count(mu) = 2 sum_B N_B(mu) - N is accepted at |count| <= ftol:
-    # fp64 passes resolve counts to ~1e-13 N; complex64 passes carry ~1e-8 N of
-    # rounding noise, so their search stops at 1e-7 N and refine() finishes the
-    # root in fp64.  Brent's xtol is the reference's (brentq xtol=1e-12).
+    # fp64: mu_ftol N (default 1e-11 N, which reproduces the reference's brentq
+    # roots closely enough for 1e-10 Ha/electron SCF parity; the reference itself
+    # keeps any mu with |count| <= 1e-8 N, lcao3d.py:412, 428); complex64 passes
+    # carry ~1e-8 N of rounding noise, so their search stops at 1e-7 N and
+    # refine() finishes the root in fp64.  Brent's xtol is the reference's
+    # (brentq xtol=1e-12).
     def _ftol(self, N):
-        return (1e-11 if self.precision == "fp64" else 1e-7) * N
+        return (self.mu_ftol if self.precision == "fp64" else 1e-7) * N
 
     def _count(self, mu, e_lo, N):
-        # trial passes also form the density (a small part of a pass), so that
-        # the pass at the accepted mu need not be repeated
-        n, _ = self.plan.run(mu, e_lo, want_density=True)
+        # trial passes keep their D_B blocks on the device; if the accepted mu is
+        # the last trial, its density is formed from them without new solves
+        n, _ = self.plan.run(mu, e_lo, want_density=False)
         return 2.0 * n - N
 
     def _note_slope(self, mu0, f0, mu1, f1):
@@ -269,8 +280,8 @@ class LRFTLCAOCpp:
 
     def _brent(self, N, e_lo, lo, hi, f_lo, f_hi):
         mu, info = self.plan.mu_solve(N, e_lo, lo, hi, f_lo, f_hi, self.mu_xtol, self._ftol(N))
-        if info[2] == 1.0:      # the density buffer holds the pass at the returned mu
-            self.plan.last = (mu, float(e_lo), 0.5 * (info[1] + N), float(info[3]))
+        if info[2] == 1.0:      # the device D_B blocks are those of the returned mu
+            self.plan.last = (mu, float(e_lo), 0.5 * (info[1] + N), float(info[3]), False)
         return mu, info[1]
 
     def _solve_mu_cold(self, lam_min, e_lo, N):
@@ -351,6 +362,8 @@ class LRFTLCAOCpp:
         last = self.plan.last
         if last is not None and last[0] == mu and last[1] == e_lo:
             n, e = last[2], last[3]
+            if not last[4]:
+                self.plan.density_last()
         else:
             n, e = self.plan.run(mu, e_lo, want_density=True)
         return 2.0 * self.plan.dens, 2.0 * e, n
@@ -388,17 +401,16 @@ class LRFTLCAOCpp:
             mu, e_lo = self.mu, self.e_lo
             c_lo, c_hi = self.plan.inertia([e_lo + m / 3, e_lo + 2 * m])
             if c_lo == 0 and c_hi > 0:            # keep the filter edge (lcao3d.py:420)
-                dens, e_band, nsum = self._finish(mu, e_lo)
+                nsum, _ = self.plan.run(mu, e_lo, want_density=False)
                 cnt = 2.0 * nsum - N
-                if self.lag_mu and abs(cnt) < 0.25:
-                    self.history.append("warm-lag")
-                    return dens, e_band, mu
-                if abs(cnt) > tol:
+                if abs(cnt) <= tol or (self.lag_mu and abs(cnt) < 0.25):
+                    # keep mu (lcao3d.py:423-427, 447-448): the density of this pass
+                    dens, e_band, _ = self._finish(mu, e_lo)
+                    self.history.append("warm-keep" if abs(cnt) <= tol else "warm-lag")
+                else:
                     mu = self._solve_mu_warm(mu, cnt, e_lo, N)
                     dens, e_band, _ = self._finish(mu, e_lo)
                     self.history.append("warm-resolve")
-                else:
-                    self.history.append("warm-keep")
                 self.mu = mu
                 return dens, e_band, mu
         lam_min = self.plan.lambda_min()

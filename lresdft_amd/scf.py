@@ -75,12 +75,13 @@ class GridOps:
         return ex + torch.where(high, ec_h, ec_l), vx + torch.where(high, vc_h, vc_l)
 
 
-def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion=None, dens0=None):
+def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion=None, dens0=None, fp32_floor=1e-6):
     """Device SCF; ``solver.density_device(v)`` returns (density tensor [ng], e_band, mu).
 
-    With a mixed-precision solver the iterations run on complex64 passes and
-    the converged step is finished in fp64 (``solver.refine()``): the energy
-    and the returned density are fp64 results at the converged input density."""
+    With a mixed-precision solver the iterations run on complex64 passes until
+    the residual falls below ``fp32_floor`` (then fp64 passes), and a step that
+    converged on complex64 passes is finished in fp64 (``solver.refine()``): the
+    energy and the returned density are always fp64 results."""
     grid = GridOps(cr) if grid is None else grid
     torch = grid.torch
     t0 = time.perf_counter()
@@ -90,6 +91,7 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
             else dens0.clone())
     hist_n, hist_r, hist_g = [], [], []
     converged = False
+    handed_over = False
     t_solver = 0.0
     for it in range(1, maxiter + 1):
         v = vion + grid.hartree(dens) + grid.lda_xc(dens)[1]
@@ -102,6 +104,11 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
         if err < tol:
             converged = True
             break
+        if err < fp32_floor and getattr(solver, "precision", "fp64") == "mixed":
+            # complex64 densities carry ~1e-7 relative noise: below this residual the
+            # remaining iterations use fp64 passes (the converged step is fp64 anyway)
+            solver.set_precision("fp64")
+            handed_over = True
         hist_n.append(dens.clone())
         hist_r.append(resid.clone())
         # Gram matrix of the residuals, one new row of dot products per iteration
@@ -127,7 +134,9 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
             dens = dens + mix * resid
         dens = torch.clamp(dens, min=1e-12)
         dens = dens * (N / (float(dens.sum()) * dV))
-    if getattr(solver, "precision", "fp64") == "mixed" and hasattr(solver, "refine"):
+    if handed_over:
+        solver.set_precision("mixed")
+    elif getattr(solver, "precision", "fp64") == "mixed" and hasattr(solver, "refine"):
         # fp64 finish: mu, density and band energy of the converged input potential in fp64
         ts = time.perf_counter()
         nout, eband, mu = solver.refine()
