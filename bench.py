@@ -51,6 +51,7 @@ def run_scf(C, cr, grid, vion, precision, tol):
     C.set_precision(precision)
     C.mu = None
     C.history = []
+    C.pass_log = []
     p0 = C.passes
     torch.cuda.synchronize()
     t = time.perf_counter()
@@ -59,6 +60,7 @@ def run_scf(C, cr, grid, vion, precision, tol):
     r["wall"] = time.perf_counter() - t
     r["passes"] = C.passes - p0
     r["history"] = list(C.history)
+    r["pass_log"] = list(C.pass_log)
     return r
 
 
@@ -177,6 +179,7 @@ def main():
         "iterations": {"fp64": fp64[-1]["iterations"], "mixed": mixed[-1]["iterations"]},
         "passes_per_scf": {"fp64": fp64[-1]["passes"], "mixed": mixed[-1]["passes"]},
         "history": {"fp64": fp64[-1]["history"], "mixed": mixed[-1]["history"]},
+        "passes_per_iteration": {"fp64": fp64[-1]["pass_log"], "mixed": mixed[-1]["pass_log"]},
         "roofline": roofline(st32["fp32"], flops, PEAK_FP32) or roofline(st32["fp64"], flops, PEAK_FP64),
         "roofline_fp64": roofline(st64["fp64"], flops, PEAK_FP64),
         "cpu_baseline": None,

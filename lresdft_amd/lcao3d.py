@@ -246,6 +246,8 @@ class LRFTLCAOCpp:
         self.history = []
         self.precision = precision
         self._slope = None
+        self.pass_log = []      # passes per density call
+        self.count_tol = None   # optional looser mu tolerance (relative to N) set by the SCF driver
 
     @property
     def passes(self):
@@ -266,7 +268,10 @@ class LRFTLCAOCpp:
     # refine() finishes the root in fp64.  Brent's xtol is the reference's
     # (brentq xtol=1e-12).
     def _ftol(self, N):
-        return (self.mu_ftol if self.precision == "fp64" else 1e-7) * N
+        base = self.mu_ftol if self.precision == "fp64" else 1e-7
+        # an SCF driver may allow a looser root while its density residual is large
+        # (count_tol, relative to N); the converged step is polished by refine()
+        return max(base, self.count_tol or 0.0) * N
 
     def _count(self, mu, e_lo, N):
         # trial passes keep their D_B blocks on the device; if the accepted mu is
@@ -331,24 +336,28 @@ class LRFTLCAOCpp:
         raise StaleMu("no electron-count bracket around the previous mu")
 
     def refine(self, N=None):
-        """fp64 finish of a mixed-precision step: an fp64 pass at the current
-        (mu, e_lo); if its count misses N by more than the reference's keep
-        tolerance 1e-8 N, secant/Brent steps in fp64 move mu.  Returns the fp64
-        (density tensor, e_band, mu) and leaves the precision as it was."""
+        """fp64 finish of a converged step (on the H of the last call): an fp64
+        pass at the current (mu, e_lo) and, if its count misses N by more than
+        the strict fp64 tolerance mu_ftol N, secant/Brent steps in fp64.  Serves
+        the mixed-precision path and steps solved with a looser count_tol.
+        Returns the fp64 (density tensor, e_band, mu); precision and count_tol
+        are left as they were."""
         N = self.b.cr.n_elec if N is None else N
-        prec = self.precision
+        prec, ctol = self.precision, self.count_tol
         self.set_precision("fp64")
+        self.count_tol = None
         try:
             mu, e_lo = self.mu, self.e_lo
-            dens, e_band, nsum = self._finish(mu, e_lo)
+            nsum, _ = self.plan.run(mu, e_lo, want_density=False)
             cnt = 2.0 * nsum - N
-            if abs(cnt) > 1e-8 * N:
+            if abs(cnt) > self._ftol(N):
                 mu = self._solve_mu_warm(mu, cnt, e_lo, N)
-                dens, e_band, _ = self._finish(mu, e_lo)
                 self.mu = mu
+            dens, e_band, _ = self._finish(mu, e_lo)
             return dens, e_band, mu
         finally:
             self.set_precision(prec)
+            self.count_tol = ctol
 
     # -- density (lcao3d.py:400-456) ------------------------------------------
     def density(self, v, N=None):
@@ -388,11 +397,14 @@ class LRFTLCAOCpp:
             raise RuntimeError("density_device needs device_h=True")
         N = self.b.cr.n_elec if N is None else N
         self.plan.set_potential(v_dev)
+        p0 = self.plan.passes
         try:
-            return self._density_loaded(N)
+            out = self._density_loaded(N)
         except StaleMu:
             self.mu = None
-            return self._density_loaded(N)
+            out = self._density_loaded(N)
+        self.pass_log.append(self.plan.passes - p0)
+        return out
 
     def _density_loaded(self, N):
         tol = max(1e-8 * N, self._ftol(N))   # the reference's keep-mu rule, floored at the pass precision

@@ -78,10 +78,13 @@ class GridOps:
 def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion=None, dens0=None, fp32_floor=1e-6):
     """Device SCF; ``solver.density_device(v)`` returns (density tensor [ng], e_band, mu).
 
-    With a mixed-precision solver the iterations run on complex64 passes until
-    the residual falls below ``fp32_floor`` (then fp64 passes), and a step that
-    converged on complex64 passes is finished in fp64 (``solver.refine()``): the
-    energy and the returned density are always fp64 results."""
+    A solver with ``count_tol``/``refine`` (``LRFTLCAOCpp``) solves mu each step
+    to a count accuracy 1e-3 x the current density residual (clamped to
+    [1e-11, 1e-5] N), and the converged step is finished in fp64 to the strict
+    tolerance (``solver.refine()``).  With mixed precision the iterations run on
+    complex64 passes until the residual falls below ``fp32_floor`` (then fp64
+    passes).  The energy and the returned density are always fp64 results at
+    the converged input density."""
     grid = GridOps(cr) if grid is None else grid
     torch = grid.torch
     t0 = time.perf_counter()
@@ -93,8 +96,14 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
     converged = False
     handed_over = False
     t_solver = 0.0
+    err = None
+    adaptive = hasattr(solver, "count_tol") and hasattr(solver, "refine")
     for it in range(1, maxiter + 1):
         v = vion + grid.hartree(dens) + grid.lda_xc(dens)[1]
+        if adaptive:
+            # mu to a count accuracy that matches the current density residual; the
+            # converged step is polished in fp64 to the strict tolerance below
+            solver.count_tol = 1e-5 if err is None else min(1e-5, max(1e-3 * err, 1e-11))
         ts = time.perf_counter()
         nout, eband, mu = solver.density_device(v.reshape(-1).contiguous())
         nout = nout.reshape(cr.shape)
@@ -134,14 +143,16 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
             dens = dens + mix * resid
         dens = torch.clamp(dens, min=1e-12)
         dens = dens * (N / (float(dens.sum()) * dV))
-    if handed_over:
-        solver.set_precision("mixed")
-    elif getattr(solver, "precision", "fp64") == "mixed" and hasattr(solver, "refine"):
-        # fp64 finish: mu, density and band energy of the converged input potential in fp64
+    if adaptive:
+        # fp64 finish: mu to the strict count tolerance, density and band energy of
+        # the converged input potential in fp64
+        solver.count_tol = None
         ts = time.perf_counter()
         nout, eband, mu = solver.refine()
         nout = nout.reshape(cr.shape)
         t_solver += time.perf_counter() - ts
+    if handed_over:
+        solver.set_precision("mixed")
     vh = grid.hartree(dens)
     exc, vxc = grid.lda_xc(dens)
     E_H = 0.5 * float(torch.sum(dens * vh)) * dV

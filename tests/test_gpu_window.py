@@ -197,7 +197,7 @@ def test_device_hamiltonian_equals_csr_route(sys3):
 
 def test_gpu_scf_driver_matches_oracle_scf():
     """Device SCF (torch Hartree/LDA/DIIS + device H(v) + pole passes) vs the
-    oracle's CPU SCF with the eigen route, n=2 R_w 8 M 32, tol 1e-8."""
+    oracle's CPU SCF with the eigen route, n=2 R_w 8 M 32, tol 1e-8: energy to 1e-10 Ha/e."""
     print("[gpu] device SCF n=2 R_w=8 M=32", flush=True)
     from lresdft_amd import system
     from lresdft_amd.lcao3d import LRFTLCAOCpp
@@ -209,7 +209,10 @@ def test_gpu_scf_driver_matches_oracle_scf():
     bo = LCAOBasis(cro, **BASIS)
     ref = scf_lcao(cro, LResEigen(bo, cells, R_w=8.0, M=32, full_spectrum=True, reuse_vectors=False), tol=1e-8)
     gpu = scf_gpu(cr, LRFTLCAOCpp(b, cells, R_w=8.0, M=32), tol=1e-8)
-    assert gpu["converged"] and gpu["iterations"] == ref["iterations"]
+    # the device driver solves mu loosely while the density residual is large (its
+    # trajectory may take a different number of steps) and polishes the converged
+    # step to the strict tolerance, so the energy keeps the 1e-10 bar
+    assert gpu["converged"] and abs(gpu["iterations"] - ref["iterations"]) <= 2
     assert abs(gpu["energy"] - ref["energy"]) / cr.n_elec < 1e-10
 
 
