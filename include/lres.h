@@ -68,9 +68,14 @@ int lres_plan_sizes(const lres_plan* plan, int32_t* h_n, int32_t* h_k);
 
 /* Window buckets of the plan (windows grouped by size, one pole-kernel launch
  * each): for bucket i < min(cap, *h_nbucket), h_out[8i..8i+7] = {largest n_W,
- * windows, fp64 tile T, fp64 grid G, fp64 register budget (0: <= 128 VGPRs,
- * 1: <= 512), fp32 T, fp32 G, fp32 budget} (zeros until the first pass in that
- * precision has run the autotuner). */
+ * windows, fp64 T, fp64 G, fp64 kind, fp32 T, fp32 G, fp32 kind} (zeros until
+ * the first pass in that precision has run the autotuner).  kind 0 / 1: vector
+ * ALU Gauss-Jordan with a T x T tile per thread on a G x G grid and a register
+ * budget of 128 / 512; kind 2: matrix cores (MFMA), T = 16 x 16 tiles per
+ * dimension, G = waves per (window, pole group).  By default fp32 buckets use
+ * kind 2 and fp64 buckets kinds 0 / 1; the environment variable
+ * LRES_POLE_KERNEL=mfma or =valu, read at plan creation and tuning, selects one
+ * family for both. */
 int lres_plan_buckets(const lres_plan* plan, int32_t* h_out, int cap, int* h_nbucket);
 
 /* Set precision (LRES_FP64 / LRES_MIXED) for later passes. */

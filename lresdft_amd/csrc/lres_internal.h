@@ -47,3 +47,21 @@ hipError_t launch_inertia_big(int bmax, int nmax, const lres::WinDesc* wins, con
                               const double* SW, const double* HW, const double* sigma, int nshift, void* work,
                               const int64_t* ws_off, int ngroup, int32_t* counts, unsigned long long* total,
                               hipStream_t st);
+// Pole solves on the matrix cores for windows of up to 128 functions: nt = 16 x 16
+// tiles per dimension (2..8), nwc = waves per (window, pole group) splitting the
+// tile columns (1..nt).  One object per (precision, nt),
+// compiled in parallel.
+#define LRES_MFMA_PART_DECL(r, nt)                                                                              \
+    hipError_t lres_mfma_part_##r##_##nt(int nwc, const lres::WinDesc* wins, const int32_t* ids, int nids,     \
+                                         const double* SW, const double* HW, int M, int ngroup, double c,     \
+                                         double Rr, double* Dpart, double* Fpart, hipStream_t st);
+#define LRES_MFMA_PARTS(X) X(float, 2) X(float, 3) X(float, 4) X(float, 5) X(float, 6) X(float, 7) X(float, 8) \
+    X(double, 2) X(double, 3) X(double, 4) X(double, 5) X(double, 6) X(double, 7) X(double, 8)
+LRES_MFMA_PARTS(LRES_MFMA_PART_DECL)
+hipError_t launch_pole_mfma(int fp32, int nt, int nwc, const lres::WinDesc* wins, const int32_t* ids, int nids,
+                            const double* SW, const double* HW, int M, int ngroup, double c, double Rr,
+                            double* Dpart, double* Fpart, hipStream_t st);
+// accumulator registers per lane of one wave of the MFMA pole kernel
+int mfma_acc_regs(int fp32, int nt, int nwc);
+// scratch bytes per lane of an MFMA variant (register spills; -1: no such variant)
+int mfma_scratch_bytes(int fp32, int nt, int nwc);

@@ -5,7 +5,8 @@
 //            "T-last" order: the m = n - k window functions without support on
 //            own(B) first, then the k functions of T_B (both ascending).
 //   Q        per window k x k float64, Q_B = sum_{g in own(B)} x_g x_g^T dV.
-//   Dpart/Fpart  per (window, pole group) k x k float64 partial pole sums.
+//   Dpart/Fpart  per (window, pole group) k x k float64 partial pole sums, upper
+//            triangle only (G[T,T] of a complex symmetric pencil is symmetric).
 //   D/F      per window k x k float64: D_B = 2 Re sum_j w_j G_j[T,T], F_B likewise with w_j z_j.
 //   owned rows of X restricted to T: CSR over owned points, int32 T-local columns.
 //
@@ -23,36 +24,10 @@
 
 #include <algorithm>
 
+#include "lres_device.h"
 #include "lres_internal.h"
 
 namespace lres {
-
-template <typename R>
-struct cx {
-    R re, im;
-};
-
-template <typename R>
-__device__ __forceinline__ cx<R> cmul(cx<R> a, cx<R> b) {
-    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
-}
-
-template <typename R>
-__device__ __forceinline__ cx<R> crecip(cx<R> a) {
-    const R d = a.re * a.re + a.im * a.im;
-    const R s = R(1) / d;
-    return {a.re * s, -a.im * s};
-}
-
-// z_j = c + R e^{i theta_j}, w_j = R e^{i theta_j} / M, theta_j = pi (2j+1)/M (lrft/core.py:130-139)
-__device__ __forceinline__ void pole(int j, int M, double c, double Rr, cx<double>& z, cx<double>& w,
-                                     cx<double>& wz) {
-    double s, co;
-    sincospi((double)(2 * j + 1) / (double)M, &s, &co);
-    z = {c + Rr * co, Rr * s};
-    w = {Rr * co / M, Rr * s / M};
-    wz = cmul(w, z);
-}
 
 // ---------------------------------------------------------------------------
 // gather of dense H_W from a CSR H (src/lrft_cpp.cpp:117-127), one block per window
@@ -247,7 +222,7 @@ __global__ void __launch_bounds__(MAXT) k_pole_gj(const WinDesc* __restrict__ wi
 #pragma unroll
             for (int j = 0; j < T; ++j) {
                 const int col = tx + G * j - mm;
-                if (r >= 0 && col >= 0 && r < kk && col < kk) {
+                if (r >= 0 && r <= col && col < kk) {
                     const double gr = (double)A[i][j].re, gi = (double)A[i][j].im;
                     const double dv = wt.re * gr - wt.im * gi, fv = wz.re * gr - wz.im * gi;
                     const int64_t q = (int64_t)r * kk + col;
@@ -294,10 +269,14 @@ __global__ void __launch_bounds__(256) k_reduce(const WinDesc* __restrict__ wins
     const double* fp = Fpart + d.kk_off * ngroup;
     double sn = 0.0, se = 0.0;
     for (int64_t q = threadIdx.x; q < kk; q += blockDim.x) {
+        // the pole kernels write the upper triangle of each partial (G[T,T] is
+        // complex symmetric): entry (i, j) is read from (min, max)
+        const int i = (int)(q / d.k), j = (int)(q - (int64_t)i * d.k);
+        const int64_t qs = i <= j ? q : (int64_t)j * d.k + i;
         double dv = 0.0, fv = 0.0;
         for (int gq = 0; gq < ngroup; ++gq) {
-            dv += dp[gq * kk + q];
-            fv += fp[gq * kk + q];
+            dv += dp[gq * kk + qs];
+            fv += fp[gq * kk + qs];
         }
         dv *= 2.0;
         fv *= 2.0;
@@ -509,6 +488,38 @@ hipError_t launch_pole_gj(int T, int G, int wide, int fp32, const WinDesc* wins,
                           double* Dpart, double* Fpart, hipStream_t st) {
     if (fp32) return launch_pole_t<float>(T, G, wide, wins, ids, nids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart, st);
     return launch_pole_t<double>(T, G, wide, wins, ids, nids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart, st);
+}
+
+int mfma_acc_regs(int fp32, int nt, int nwc) {
+    const int tj = (nt + nwc - 1) / nwc;
+    return nt * tj * (fp32 ? 8 : 16);
+}
+
+static bool mfma_nwc_ok(int nt, int nwc) {
+    return nt >= 2 && nt <= 8 && nwc >= 1 && nwc <= nt;
+}
+
+hipError_t launch_pole_mfma(int fp32, int nt, int nwc, const WinDesc* wins, const int32_t* ids, int nids,
+                            const double* SW, const double* HW, int M, int ngroup, double c, double Rr,
+                            double* Dpart, double* Fpart, hipStream_t st) {
+    if (!mfma_nwc_ok(nt, nwc)) return hipErrorInvalidValue;
+#define LRES_MFMA_CALL(r, NT)                                                                                  \
+    if (nt == NT && (fp32 != 0) == (sizeof(r) == 4))                                                           \
+        return lres_mfma_part_##r##_##NT(nwc, wins, ids, nids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart, st);
+    LRES_MFMA_PARTS(LRES_MFMA_CALL)
+#undef LRES_MFMA_CALL
+    return hipErrorInvalidValue;
+}
+
+int mfma_scratch_bytes(int fp32, int nt, int nwc) {
+    if (!mfma_nwc_ok(nt, nwc)) return -1;
+#define LRES_MFMA_CALL(r, NT)                                                                                  \
+    if (nt == NT && (fp32 != 0) == (sizeof(r) == 4))                                                           \
+        return (int)lres_mfma_part_##r##_##NT(-nwc, nullptr, nullptr, 0, nullptr, nullptr, 0, 0, 0.0, 0.0,    \
+                                              nullptr, nullptr, nullptr);
+    LRES_MFMA_PARTS(LRES_MFMA_CALL)
+#undef LRES_MFMA_CALL
+    return -1;
 }
 
 hipError_t launch_reduce(const WinDesc* wins, int nwin, int ngroup, const double* Dpart, const double* Fpart,
@@ -800,6 +811,7 @@ __global__ void __launch_bounds__(kBigThreads) k_pole_big(const WinDesc* __restr
         for (int p0 = 0; p0 < n; p0 += bmax) big_block_step<E>(A, n, p0, min(bmax, n - p0), Cl, Rl, Pl);
         for (int q = threadIdx.x; q < k * k; q += blockDim.x) {
             const int r = q / k, col = q - (q / k) * k;
+            if (r > col) continue;   // upper triangle (the partial sums are symmetric)
             const E a = A[(int64_t)(r + m) * n + col + m];
             const double gr = (double)a.re, gi = (double)a.im;
             const double dv = wt.re * gr - wt.im * gi, fv = wz.re * gr - wz.im * gi;

@@ -1,0 +1,407 @@
+// Pole solves on the matrix cores (MFMA) for windows of up to 128 functions.
+//
+// One workgroup of NWC waves handles one (window, pole group).  Per pole it
+// holds A = z S_W - H_W, padded to 16 NT x 16 NT (zero off-diagonal, 1 on the
+// padding diagonal), as NT x NT tiles of 16 x 16 in MFMA accumulator
+// registers; wave w owns the tile columns J = w, w + NWC, ...  The
+// Gauss-Jordan sweep runs in blocks of four pivots P = {4p, .., 4p + 3}.  With
+// Pi = A[P,P]^-1 and
+//     C  = A[:,P] - I[:,P],        Rt = Pi (A[P,:] + I[P,:]),
+// the single rank-4 update A <- A - C Rt is exactly the in-place Gauss-Jordan
+// step of the block: A[P,P] -> Pi, A[P,j] -> Pi A[P,j], A[i,P] -> -A[i,P] Pi,
+// A[i,j] -> A[i,j] - A[i,P] Pi A[P,j].  (Row block P: A_Pj - (A_PP - I) Pi A_Pj
+// = Pi A_Pj; column block: A_iP - A_iP (I + Pi) = -A_iP Pi; pivot block:
+// A_PP - (A_PP - I)(I + Pi) = Pi.)  On the matrix cores that is four real
+// 16x16x4 MFMAs per complex tile and block.  After ceil(n/4) blocks A holds
+// A^-1, whose T x T block (T-last order) is the pole's Green's function on T.
+// No pivoting: Im A = Im(z) S_W is positive definite for Im z > 0, so every
+// leading block (and every Schur complement block) is nonsingular.
+//
+// Per block the owners of the pivot tile column and row publish A[:,P] and
+// A[P,:] to LDS (double buffered: one barrier per block; real and imaginary
+// parts in separate arrays).  Every wave then
+// inverts the 4 x 4 pivot block on 16 lanes (lane 4i + j holds entry (i, j);
+// the pivot row, column and entry come from ds_swizzle), forms its operands C
+// and Rt from LDS and updates its tiles.
+//
+// Accumulator layouts (checked on gfx950 with one-hot operands):
+//   v_mfma_f32_16x16x4f32: lane l, register r -> row 4 (l / 16) + r, column l % 16
+//   v_mfma_f64_16x16x4f64: lane l, register r -> row l / 16 + 4 r,   column l % 16
+//   both: A operand lane l -> A[l % 16][l / 16] (16 x 4), B operand lane l -> B[l / 16][l % 16] (4 x 16).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+#include "lres_device.h"
+#include "lres_internal.h"
+
+namespace lres {
+namespace {
+
+template <typename R>
+struct Mfma;
+
+template <>
+struct Mfma<float> {
+    typedef float V __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ V mma(float a, float b, V c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int g, int r) { return 4 * g + r; }
+};
+
+template <>
+struct Mfma<double> {
+    typedef double V __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ V mma(double a, double b, V c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int g, int r) { return g + 4 * r; }
+};
+
+// ds_swizzle in bit-mask mode: source lane ((lane & AND) | OR) within each 32-lane half
+template <int PAT>
+__device__ __forceinline__ float swz(float v) {
+    return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), PAT));
+}
+
+template <int PAT>
+__device__ __forceinline__ double swz(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_ds_swizzle((int)b, PAT);
+    const int hi = __builtin_amdgcn_ds_swizzle((int)(b >> 32), PAT);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+template <int PAT, typename R>
+__device__ __forceinline__ cx<R> swzc(cx<R> v) {
+    return {swz<PAT>(v.re), swz<PAT>(v.im)};
+}
+
+// Every product-sum below is written with explicit fma and this file is compiled
+// with -ffp-contract=off: no rounding is left to the compiler's choice, so all
+// waves-per-matrix variants give bitwise the same results.
+
+// 1 / a: the hardware reciprocal (1 ulp) in fp32, a correctly rounded division in fp64
+__device__ __forceinline__ cx<float> crecip_mf(cx<float> a) {
+    const float s = __builtin_amdgcn_rcpf(fmaf(a.re, a.re, a.im * a.im));
+    return {a.re * s, -a.im * s};
+}
+
+__device__ __forceinline__ cx<double> crecip_mf(cx<double> a) {
+    const double s = 1.0 / fma(a.re, a.re, a.im * a.im);
+    return {a.re * s, -a.im * s};
+}
+
+template <typename R>
+__device__ __forceinline__ cx<R> cmul_mf(cx<R> a, cx<R> b) {
+    return {fma(a.re, b.re, -(a.im * b.im)), fma(a.re, b.im, a.im * b.re)};
+}
+
+// One in-place Gauss-Jordan pivot S of the 4 x 4 block held as lane 4i + j ->
+// entry (i, j) in every 16-lane group, with the same shifted form as the block
+// update: a_ij -= (a_iS - d_iS)(a_Sj + d_Sj) / a_SS.
+template <int S, typename R>
+__device__ __forceinline__ void pivot4(cx<R>& a, int i, int j) {
+    constexpr int kRowS = 0x1C | (S << 5);           // lane 4i + S (keep bits 4..2, or S)
+    constexpr int kColS = 0x13 | ((4 * S) << 5);     // lane 4S + j (keep bits 4, 1..0, or 4S)
+    constexpr int kPiv = 0x10 | ((5 * S) << 5);      // lane 5S     (keep bit 4, or 5S)
+    cx<R> ci = swzc<kRowS>(a);
+    cx<R> rj = swzc<kColS>(a);
+    const cx<R> piv = swzc<kPiv>(a);
+    if (i == S) ci.re -= R(1);
+    if (j == S) rj.re += R(1);
+    const cx<R> rs = cmul_mf(rj, crecip_mf(piv));
+    a.re = fma(-ci.re, rs.re, fma(ci.im, rs.im, a.re));
+    a.im = fma(-ci.re, rs.im, fma(-ci.im, rs.re, a.im));
+}
+
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+template <typename R, int NT, int NWC, int WPE>
+__global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(WPE))) k_pole_mfma(const WinDesc* __restrict__ wins,
+                                                        const int32_t* __restrict__ ids,
+                                                        const double* __restrict__ SW,
+                                                        const double* __restrict__ HW, int M, int ngroup, double c,
+                                                        double Rr, double* __restrict__ Dpart,
+                                                        double* __restrict__ Fpart) {
+    using MF = Mfma<R>;
+    using V = typename MF::V;
+    constexpr int TJ = (NT + NWC - 1) / NWC;   // tile columns per wave
+    constexpr int NR = 16 * NT;                 // padded order
+    constexpr bool kEven = NT % NWC == 0;
+    // panels of the current block, planar (real and imaginary parts apart) so that
+    // accumulator registers go to LDS and come back as MFMA operands without
+    // repacking: column panel cp[b][s][i] = A[i][4p + s], row panel rp[b][j][s] = A[4p + s][j]
+    __shared__ __attribute__((aligned(16))) R cpr[2][4][NR];
+    __shared__ __attribute__((aligned(16))) R cpi[2][4][NR];
+    __shared__ __attribute__((aligned(16))) R rpr[2][NR][4];
+    __shared__ __attribute__((aligned(16))) R rpi[2][NR][4];
+    const int w = ids[blockIdx.x];
+    const int gq = blockIdx.y;
+    const WinDesc d = wins[w];
+    const int n = d.n, m = d.m, k = d.k;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const double* S = SW + d.mat_off;
+    const double* H = HW + d.mat_off;
+    double* dp = Dpart + d.kk_off * ngroup + (int64_t)gq * k * k;
+    double* fp = Fpart + d.kk_off * ngroup + (int64_t)gq * k * k;
+    const int nst = (n + 3) >> 2;
+
+    V are[NT][TJ], aim[NT][TJ];
+    auto owned = [&](int j) __attribute__((always_inline)) { return kEven || wv + NWC * j < NT; };
+
+    // pivot columns and rows of block p = 4 JPc + pq from tile column / row JPc into buffer b
+    auto publish = [&](auto jpc, int pq, int b, int li, int lg) __attribute__((always_inline)) {
+        constexpr int JPc = decltype(jpc)::value;
+        constexpr int jc = JPc / NWC;
+        if (wv == JPc % NWC && (li >> 2) == pq) {
+            const int sc = li & 3;
+#pragma unroll
+            for (int I = 0; I < NT; ++I) {
+                if constexpr (sizeof(R) == 4) {   // rows 4 lg .. 4 lg + 3: one 16-byte store
+                    *reinterpret_cast<V*>(&cpr[b][sc][16 * I + 4 * lg]) = are[I][jc];
+                    *reinterpret_cast<V*>(&cpi[b][sc][16 * I + 4 * lg]) = aim[I][jc];
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        cpr[b][sc][16 * I + MF::row(lg, r)] = are[I][jc][r];
+                        cpi[b][sc][16 * I + MF::row(lg, r)] = aim[I][jc][r];
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < TJ; ++j) {
+            if (!owned(j)) continue;
+            const int col = 16 * (wv + NWC * j) + li;
+            if constexpr (sizeof(R) == 4) {   // lane group pq holds rows 4 pq + s in register s
+                if (lg == pq) {
+                    *reinterpret_cast<V*>(&rpr[b][col][0]) = are[JPc][j];
+                    *reinterpret_cast<V*>(&rpi[b][col][0]) = aim[JPc][j];
+                }
+            } else {                          // register pq holds row 4 pq + lg
+                const V& xr = are[JPc][j];
+                const V& xi = aim[JPc][j];
+                rpr[b][col][lg] = pq == 0 ? xr[0] : pq == 1 ? xr[1] : pq == 2 ? xr[2] : xr[3];
+                rpi[b][col][lg] = pq == 0 ? xi[0] : pq == 1 ? xi[1] : pq == 2 ? xi[2] : xi[3];
+            }
+        }
+        // distinct per tile: stops the two publish branches of a block from being
+        // tail-merged into one with a run-time tile index (which would move the
+        // accumulators to scratch memory)
+        asm volatile("; publish tile %0" ::"n"(JPc));
+    };
+
+    bool first = true;
+    for (int jp = gq; jp < M / 2; jp += ngroup) {
+        cx<double> z, wt, wz;
+        pole(jp, M, c, Rr, z, wt, wz);
+        // the order made opaque per pole: keeps the per-entry offsets and bounds
+        // tests of the load from being held in registers across the sweep
+        int nn = n, ln0 = lane;
+        asm volatile("" : "+s"(nn), "+v"(ln0));
+        const int li = ln0 & 15, lg = ln0 >> 4;
+#pragma unroll
+        for (int I = 0; I < NT; ++I) {
+#pragma unroll
+            for (int j = 0; j < TJ; ++j) {
+                const int col = 16 * (wv + NWC * j) + li;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = 16 * I + MF::row(lg, r);
+                    // every entry loaded (entry 0 stands in for the padding) and
+                    // masked arithmetically, so no branch per entry; the padding is
+                    // 0 off and 1 on the diagonal
+                    const bool in = row < nn && col < nn;
+                    const int64_t off = in ? (int64_t)row * nn + col : 0;
+                    const double f = in ? 1.0 : 0.0;
+                    const double pad = (!in && row == col) ? 1.0 : 0.0;
+                    const double s = S[off] * f;
+                    const double h = H[off] * f;
+                    are[I][j][r] = (R)(fma(z.re, s, pad) - h);
+                    aim[I][j][r] = (R)(z.im * s);
+                }
+            }
+            // this tile row's entries are formed before the next row's loads issue
+#pragma unroll
+            for (int j = 0; j < TJ; ++j) asm volatile("" : "+v"(are[I][j]), "+v"(aim[I][j]));
+        }
+        {
+            int ln = lane;
+            asm volatile("" : "+v"(ln));
+            publish(std::integral_constant<int, 0>{}, 0, 0, ln & 15, ln >> 4);
+        }
+
+        static_for<0, NT>([&](auto jpc) __attribute__((always_inline)) {
+            constexpr int JP = decltype(jpc)::value;
+            const int ppend = min(4, nst - 4 * JP);
+            for (int pp = 0; pp < ppend; ++pp) {
+                const int p = 4 * JP + pp, b = p & 1;
+                __syncthreads();
+                // lane indices made opaque per block: per-lane LDS addresses and
+                // masks are recomputed here rather than held across the sweep
+                int ln = lane;
+                asm volatile("" : "+v"(ln));
+                const int li = ln & 15, lg = ln >> 4;
+                // Pi = A[P,P]^-1 on 16 lanes (four identical copies per wave)
+                const int pi = li >> 2, pj = li & 3;
+                cx<R> a = {rpr[b][4 * p + pj][pi], rpi[b][4 * p + pj][pi]};
+                pivot4<0>(a, pi, pj);
+                pivot4<1>(a, pi, pj);
+                pivot4<2>(a, pi, pj);
+                pivot4<3>(a, pi, pj);
+                cx<R> prow[4];   // row lg of Pi
+#pragma unroll
+                for (int s = 0; s < 4; ++s) prow[s] = {__shfl(a.re, 4 * lg + s, 64), __shfl(a.im, 4 * lg + s, 64)};
+                // A operands: C[16 I + li][lg], the identity shift only in tile row JP
+                R car[NT], cai[NT];
+#pragma unroll
+                for (int I = 0; I < NT; ++I) {
+                    car[I] = cpr[b][lg][16 * I + li];
+                    cai[I] = cpi[b][lg][16 * I + li];
+                    if (I == JP && li == 4 * pp + lg) car[I] -= R(1);
+                }
+                // B operands: Rt[lg][16 J + li] = sum_s Pi[lg][s] (A[4p + s][col] + d), the
+                // identity shift only in tile column JP
+                R nre[TJ], rim[TJ], nim[TJ];
+#pragma unroll
+                for (int j = 0; j < TJ; ++j) {
+                    nre[j] = rim[j] = nim[j] = R(0);
+                    if (!owned(j)) continue;
+                    const int col = 16 * (wv + NWC * j) + li;
+                    R tr[4], ti[4];
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) {
+                        tr[s] = rpr[b][col][s];
+                        ti[s] = rpi[b][col][s];
+                    }
+                    if (j == JP / NWC && wv == JP % NWC) {
+#pragma unroll
+                        for (int s = 0; s < 4; ++s)
+                            if (li == 4 * pp + s) tr[s] += R(1);
+                    }
+                    R xr = R(0), xi = R(0);
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) {
+                        xr = fma(prow[s].re, tr[s], fma(-prow[s].im, ti[s], xr));
+                        xi = fma(prow[s].re, ti[s], fma(prow[s].im, tr[s], xi));
+                    }
+                    nre[j] = -xr;
+                    rim[j] = xi;
+                    nim[j] = -xi;
+                }
+                // A <- A - C Rt:  Re -= Cr Rr - Ci Ri,  Im -= Cr Ri + Ci Rr
+#pragma unroll
+                for (int I = 0; I < NT; ++I)
+#pragma unroll
+                    for (int j = 0; j < TJ; ++j) {
+                        if (!owned(j)) continue;
+                        are[I][j] = MF::mma(car[I], nre[j], are[I][j]);
+                        are[I][j] = MF::mma(cai[I], rim[j], are[I][j]);
+                        aim[I][j] = MF::mma(car[I], nim[j], aim[I][j]);
+                        aim[I][j] = MF::mma(cai[I], nre[j], aim[I][j]);
+                    }
+                if (p + 1 < nst) {
+                    if (pp < 3)
+                        publish(jpc, pp + 1, b ^ 1, li, lg);
+                    else
+                        publish(std::integral_constant<int, (JP + 1 < NT ? JP + 1 : JP)>{}, 0, b ^ 1, li, lg);
+                }
+            }
+        });
+        __syncthreads();   // the next pole's first publish rewrites buffer 0
+
+        // accumulate Re(w G) and Re(w z G) on T x T into this group's partial slot
+        int kk = k, mm = m, ln1 = lane;
+        asm volatile("" : "+s"(kk), "+s"(mm), "+v"(ln1));
+        const int ei = ln1 & 15, eg = ln1 >> 4;
+#pragma unroll
+        for (int I = 0; I < NT; ++I)
+#pragma unroll
+            for (int j = 0; j < TJ; ++j) {
+                if (!owned(j)) continue;
+                const int col = 16 * (wv + NWC * j) + ei - mm;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = 16 * I + MF::row(eg, r) - mm;
+                    if (row >= 0 && row <= col && col < kk) {   // upper triangle: G[T,T] is symmetric
+                        const double gr = (double)are[I][j][r], gi = (double)aim[I][j][r];
+                        const double dv = fma(wt.re, gr, -(wt.im * gi)), fv = fma(wz.re, gr, -(wz.im * gi));
+                        const int64_t q = (int64_t)row * kk + col;
+                        if (first) {
+                            dp[q] = dv;
+                            fp[q] = fv;
+                        } else {
+                            dp[q] += dv;
+                            fp[q] += fv;
+                        }
+                    }
+                }
+            }
+        first = false;
+    }
+}
+
+// Waves per SIMD the register allocation is held to: two when the accumulators
+// of a wave leave room for the load prologue, else one.
+template <typename R, int NT, int NWC>
+constexpr int mfma_waves() {
+    return NT * ((NT + NWC - 1) / NWC) * (sizeof(R) == 4 ? 8 : 16) <= 160 ? 2 : 1;
+}
+
+template <typename R, int NT, int NWC>
+void launch_one(const WinDesc* wins, const int32_t* ids, int nids, const double* SW, const double* HW, int M,
+                int ngroup, double c, double Rr, double* Dpart, double* Fpart, hipStream_t st) {
+    hipLaunchKernelGGL((k_pole_mfma<R, NT, NWC, mfma_waves<R, NT, NWC>()>), dim3(nids, ngroup), dim3(64 * NWC), 0,
+                       st, wins, ids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart);
+}
+
+// private (scratch) bytes per lane of a variant: nonzero means register spills
+template <typename R, int NT, int NWC>
+int scratch_one() {
+    hipFuncAttributes a;
+    if (hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_pole_mfma<R, NT, NWC, mfma_waves<R, NT, NWC>()>)) !=
+        hipSuccess)
+        return -1;
+    return (int)a.localSizeBytes;
+}
+
+}  // namespace
+}  // namespace lres
+
+// One object per (precision, NT): the build compiles this file once per part
+// with LRES_MFMA_R / LRES_MFMA_NT set (in parallel), and the dispatcher in
+// lres_kernels.hip calls the parts by name.
+#if defined(LRES_MFMA_R) && defined(LRES_MFMA_NT)
+#define LRES_MFMA_NAME3(a, b, c) a##b##_##c
+#define LRES_MFMA_NAME(r, nt) LRES_MFMA_NAME3(lres_mfma_part_, r, nt)
+#define LRES_MFMA_FOR_NWC(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8)
+// nwc > 0: launch with nwc waves per (window, pole group); nwc < 0: return the
+// scratch bytes per lane of the -nwc variant (-1 if it does not exist)
+hipError_t LRES_MFMA_NAME(LRES_MFMA_R, LRES_MFMA_NT)(int nwc, const lres::WinDesc* wins, const int32_t* ids, int nids,
+                                                     const double* SW, const double* HW, int M, int ngroup, double c,
+                                                     double Rr, double* Dpart, double* Fpart, hipStream_t st) {
+    using R = LRES_MFMA_R;
+    constexpr int NT = LRES_MFMA_NT;
+#define LRES_MFMA_CASE(W)                                                                                     \
+    if constexpr (W <= NT) {                                                                                  \
+        if (nwc == W) {                                                                                       \
+            lres::launch_one<R, NT, W>(wins, ids, nids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart, st);          \
+            return hipGetLastError();                                                                         \
+        }                                                                                                     \
+        if (nwc == -W) return (hipError_t)lres::scratch_one<R, NT, W>();                                      \
+    }
+    LRES_MFMA_FOR_NWC(LRES_MFMA_CASE)
+#undef LRES_MFMA_CASE
+    return nwc < 0 ? (hipError_t)-1 : hipErrorInvalidValue;
+}
+#endif

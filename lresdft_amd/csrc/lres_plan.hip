@@ -59,13 +59,55 @@ struct DevBuf {
 
 constexpr int kMaxRegisterN = 128;    // register-resident Gauss-Jordan up to n = 128
 
-// Kernel shape of one bucket: T x T tile per thread on a G x G thread grid.
+// Kernel shape of one bucket.  wide 0 / 1: vector-ALU Gauss-Jordan, T x T tile
+// per thread on a G x G thread grid, register budget 128 / 512; wide 2: matrix
+// cores, T = 16 x 16 tiles per dimension, G = waves per (window, pole group).
 struct Shape {
     int T = 0, G = 0, wide = 0;
 };
+constexpr int kMfma = 2;
+
+// Which pole kernel family runs: complex64 passes on the matrix cores, fp64
+// passes on the vector ALUs (measured faster there: the fp64 MFMA rate equals
+// the fp64 vector rate, and its accumulators cost twice the registers).
+// LRES_POLE_KERNEL=mfma or =valu selects one family for both precisions.  One
+// family per bucket and precision: the autotuner only chooses among variants
+// that give bitwise identical results.
+bool use_mfma(int fp32) {
+    const char* e = getenv("LRES_POLE_KERNEL");
+    if (e && std::string(e) == "valu") return false;
+    if (e && std::string(e) == "mfma") return true;
+    return fp32 != 0;
+}
+
+std::vector<Shape> mfma_shapes(int nmax, int fp32) {
+    std::vector<Shape> out;
+    const int nt = (nmax + 15) / 16;
+    if (nt < 2 || nt > 8) return out;
+    // LRES_MFMA_WAVES=w restricts the candidates to w waves per (window, pole
+    // group), spills allowed (used to check that every variant gives the same bits)
+    const char* e = getenv("LRES_MFMA_WAVES");
+    const int force = e ? atoi(e) : 0;
+    // variants without register spills; if every variant spills, the one that
+    // spills least, provided that is a few registers (at most 256 bytes per lane)
+    int least = -1, least_bytes = 1 << 30;
+    for (int nwc = 1; nwc <= 8; ++nwc) {
+        if (nwc > nt || (force > 0 && nwc != force)) continue;
+        const int scratch = mfma_scratch_bytes(fp32, nt, nwc);
+        if (scratch < 0) continue;
+        if (scratch == 0 || force > 0) out.push_back({nt, nwc, kMfma});
+        if (scratch < least_bytes) least = nwc, least_bytes = scratch;
+    }
+    if (out.empty() && least > 0 && least_bytes <= 256) out.push_back({nt, least, kMfma});
+    return out;
+}
 
 // Valid pole-kernel shapes for windows of at most nmax functions.
-std::vector<Shape> pole_shapes(int nmax) {
+std::vector<Shape> pole_shapes(int nmax, int fp32) {
+    if (use_mfma(fp32)) {
+        auto m = mfma_shapes(nmax, fp32);
+        if (!m.empty()) return m;
+    }
     std::vector<Shape> out;
     for (int T = 2; T <= 6; ++T) {
         const int G = (nmax + T - 1) / T;
@@ -76,11 +118,19 @@ std::vector<Shape> pole_shapes(int nmax) {
     return out;
 }
 
-Shape default_pole_shape(int nmax) {
-    const auto v = pole_shapes(nmax);
+Shape default_pole_shape(int nmax, int fp32) {
+    const auto v = pole_shapes(nmax, fp32);
     for (const auto& s : v)
-        if (s.T == 4 && s.wide == 0) return s;
+        if ((s.wide == kMfma && s.G == 2) || (s.wide == 0 && s.T == 4)) return s;
     return v.empty() ? Shape{} : v.front();
+}
+
+hipError_t launch_pole(const Shape& sh, int fp32, const WinDesc* wins, const int32_t* ids, int nids, const double* SW,
+                       const double* HW, int M, int ngroup, double c, double Rr, double* Dpart, double* Fpart,
+                       hipStream_t st) {
+    if (sh.wide == kMfma)
+        return launch_pole_mfma(fp32, sh.T, sh.G, wins, ids, nids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart, st);
+    return launch_pole_gj(sh.T, sh.G, sh.wide, fp32, wins, ids, nids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart, st);
 }
 
 Shape inertia_shape(int nmax) {
@@ -183,9 +233,9 @@ void gather_h(lres_plan* P, const int64_t* Hp, const int64_t* Hi, const double* 
 // after one warm-up, HIP events) and keep the fastest.  All shapes apply the
 // same operations to every matrix entry, so the choice never changes results.
 void tune_bucket(lres_plan* P, lres_plan::Bucket& b, int fp32, double c, double R) {
-    const auto cands = pole_shapes(b.nmax);
+    const auto cands = pole_shapes(b.nmax, fp32);
     require(!cands.empty(), "no kernel shape for windows of " + std::to_string(b.nmax) + " functions");
-    b.shape[fp32] = default_pole_shape(b.nmax);
+    b.shape[fp32] = default_pole_shape(b.nmax, fp32);
     b.tuned[fp32] = true;
     if (!P->autotune || cands.size() == 1) return;
     hipEvent_t e0, e1;
@@ -194,9 +244,8 @@ void tune_bucket(lres_plan* P, lres_plan::Bucket& b, int fp32, double c, double 
     float best = 1e30f;
     for (const auto& sh : cands) {
         auto launch = [&] {
-            hip_check(launch_pole_gj(sh.T, sh.G, sh.wide, fp32, P->d_wins.p, P->d_bucket_ids.p + b.id_off, (int)b.ids.size(),
-                                     P->d_SW.p, P->d_HW.p, P->M, P->ngroup, c, R, P->d_Dpart.p, P->d_Fpart.p,
-                                     P->st),
+            hip_check(launch_pole(sh, fp32, P->d_wins.p, P->d_bucket_ids.p + b.id_off, (int)b.ids.size(), P->d_SW.p,
+                                  P->d_HW.p, P->M, P->ngroup, c, R, P->d_Dpart.p, P->d_Fpart.p, P->st),
                       "pole kernel (tuning)");
         };
         launch();
@@ -208,8 +257,9 @@ void tune_bucket(lres_plan* P, lres_plan::Bucket& b, int fp32, double c, double 
         float ms = 0;
         hip_check(hipEventElapsedTime(&ms, e0, e1), "event time");
         if (P->tune_log)
-            fprintf(stderr, "[lres tune] nmax=%d windows=%zu %s T=%d G=%d budget=%s: %.4f ms/launch\n", b.nmax,
-                    b.ids.size(), fp32 ? "fp32" : "fp64", sh.T, sh.G, sh.wide ? "512" : "128", ms / 2);
+            fprintf(stderr, "[lres tune] nmax=%d windows=%zu %s %s T=%d G=%d: %.4f ms/launch\n", b.nmax,
+                    b.ids.size(), fp32 ? "fp32" : "fp64",
+                    sh.wide == kMfma ? "mfma" : (sh.wide ? "valu512" : "valu128"), sh.T, sh.G, ms / 2);
         if (ms < best) {
             best = ms;
             b.shape[fp32] = sh;
@@ -235,9 +285,8 @@ void run_pass(lres_plan* P, double mu, double e_lo, double* d_dens, double* d_NB
             continue;
         }
         if (!b.tuned[fp32]) tune_bucket(P, b, fp32, c, R);
-        hip_check(launch_pole_gj(b.shape[fp32].T, b.shape[fp32].G, b.shape[fp32].wide, fp32, P->d_wins.p, P->d_bucket_ids.p + b.id_off,
-                                 (int)b.ids.size(), P->d_SW.p, P->d_HW.p, P->M, P->ngroup, c, R, P->d_Dpart.p,
-                                 P->d_Fpart.p, P->st),
+        hip_check(launch_pole(b.shape[fp32], fp32, P->d_wins.p, P->d_bucket_ids.p + b.id_off, (int)b.ids.size(),
+                              P->d_SW.p, P->d_HW.p, P->M, P->ngroup, c, R, P->d_Dpart.p, P->d_Fpart.p, P->st),
                   "pole kernel");
     }
     hip_check(hipEventRecord(P->ev[3], P->st), "event");
@@ -570,7 +619,8 @@ int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, 
                 }
             } else {
                 b.ishape = inertia_shape(b.nmax);
-                require(!pole_shapes(b.nmax).empty(), "no kernel shape for windows of " + std::to_string(b.nmax));
+                require(!pole_shapes(b.nmax, 0).empty() && !pole_shapes(b.nmax, 1).empty(),
+                        "no kernel shape for windows of " + std::to_string(b.nmax));
             }
         }
         P->d_ws_off.upload(ws_off.data(), nwin, P->st);

@@ -264,11 +264,11 @@ class LRFTLCAOCpp:
     # fp64: mu_ftol N (default 1e-11 N, which reproduces the reference's brentq
     # roots closely enough for 1e-10 Ha/electron SCF parity; the reference itself
     # keeps any mu with |count| <= 1e-8 N, lcao3d.py:412, 428); complex64 passes
-    # carry ~1e-8 N of rounding noise, so their search stops at 1e-7 N and
-    # refine() finishes the root in fp64.  Brent's xtol is the reference's
-    # (brentq xtol=1e-12).
+    # carry up to ~5e-7 N of rounding noise in the count (measured at 54 and 1024
+    # electrons), so their search stops at 1e-6 N and refine() finishes the root
+    # in fp64.  Brent's xtol is the reference's (brentq xtol=1e-12).
     def _ftol(self, N):
-        base = self.mu_ftol if self.precision == "fp64" else 1e-7
+        base = self.mu_ftol if self.precision == "fp64" else 1e-6
         # an SCF driver may allow a looser root while its density residual is large
         # (count_tol, relative to N); the converged step is polished by refine()
         return max(base, self.count_tol or 0.0) * N

@@ -75,7 +75,7 @@ class GridOps:
         return ex + torch.where(high, ec_h, ec_l), vx + torch.where(high, vc_h, vc_l)
 
 
-def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion=None, dens0=None, fp32_floor=1e-6):
+def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion=None, dens0=None, fp32_floor=1e-5):
     """Device SCF; ``solver.density_device(v)`` returns (density tensor [ng], e_band, mu).
 
     A solver with ``count_tol``/``refine`` (``LRFTLCAOCpp``) solves mu each step
@@ -83,7 +83,8 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
     [1e-11, 1e-5] N), and the converged step is finished in fp64 to the strict
     tolerance (``solver.refine()``).  With mixed precision the iterations run on
     complex64 passes until the residual falls below ``fp32_floor`` (then fp64
-    passes).  The energy and the returned density are always fp64 results at
+    passes): complex64 densities differ from fp64 by ~2e-6 in this residual's
+    norm, so the default floor 1e-5 hands over before that noise stalls DIIS.  The energy and the returned density are always fp64 results at
     the converged input density."""
     grid = GridOps(cr) if grid is None else grid
     torch = grid.torch
@@ -114,8 +115,8 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
             converged = True
             break
         if err < fp32_floor and getattr(solver, "precision", "fp64") == "mixed":
-            # complex64 densities carry ~1e-7 relative noise: below this residual the
-            # remaining iterations use fp64 passes (the converged step is fp64 anyway)
+            # below this residual the complex64 noise would dominate: the remaining
+            # iterations use fp64 passes (the converged step is fp64 anyway)
             solver.set_precision("fp64")
             handed_over = True
         hist_n.append(dens.clone())

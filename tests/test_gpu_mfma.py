@@ -1,0 +1,132 @@
+"""Matrix-core (MFMA) pole kernel: parity with the vector-ALU kernel and the oracle.
+
+Bars:
+  fp64 MFMA pass vs the oracle pole pass and vs the vector-ALU kernel: density,
+  N_B and E_B max-abs 1e-12 (same bar as the vector-ALU kernel's own parity);
+  fp32 MFMA pass vs the oracle: density max-abs 2e-5 and sum N_B within 1e-5 N
+  (complex64 arithmetic over a window of 84 functions);
+  every waves-per-matrix variant gives bitwise the same pass (the autotuner's
+  choice never changes results);
+  uneven windows (random centres, n_W 36..96 in one plan, six buckets) vs the
+  oracle at 1e-12 in fp64 (at most one bucket may fall back to the vector-ALU
+  kernel when every MFMA variant of its size spills) and at the fp32 bar in
+  complex64 (all buckets on the matrix cores).
+fp64 passes run on the vector ALUs by default; these tests force the matrix
+cores with LRES_POLE_KERNEL=mfma.
+"""
+import os
+
+import numpy as np
+import pytest
+from scipy import linalg
+
+from oracle.lres_oracle import BASIS, LCAOBasis, build_windows, cell_centres, guess_v, h2_crystal, pole_window_pass
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def sys3():
+    cr = h2_crystal((3, 3, 3), a=6.0, pts_per_cell=10, rc=0.6)
+    b = LCAOBasis(cr, **BASIS)
+    return cr, b, guess_v(cr)
+
+
+def _mu_window(H, S, W, margin=0.3, above=0.35):
+    lam_min = min(linalg.eigh(H[w][:, w].toarray(), S[w][:, w].toarray(), eigvals_only=True)[0] for w in W)
+    return lam_min - margin, lam_min + above
+
+
+def _pass(b, cells, H, mu, e_lo, M=32, precision="fp64", Rw=8.0, env=None):
+    from lresdft_amd.lcao3d import LRFTLCAOCpp, csr64
+    old = {k: os.environ.get(k) for k in (env or {})}
+    os.environ.update(env or {})
+    try:
+        C = LRFTLCAOCpp(b, cells, R_w=Rw, M=M, precision=precision)
+        C.plan.set_h(csr64(H))
+        nsum, esum = C.plan.run(mu, e_lo, want_density=True)
+        out = (C.plan.dens.cpu().numpy().copy(), C.plan.NB.cpu().numpy().copy(), C.plan.EB.cpu().numpy().copy(),
+               nsum, esum, C.plan.buckets())
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    return out
+
+
+def test_mfma_fp64_matches_valu_and_oracle(sys3):
+    cr, b, v = sys3
+    print("[gpu] mfma fp64 vs valu vs oracle, n=3 R_w=8 M=32", flush=True)
+    cells = cell_centres((3, 3, 3), 6.0)
+    H = b.H(v)
+    W, own = build_windows(b, cells, 8.0)
+    e_lo, mu = _mu_window(H, b.S, W)
+    dm, NBm, EBm, _, _, bk = _pass(b, cells, H, mu, e_lo, env={"LRES_POLE_KERNEL": "mfma"})
+    assert all(row[4] == 2 for row in bk), bk          # the matrix-core kernel ran
+    dv, NBv, EBv, _, _, bkv = _pass(b, cells, H, mu, e_lo, env={"LRES_POLE_KERNEL": "valu"})
+    assert all(row[4] != 2 for row in bkv), bkv
+    dp, NBo, EBo = pole_window_pass(H, b.S, b.Xr, W, own, cr.dV, mu, e_lo, 32, True)
+    assert np.max(np.abs(dm - dp)) < 1e-12
+    assert np.max(np.abs(NBm - NBo)) < 1e-12 and np.max(np.abs(EBm - EBo)) < 1e-12
+    assert np.max(np.abs(dm - dv)) < 1e-12
+    assert np.max(np.abs(NBm - NBv)) < 1e-12 and np.max(np.abs(EBm - EBv)) < 1e-12
+
+
+def test_mfma_fp32_error_is_fp32_level(sys3):
+    cr, b, v = sys3
+    print("[gpu] mfma fp32 vs oracle, n=3 R_w=8 M=32", flush=True)
+    cells = cell_centres((3, 3, 3), 6.0)
+    H = b.H(v)
+    W, own = build_windows(b, cells, 8.0)
+    e_lo, mu = _mu_window(H, b.S, W)
+    d32, NB32, EB32, n32, e32, bk = _pass(b, cells, H, mu, e_lo, precision="mixed")
+    assert all(row[7] == 2 for row in bk), bk
+    dp, NBo, EBo = pole_window_pass(H, b.S, b.Xr, W, own, cr.dV, mu, e_lo, 32, True)
+    assert np.max(np.abs(d32 - dp)) < 2e-5
+    assert abs(n32 - NBo.sum()) < 1e-5 * cr.n_elec
+    assert np.max(np.abs(d32 - dp)) > 0.0      # really the complex64 path
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_every_wave_split_gives_the_same_bits(sys3, precision):
+    cr, b, v = sys3
+    print(f"[gpu] mfma wave splits {precision}", flush=True)
+    cells = cell_centres((3, 3, 3), 6.0)
+    H = b.H(v)
+    W, _ = build_windows(b, cells, 8.0)
+    e_lo, mu = _mu_window(H, b.S, W)
+    ref = None
+    for nwc in (1, 2, 3, 4, 6):
+        d, NB, EB, ns, es, bk = _pass(b, cells, H, mu, e_lo, precision=precision,
+                                      env={"LRES_MFMA_WAVES": str(nwc), "LRES_POLE_KERNEL": "mfma"})
+        col = 3 if precision == "fp64" else 6
+        assert all(row[col] == nwc for row in bk), (nwc, bk)
+        if ref is None:
+            ref = (d, NB, EB, ns, es)
+        else:
+            assert np.array_equal(d, ref[0]) and np.array_equal(NB, ref[1]) and np.array_equal(EB, ref[2]), nwc
+            assert ns == ref[3] and es == ref[4]
+
+
+def test_mfma_uneven_windows_match_oracle():
+    """Random centres: windows of 36 to 96 functions, several buckets and tile counts."""
+    from lresdft_amd import system
+    cr, centres = system.random_h2_box(27, 6.0 * 3, dmin=3.0, seed=7, dx=0.6, rc=0.6)
+    b = LCAOBasis(cr, **BASIS)
+    v = guess_v(cr)
+    H = b.H(v)
+    W, own = build_windows(b, centres, 6.0)
+    sizes = np.array([len(w) for w in W])
+    print(f"[gpu] mfma uneven windows n_W {sizes.min()}..{sizes.max()}", flush=True)
+    assert sizes.max() <= 128 and sizes.min() < 64
+    e_lo, mu = _mu_window(H, b.S, W)
+    dp, NBo, EBo = pole_window_pass(H, b.S, b.Xr, W, own, cr.dV, mu, e_lo, 32, True)
+    d, NB, EB, _, _, bk = _pass(b, centres, H, mu, e_lo, Rw=6.0, env={"LRES_POLE_KERNEL": "mfma"})
+    assert len(bk) > 1 and sum(row[4] == 2 for row in bk) >= len(bk) - 1, bk
+    assert np.max(np.abs(d - dp)) < 1e-12
+    assert np.max(np.abs(NB - NBo)) < 1e-12 and np.max(np.abs(EB - EBo)) < 1e-12
+    d32, _, _, n32, _, bk32 = _pass(b, centres, H, mu, e_lo, Rw=6.0, precision="mixed")
+    assert all(row[7] == 2 for row in bk32), bk32
+    assert np.max(np.abs(d32 - dp)) < 2e-5 and abs(n32 - NBo.sum()) < 1e-5 * cr.n_elec
