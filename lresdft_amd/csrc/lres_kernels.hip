@@ -539,6 +539,27 @@ hipError_t launch_density(const WinDesc* wins, int nwin, int kmax, const double*
     return hipGetLastError();
 }
 
+// per window the smallest Rayleigh quotient of a basis function, min_a H_aa / S_aa
+// (an upper bound of the window's lowest generalized eigenvalue)
+__global__ void __launch_bounds__(64) k_diag_ratio(const WinDesc* __restrict__ wins, const double* __restrict__ SW,
+                                                   const double* __restrict__ HW, double* __restrict__ out) {
+    const int w = blockIdx.x;
+    const WinDesc d = wins[w];
+    double v = 1.0 / 0.0;
+    for (int a = threadIdx.x; a < d.n; a += 64) {
+        const int64_t q = d.mat_off + (int64_t)a * d.n + a;
+        v = fmin(v, HW[q] / SW[q]);
+    }
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_down(v, o, 64));
+    if (threadIdx.x == 0) out[w] = v;
+}
+
+hipError_t launch_diag_ratio(const WinDesc* wins, int nwin, const double* SW, const double* HW, double* out,
+                             hipStream_t st) {
+    hipLaunchKernelGGL(k_diag_ratio, dim3(nwin), dim3(64), 0, st, wins, SW, HW, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_sum2(const double* a, const double* b, int n, double* out, hipStream_t st) {
     hipLaunchKernelGGL(k_sum2, dim3(1), dim3(256), 0, st, a, b, n, out);
     return hipGetLastError();

@@ -180,6 +180,9 @@ struct lres_plan {
     DevBuf<double> d_Hv;
     DevBuf<double> d_sigma;
     DevBuf<unsigned long long> d_itot;
+    DevBuf<int32_t> d_cand, d_icnt;              // lambda_min: candidate windows, per-window counts
+    DevBuf<double> d_rq;                         // lambda_min: per-window Rayleigh bound
+    std::vector<int32_t> bucket_of;              // window -> bucket index
     // H(v) assembly on the device (lres_plan_set_basis / lres_set_potential)
     bool have_basis = false;
     int nf_max = 0;
@@ -313,22 +316,40 @@ void run_pass(lres_plan* P, double mu, double e_lo, double* d_dens, double* d_NB
     }
 }
 
-void inertia(lres_plan* P, const double* sig, int ns, int64_t* h_total, int32_t* d_counts) {
+// Sylvester inertia counts #{eigenvalues of (H_W, S_W) below sigma} summed over
+// windows, at ns shifts; per-window counts into d_counts[w * ns + s] if given.
+// cand (optional): only these windows (host list of window indices).
+void inertia(lres_plan* P, const double* sig, int ns, int64_t* h_total, int32_t* d_counts,
+             const std::vector<int32_t>* cand = nullptr) {
     require(P->have_H, "no Hamiltonian loaded");
     require(ns > 0 && ns <= 1024, "nshift must be in 1..1024");
     P->d_sigma.upload(sig, ns, P->st);
     if ((int)P->d_itot.n < ns) P->d_itot.alloc(ns);
     hip_check(hipMemsetAsync(P->d_itot.p, 0, ns * sizeof(unsigned long long), P->st), "memset");
-    for (const auto& b : P->buckets) {
+    std::vector<int32_t> sub;        // candidate ids grouped by bucket
+    std::vector<int64_t> sub_off;    // per bucket: first entry in sub
+    if (cand) {
+        for (const auto& b : P->buckets) {
+            sub_off.push_back((int64_t)sub.size());
+            for (int32_t w : *cand)
+                if (P->bucket_of[w] == (int32_t)(&b - P->buckets.data())) sub.push_back(w);
+        }
+        sub_off.push_back((int64_t)sub.size());
+        if (sub.empty()) sub.push_back(0);   // keeps the upload non-empty; no launch reads it
+        P->d_cand.upload(sub.data(), sub.size(), P->st);
+    }
+    for (size_t bi = 0; bi < P->buckets.size(); ++bi) {
+        const auto& b = P->buckets[bi];
+        const int32_t* ids = cand ? P->d_cand.p + sub_off[bi] : P->d_bucket_ids.p + b.id_off;
+        const int nids = cand ? (int)(sub_off[bi + 1] - sub_off[bi]) : (int)b.ids.size();
+        if (nids == 0) continue;
         if (b.big)
-            hip_check(launch_inertia_big(b.bmax, b.nmax, P->d_wins.p, P->d_bucket_ids.p + b.id_off, (int)b.ids.size(),
-                                         P->d_SW.p, P->d_HW.p, P->d_sigma.p, ns, P->d_work.p, P->d_ws_off.p,
-                                         P->ngroup, d_counts, P->d_itot.p, P->st),
+            hip_check(launch_inertia_big(b.bmax, b.nmax, P->d_wins.p, ids, nids, P->d_SW.p, P->d_HW.p, P->d_sigma.p,
+                                         ns, P->d_work.p, P->d_ws_off.p, P->ngroup, d_counts, P->d_itot.p, P->st),
                       "blocked inertia kernel");
         else
-            hip_check(launch_inertia(b.ishape.T, b.ishape.G, P->d_wins.p, P->d_bucket_ids.p + b.id_off,
-                                     (int)b.ids.size(), P->d_SW.p, P->d_HW.p, P->d_sigma.p, ns, d_counts,
-                                     P->d_itot.p, P->st),
+            hip_check(launch_inertia(b.ishape.T, b.ishape.G, P->d_wins.p, ids, nids, P->d_SW.p, P->d_HW.p,
+                                     P->d_sigma.p, ns, d_counts, P->d_itot.p, P->st),
                       "inertia kernel");
     }
     std::vector<unsigned long long> tot(ns);
@@ -603,7 +624,9 @@ int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, 
         std::vector<int32_t> ids;
         std::vector<int64_t> ws_off(nwin, 0);
         int64_t ws_total = 0;
+        P->bucket_of.assign(nwin, -1);
         for (auto& b : P->buckets) {
+            for (int32_t w : b.ids) P->bucket_of[w] = (int32_t)(&b - P->buckets.data());
             b.id_off = (int64_t)ids.size();
             ids.insert(ids.end(), b.ids.begin(), b.ids.end());
             b.big = b.nmax > kMaxRegisterN;
@@ -925,17 +948,15 @@ int lres_lambda_min(lres_plan* P, double tol, double* out) {
     return guarded([&] {
         require(P != nullptr, "null plan");
         require(tol > 0, "tol must be positive");
+        require(P->have_H, "no Hamiltonian loaded");
+        const int nwin = (int)P->nwin;
         // upper bound: the smallest Rayleigh quotient H_aa / S_aa over all windows
-        std::vector<double> hw(P->d_HW.n), sw(P->d_SW.n);
-        hip_check(hipMemcpyAsync(hw.data(), P->d_HW.p, hw.size() * 8, hipMemcpyDeviceToHost, P->st), "copy");
-        hip_check(hipMemcpyAsync(sw.data(), P->d_SW.p, sw.size() * 8, hipMemcpyDeviceToHost, P->st), "copy");
+        if ((int64_t)P->d_rq.n < nwin) P->d_rq.alloc(nwin);
+        hip_check(launch_diag_ratio(P->d_wins.p, nwin, P->d_SW.p, P->d_HW.p, P->d_rq.p, P->st), "diag kernel");
+        std::vector<double> rq(nwin);
+        hip_check(hipMemcpyAsync(rq.data(), P->d_rq.p, nwin * sizeof(double), hipMemcpyDeviceToHost, P->st), "copy");
         hip_check(hipStreamSynchronize(P->st), "sync");
-        double hi = std::numeric_limits<double>::infinity();
-        for (const auto& d : P->wins)
-            for (int a = 0; a < d.n; ++a) {
-                const size_t q = d.mat_off + (size_t)a * d.n + a;
-                hi = std::min(hi, hw[q] / sw[q]);
-            }
+        double hi = *std::min_element(rq.begin(), rq.end());
         hi += 1e-12 * (1.0 + std::fabs(hi));
         int64_t cnt = 0;
         inertia(P, &hi, 1, &cnt, nullptr);
@@ -949,17 +970,33 @@ int lres_lambda_min(lres_plan* P, double tol, double* out) {
             step *= 2.0;
             lo -= step;
         }
-        // multisection with 15 interior shifts per launch
+        // multisection with 15 interior shifts per launch.  Once a shift with a
+        // nonzero total is known, only the windows counting an eigenvalue below it
+        // can hold the minimum: later rounds factor those windows alone.
         const int K = 15;
         std::vector<double> sg(K);
         std::vector<int64_t> tot(K);
+        std::vector<int32_t> cand;
+        std::vector<int32_t> cnt_w;
+        if ((int64_t)P->d_icnt.n < (int64_t)nwin * K) P->d_icnt.alloc((size_t)nwin * K);
+        bool restricted = false;
         while (hi - lo > tol) {
             for (int s = 0; s < K; ++s) sg[s] = lo + (hi - lo) * (s + 1) / (K + 1);
-            inertia(P, sg.data(), K, tot.data(), nullptr);
+            inertia(P, sg.data(), K, tot.data(), restricted ? nullptr : P->d_icnt.p, restricted ? &cand : nullptr);
             double nlo = lo, nhi = hi;
+            int first_pos = -1;
             for (int s = 0; s < K; ++s) {
                 if (tot[s] == 0) nlo = sg[s];
-                else { nhi = sg[s]; break; }
+                else { nhi = sg[s]; first_pos = s; break; }
+            }
+            if (!restricted && first_pos >= 0) {
+                cnt_w.resize((size_t)nwin * K);
+                hip_check(hipMemcpyAsync(cnt_w.data(), P->d_icnt.p, cnt_w.size() * sizeof(int32_t),
+                                         hipMemcpyDeviceToHost, P->st), "copy");
+                hip_check(hipStreamSynchronize(P->st), "sync");
+                for (int w = 0; w < nwin; ++w)
+                    if (cnt_w[(size_t)w * K + first_pos] > 0) cand.push_back(w);
+                restricted = !cand.empty();
             }
             if (nlo == lo && nhi == hi) break;   // no progress at double resolution
             lo = nlo;
