@@ -163,6 +163,35 @@ int lres_plan_stats(lres_plan* plan, double* h_out4, int reset);
  * on the plan's stream: {gather (set_hamiltonian), pole solves, reduction}. */
 int lres_last_timing(const lres_plan* plan, double* h_ms3);
 
+/* Layout of the plan's per-window blocks (H_W, S_W and the adjoint's
+ * gradients): window w occupies doubles [h_mat_off[w], h_mat_off[w + 1]) as an
+ * n_W x n_W row-major matrix in "T-last" order; h_fn (optional, length
+ * sum n_W, window w from Woff[w] - Woff[0]) gives the global basis function at
+ * each position.  Replaces the window lists of LRFTLCAO.__init__
+ * (lrft/lcao3d.py:155) for callers that map block gradients back to H and S. */
+int lres_plan_layout(const lres_plan* plan, int64_t* h_mat_off, int64_t* h_fn);
+
+/* Replace the plan's window blocks H_W and/or S_W (device, the lres_plan_layout
+ * layout; NULL keeps a block), or copy them out.  For differentiating through
+ * the window step; no counterpart in the reference. */
+int lres_set_window_blocks(lres_plan* plan, const double* d_HW, const double* d_SW);
+int lres_get_window_blocks(const lres_plan* plan, double* d_HW, double* d_SW);
+
+/* Adjoint of the window step at the fixed circle (mu, e_lo), fp64: the
+ * gradients of sum_B (alpha E_B + beta N_B) with respect to every window's H_W
+ * and S_W, written to d_gH and d_gS (device, lres_plan_layout layout, no spin
+ * factor):
+ *   dH_W:  2 Re sum_j w_j p(z_j)     G_j[:,T] Q_B G_j[T,:]
+ *   dS_W: -2 Re sum_j w_j z_j p(z_j) G_j[:,T] Q_B G_j[T,:],   p(z) = alpha z + beta,
+ * G_j = (z_j S_W - H_W)^-1 (lrft/core.py:130-139 poles; N_B, E_B as
+ * lres_window_pass).  alpha = 1, beta = 0 gives the band-energy gradient;
+ * alpha = 0, beta = 1 the count's.  The fixed-N derivative of the band energy
+ * (mu moving with the perturbation, e_lo fixed) is dE - (dE/dmu)/(dN/dmu) dN,
+ * i.e. alpha = 1, beta = -(dE/dmu)/(dN/dmu).  No counterpart in the reference
+ * (its 3D path has no forces, README.md:35).
+ * Allocates workspaces of up to 2 x 4 GiB (windows are processed in chunks). */
+int lres_adjoint(lres_plan* plan, double mu, double e_lo, double alpha, double beta, double* d_gH, double* d_gS);
+
 #ifdef __cplusplus
 }
 #endif

@@ -67,3 +67,9 @@ hipError_t launch_pole_mfma(int fp32, int nt, int nwc, const lres::WinDesc* wins
 int mfma_acc_regs(int fp32, int nt, int nwc);
 // scratch bytes per lane of an MFMA variant (register spills; -1: no such variant)
 int mfma_scratch_bytes(int fp32, int nt, int nwc);
+// Adjoint at a fixed circle: d/dH_W and d/dS_W of sum_B (alpha E_B + beta N_B) for
+// the windows ids[0..nids) (n <= nmax), fp64.  Workspaces Gw, Zw: woff[i] is the
+// offset (complex entries) of window ids[i], (M/2) n^2 entries each.
+hipError_t launch_adjoint(int bmax, int nmax, const lres::WinDesc* wins, const int32_t* ids, const int64_t* woff,
+                          int nids, const double* SW, const double* HW, const double* Q, int M, double c, double Rr,
+                          double alpha, double beta, void* Gw, void* Zw, double* gH, double* gS, hipStream_t st);

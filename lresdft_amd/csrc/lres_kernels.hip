@@ -916,3 +916,130 @@ hipError_t launch_inertia_big(int bmax, int nmax, const WinDesc* wins, const int
                        ws_off, std::min(ngroup, nshift), counts, total);
     return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// Adjoint of the window step at a fixed circle (mu, e_lo).  With the weight
+// p(z) = alpha z + beta per pole and G_j = (z_j S_W - H_W)^-1,
+//   d/dH_W sum_B (alpha E_B + beta N_B) =  2 Re sum_j w_j p(z_j)     G_j[:,T] Q_B G_j[T,:]
+//   d/dS_W sum_B (alpha E_B + beta N_B) = -2 Re sum_j w_j z_j p(z_j) G_j[:,T] Q_B G_j[T,:]
+// (E_B = Tr(F_B Q_B), N_B = Tr(D_B Q_B); dG = G dH G - z G dS G; G and Q are
+// symmetric).  Three kernels per chunk of windows: every G_j by the blocked
+// in-place Gauss-Jordan sweep (fp64) into a workspace, Z_j = Q_B G_j[T,:], and
+// the pole sum of G_j[:,T] Z_j accumulated tile by tile in registers, poles in
+// order (bitwise reproducible).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBigThreads) k_adj_invert(const WinDesc* __restrict__ wins,
+                                                            const int32_t* __restrict__ ids,
+                                                            const int64_t* __restrict__ woff,
+                                                            const double* __restrict__ SW,
+                                                            const double* __restrict__ HW, int M, double c,
+                                                            double Rr, int bmax, cx<double>* __restrict__ Gw) {
+    extern __shared__ double lds_raw[];
+    using E = cx<double>;
+    const int w = ids[blockIdx.x];
+    const int j = blockIdx.y;
+    const WinDesc d = wins[w];
+    const int n = d.n;
+    E* A = Gw + woff[blockIdx.x] + (int64_t)j * n * n;
+    E* Cl = reinterpret_cast<E*>(lds_raw);
+    E* Rl = Cl + (size_t)n * bmax;
+    E* Pl = Rl + (size_t)n * bmax;
+    const double* S = SW + d.mat_off;
+    const double* H = HW + d.mat_off;
+    cx<double> z, wt, wz;
+    pole(j, M, c, Rr, z, wt, wz);
+    for (int64_t q = threadIdx.x; q < (int64_t)n * n; q += blockDim.x) A[q] = {z.re * S[q] - H[q], z.im * S[q]};
+    __syncthreads();
+    for (int p0 = 0; p0 < n; p0 += bmax) big_block_step<E>(A, n, p0, min(bmax, n - p0), Cl, Rl, Pl);
+}
+
+// Z_j = Q_B G_j[T,:] (k x n) into a second workspace of G's shape
+__global__ void __launch_bounds__(256) k_adj_qg(const WinDesc* __restrict__ wins, const int32_t* __restrict__ ids,
+                                                const int64_t* __restrict__ woff, const double* __restrict__ Q,
+                                                const cx<double>* __restrict__ Gw, cx<double>* __restrict__ Zw) {
+    const int w = ids[blockIdx.x];
+    const int j = blockIdx.y;
+    const WinDesc d = wins[w];
+    const int n = d.n, m = d.m, k = d.k;
+    const cx<double>* G = Gw + woff[blockIdx.x] + (int64_t)j * n * n;
+    cx<double>* Z = Zw + woff[blockIdx.x] + (int64_t)j * n * n;
+    const double* Qb = Q + d.kk_off;
+    for (int64_t e = threadIdx.x; e < (int64_t)k * n; e += blockDim.x) {
+        const int t = (int)(e / n), b = (int)(e - (int64_t)t * n);
+        double re = 0.0, im = 0.0;
+        for (int u = 0; u < k; ++u) {
+            const double q = Qb[(int64_t)t * k + u];
+            const cx<double> g = G[(int64_t)(m + u) * n + b];
+            re = fma(q, g.re, re);
+            im = fma(q, g.im, im);
+        }
+        Z[e] = {re, im};
+    }
+}
+
+// gH, gS (n x n per window, the plan's T-last layout): one 16 x 16 output tile
+// per block, the pole sum in registers
+__global__ void __launch_bounds__(256) k_adj_accum(const WinDesc* __restrict__ wins, const int32_t* __restrict__ ids,
+                                                   const int64_t* __restrict__ woff,
+                                                   const cx<double>* __restrict__ Gw,
+                                                   const cx<double>* __restrict__ Zw, int M, double c, double Rr,
+                                                   double alpha, double beta, double* __restrict__ gH,
+                                                   double* __restrict__ gS) {
+    __shared__ cx<double> Gs[16][17];
+    __shared__ cx<double> Zs[16][17];
+    const int w = ids[blockIdx.x];
+    const WinDesc d = wins[w];
+    const int n = d.n, m = d.m, k = d.k;
+    const int nt = (n + 15) / 16;
+    const int TI = blockIdx.y / nt, TJ = blockIdx.y - (blockIdx.y / nt) * nt;
+    if (TI >= nt) return;   // uniform per block
+    const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
+    const int a = 16 * TI + ty, b = 16 * TJ + tx;
+    double sh = 0.0, ss = 0.0;
+    for (int j = 0; j < M / 2; ++j) {
+        const cx<double>* G = Gw + woff[blockIdx.x] + (int64_t)j * n * n;
+        const cx<double>* Z = Zw + woff[blockIdx.x] + (int64_t)j * n * n;
+        double yr = 0.0, yi = 0.0;
+        for (int t0 = 0; t0 < k; t0 += 16) {
+            const int tg = t0 + tx, tz = t0 + ty;
+            Gs[ty][tx] = (a < n && tg < k) ? G[(int64_t)a * n + m + tg] : cx<double>{0.0, 0.0};
+            Zs[ty][tx] = (tz < k && b < n) ? Z[(int64_t)tz * n + b] : cx<double>{0.0, 0.0};
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const cx<double> g = Gs[ty][q], zz = Zs[q][tx];
+                yr = fma(g.re, zz.re, fma(-g.im, zz.im, yr));
+                yi = fma(g.re, zz.im, fma(g.im, zz.re, yi));
+            }
+            __syncthreads();
+        }
+        cx<double> z, wt, wz;
+        pole(j, M, c, Rr, z, wt, wz);
+        // coefficients 2 w p(z) and -2 w z p(z), p(z) = alpha z + beta
+        const cx<double> p = {alpha * z.re + beta, alpha * z.im};
+        const cx<double> ch = cmul(wt, p);
+        const cx<double> cs = cmul(ch, z);
+        sh += 2.0 * (ch.re * yr - ch.im * yi);
+        ss -= 2.0 * (cs.re * yr - cs.im * yi);
+    }
+    if (a < n && b < n) {
+        gH[d.mat_off + (int64_t)a * n + b] = sh;
+        gS[d.mat_off + (int64_t)a * n + b] = ss;
+    }
+}
+
+hipError_t launch_adjoint(int bmax, int nmax, const WinDesc* wins, const int32_t* ids, const int64_t* woff, int nids,
+                          const double* SW, const double* HW, const double* Q, int M, double c, double Rr,
+                          double alpha, double beta, void* Gw, void* Zw, double* gH, double* gS, hipStream_t st) {
+    const size_t lds = big_lds_bytes(nmax, bmax, 16);
+    hipError_t e = hipFuncSetAttribute((const void*)k_adj_invert, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_adj_invert, dim3(nids, M / 2), dim3(kBigThreads), lds, st, wins, ids, woff, SW, HW, M, c, Rr,
+                       bmax, (cx<double>*)Gw);
+    hipLaunchKernelGGL(k_adj_qg, dim3(nids, M / 2), dim3(256), 0, st, wins, ids, woff, Q, (const cx<double>*)Gw,
+                       (cx<double>*)Zw);
+    const int nt = (nmax + 15) / 16;
+    hipLaunchKernelGGL(k_adj_accum, dim3(nids, nt * nt), dim3(256), 0, st, wins, ids, woff, (const cx<double>*)Gw,
+                       (const cx<double>*)Zw, M, c, Rr, alpha, beta, gH, gS);
+    return hipGetLastError();
+}

@@ -183,6 +183,10 @@ struct lres_plan {
     DevBuf<int32_t> d_cand, d_icnt;              // lambda_min: candidate windows, per-window counts
     DevBuf<double> d_rq;                         // lambda_min: per-window Rayleigh bound
     std::vector<int32_t> bucket_of;              // window -> bucket index
+    std::vector<int64_t> fn_at;                  // per window (Woff offsets): global function at each T-last position
+    DevBuf<double> d_adjG, d_adjZ;               // adjoint workspaces (complex entries as double pairs)
+    DevBuf<int32_t> d_adj_ids;
+    DevBuf<int64_t> d_adj_off;
     // H(v) assembly on the device (lres_plan_set_basis / lres_set_potential)
     bool have_basis = false;
     int nf_max = 0;
@@ -531,6 +535,7 @@ int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, 
         P->wins.resize(nwin);
         P->n_of.resize(nwin);
         P->k_of.resize(nwin);
+        P->fn_at.assign(Woff[nwin] - Woff[0], -1);
         std::vector<int32_t> pos_all(Woff[nwin]);
         std::vector<int32_t> loc(nb, -1);
         std::vector<double> SW_all, Q_all;
@@ -563,6 +568,7 @@ int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, 
             int32_t* pos = pos_all.data() + Woff[w];
             int nxt_nT = 0, nxt_T = m;
             for (int a = 0; a < n; ++a) pos[a] = inT[a] ? nxt_T++ : nxt_nT++;
+            for (int a = 0; a < n; ++a) P->fn_at[Woff[w] - Woff[0] + pos[a]] = W[a];
             // dense S_W in T-last order
             const size_t s0 = SW_all.size();
             SW_all.resize(s0 + (size_t)n * n, 0.0);
@@ -1062,6 +1068,88 @@ int lres_windows(const int64_t* Hp, const int64_t* Hi, const double* Hv, const i
     lres_plan_destroy(P);
     g_err = keep;
     return rc;
+}
+
+int lres_plan_layout(const lres_plan* P, int64_t* h_mat_off, int64_t* h_fn) {
+    return guarded([&] {
+        require(P != nullptr, "null plan");
+        for (int64_t w = 0; w < P->nwin; ++w) h_mat_off[w] = P->wins[w].mat_off;
+        h_mat_off[P->nwin] = P->nwin ? P->wins[P->nwin - 1].mat_off +
+                                           (int64_t)P->wins[P->nwin - 1].n * P->wins[P->nwin - 1].n
+                                     : 0;
+        if (h_fn) std::copy(P->fn_at.begin(), P->fn_at.end(), h_fn);
+    });
+}
+
+int lres_set_window_blocks(lres_plan* P, const double* d_HW, const double* d_SW) {
+    return guarded([&] {
+        require(P != nullptr, "null plan");
+        const size_t nn = P->d_SW.n;
+        if (d_HW) {
+            if (P->d_HW.n < nn) P->d_HW.alloc(nn);
+            hip_check(hipMemcpyAsync(P->d_HW.p, d_HW, nn * sizeof(double), hipMemcpyDeviceToDevice, P->st), "copy");
+            P->have_H = true;
+        }
+        if (d_SW) hip_check(hipMemcpyAsync(P->d_SW.p, d_SW, nn * sizeof(double), hipMemcpyDeviceToDevice, P->st), "copy");
+    });
+}
+
+int lres_get_window_blocks(const lres_plan* P, double* d_HW, double* d_SW) {
+    return guarded([&] {
+        require(P != nullptr, "null plan");
+        const size_t nn = P->d_SW.n;
+        if (d_HW) {
+            require(P->have_H, "no Hamiltonian loaded");
+            hip_check(hipMemcpyAsync(d_HW, P->d_HW.p, nn * sizeof(double), hipMemcpyDeviceToDevice, P->st), "copy");
+        }
+        if (d_SW) hip_check(hipMemcpyAsync(d_SW, P->d_SW.p, nn * sizeof(double), hipMemcpyDeviceToDevice, P->st), "copy");
+        hip_check(hipStreamSynchronize(P->st), "sync");
+    });
+}
+
+int lres_adjoint(lres_plan* P, double mu, double e_lo, double alpha, double beta, double* d_gH, double* d_gS) {
+    return guarded([&] {
+        require(P != nullptr, "null plan");
+        require(P->have_H, "no Hamiltonian loaded: call lres_set_hamiltonian_csr first");
+        require(mu > e_lo, "mu must lie above e_lo");
+        require(d_gH != nullptr && d_gS != nullptr, "null gradient buffer");
+        const double c = 0.5 * (mu + e_lo), R = 0.5 * (mu - e_lo);
+        const int64_t half = P->M / 2;
+        // windows in index order, in chunks whose two workspaces stay within the budget
+        const int64_t budget = int64_t(1) << 28;   // complex entries per workspace (4 GiB each)
+        int64_t w0 = 0;
+        while (w0 < P->nwin) {
+            std::vector<int32_t> ids;
+            std::vector<int64_t> off;
+            int64_t used = 0;
+            int nmax = 0;
+            int64_t w = w0;
+            for (; w < P->nwin; ++w) {
+                const int64_t need = half * P->n_of[w] * P->n_of[w];
+                if (!ids.empty() && used + need > budget) break;
+                ids.push_back((int32_t)w);
+                off.push_back(used);
+                used += need;
+                nmax = std::max(nmax, (int)P->n_of[w]);
+            }
+            int bmax = 16;
+            while (bmax > 4 && big_lds_bytes(nmax, bmax, 16) > 160 * 1024) bmax /= 2;
+            require(big_lds_bytes(nmax, bmax, 16) <= 160 * 1024,
+                    "window of " + std::to_string(nmax) + " functions exceeds the adjoint sweep's LDS");
+            if ((int64_t)P->d_adjG.n < 2 * used) {
+                P->d_adjG.alloc((size_t)(2 * used));
+                P->d_adjZ.alloc((size_t)(2 * used));
+            }
+            P->d_adj_ids.upload(ids.data(), ids.size(), P->st);
+            P->d_adj_off.upload(off.data(), off.size(), P->st);
+            hip_check(launch_adjoint(bmax, nmax, P->d_wins.p, P->d_adj_ids.p, P->d_adj_off.p, (int)ids.size(), P->d_SW.p,
+                                     P->d_HW.p, P->d_Q.p, P->M, c, R, alpha, beta, P->d_adjG.p, P->d_adjZ.p, d_gH,
+                                     d_gS, P->st),
+                      "adjoint kernels");
+            hip_check(hipStreamSynchronize(P->st), "sync");   // the id / offset lists are reused by the next chunk
+            w0 = w;
+        }
+    });
 }
 
 int lres_plan_stats(lres_plan* P, double* out4, int reset) {

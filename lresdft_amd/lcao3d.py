@@ -193,6 +193,47 @@ class WindowPlan:
         self.last = None
         return float(mu[0]), info
 
+    # -- differentiation (no counterpart in the reference: its 3D path has no forces) --
+    def layout(self):
+        """(mat_off [nwin+1], fn [sum n_W]): window w's block is the n_W x n_W row-major
+        matrix at doubles [mat_off[w], mat_off[w+1]) in T-last order; fn maps its
+        positions (window w from Woff[w] - Woff[0]) to global basis functions."""
+        mat_off = np.zeros(self.nwin + 1, np.int64)
+        fn = np.zeros(int(self.n.sum()), np.int64)
+        nat.check(nat.lib().lres_plan_layout(self.h, nat.p64(mat_off), nat.p64(fn)))
+        return mat_off, fn
+
+    def window_blocks(self):
+        """Copies of the current (H_W, S_W) blocks, flat device tensors in the layout order."""
+        torch = self.torch
+        nn = int((self.n.astype(np.int64) ** 2).sum())
+        HW = torch.empty(nn, dtype=torch.float64, device="cuda")
+        SW = torch.empty(nn, dtype=torch.float64, device="cuda")
+        nat.check(nat.lib().lres_get_window_blocks(self.h, nat.C.c_void_p(HW.data_ptr()),
+                                                   nat.C.c_void_p(SW.data_ptr())))
+        return HW, SW
+
+    def set_window_blocks(self, HW=None, SW=None):
+        self.last = None
+        for t in (HW, SW):
+            if t is not None:
+                assert t.is_cuda and t.dtype == self.torch.float64 and t.is_contiguous()
+        nat.check(nat.lib().lres_set_window_blocks(
+            self.h, nat.C.c_void_p(HW.data_ptr()) if HW is not None else None,
+            nat.C.c_void_p(SW.data_ptr()) if SW is not None else None))
+
+    def adjoint(self, mu, e_lo, alpha=1.0, beta=0.0):
+        """Gradients of sum_B (alpha E_B + beta N_B) (no spin factor) at the fixed circle
+        (mu, e_lo) with respect to every window's H_W and S_W: flat device tensors
+        (gH, gS) in the layout order, fp64."""
+        torch = self.torch
+        nn = int((self.n.astype(np.int64) ** 2).sum())
+        gH = torch.empty(nn, dtype=torch.float64, device="cuda")
+        gS = torch.empty(nn, dtype=torch.float64, device="cuda")
+        nat.check(nat.lib().lres_adjoint(self.h, float(mu), float(e_lo), float(alpha), float(beta),
+                                         nat.C.c_void_p(gH.data_ptr()), nat.C.c_void_p(gS.data_ptr())))
+        return gH, gS
+
     def stats(self, reset=False):
         """{fp64: (passes, kernel ms), fp32: (...)} of the pole kernel since the last reset."""
         t = np.zeros(4)
@@ -458,3 +499,36 @@ def lres_windows(Hc, S, X, Wflat, Woff, Oflat, Ooff, nb, ng, dV, mu, e_lo, M, wa
                                      float(mu), float(e_lo), int(M), nat.pf(dens) if want_density else None,
                                      nat.pf(NB), nat.pf(EB)))
     return dens, NB, EB
+
+
+class WindowBandEnergy:
+    """2 sum_B (alpha E_B + beta N_B) (band energy for alpha = 1, beta = 0; spin
+    factor 2) of a plan's windows as a differentiable function of the window
+    blocks at a fixed circle (mu, e_lo):
+    ``WindowBandEnergy.apply(plan, HW, SW, mu, e_lo[, alpha, beta])`` with HW, SW
+    flat fp64 device tensors in ``plan.layout()`` order.  Forward: one fp64 pass
+    on those blocks; backward: the adjoint kernels (2 Re sum_j w_j p(z_j) G Q G
+    and its S_W counterpart).  Built on first use (torch.autograd.Function)."""
+    _fn = None
+
+    @classmethod
+    def apply(cls, plan, HW, SW, mu, e_lo, alpha=1.0, beta=0.0):
+        if cls._fn is None:
+            torch = _torch()
+
+            class _F(torch.autograd.Function):
+                @staticmethod
+                def forward(ctx, HW, SW, plan, mu, e_lo, alpha, beta):
+                    plan.set_window_blocks(HW.detach().contiguous(), SW.detach().contiguous())
+                    nsum, esum = plan.run(mu, e_lo, want_density=False)
+                    ctx.plan, ctx.args = plan, (mu, e_lo, alpha, beta)
+                    return HW.new_tensor(2.0 * (alpha * esum + beta * nsum))
+
+                @staticmethod
+                def backward(ctx, g):
+                    mu, e_lo, alpha, beta = ctx.args
+                    gH, gS = ctx.plan.adjoint(mu, e_lo, alpha, beta)
+                    return 2.0 * g * gH, 2.0 * g * gS, None, None, None, None, None
+
+            cls._fn = _F
+        return cls._fn.apply(HW, SW, plan, float(mu), float(e_lo), float(alpha), float(beta))
