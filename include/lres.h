@@ -177,6 +177,15 @@ int lres_plan_layout(const lres_plan* plan, int64_t* h_mat_off, int64_t* h_fn);
 int lres_set_window_blocks(lres_plan* plan, const double* d_HW, const double* d_SW);
 int lres_get_window_blocks(const lres_plan* plan, double* d_HW, double* d_SW);
 
+/* D_B and F_B of the last pass (device, sum_B k_B^2 doubles: window after
+ * window, k_B x k_B row-major over T_B in ascending function order, which is
+ * positions n_W - k_B .. n_W - 1 of the lres_plan_layout order):
+ * D_B = 2 Re sum_j w_j G_j[T,T], F_B = 2 Re sum_j w_j z_j G_j[T,T].  The
+ * density-matrix blocks behind lrft/lcao3d.py:126-129's counts; with the
+ * adjoint they give the gradient with respect to Q_B (d E_B / d Q_B = F_B,
+ * d N_B / d Q_B = D_B). */
+int lres_pass_blocks(const lres_plan* plan, double* d_D, double* d_F);
+
 /* Adjoint of the window step at the fixed circle (mu, e_lo), fp64: the
  * gradients of sum_B (alpha E_B + beta N_B) with respect to every window's H_W
  * and S_W, written to d_gH and d_gS (device, lres_plan_layout layout, no spin

@@ -53,6 +53,7 @@ SIGNATURES = {
     "lres_set_window_blocks": (C.c_int, [_vp, _vp, _vp]),
     "lres_get_window_blocks": (C.c_int, [_vp, _vp, _vp]),
     "lres_adjoint": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_double, _vp, _vp]),
+    "lres_pass_blocks": (C.c_int, [_vp, _vp, _vp]),
 }
 
 _lib = None

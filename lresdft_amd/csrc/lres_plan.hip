@@ -1107,6 +1107,16 @@ int lres_get_window_blocks(const lres_plan* P, double* d_HW, double* d_SW) {
     });
 }
 
+int lres_pass_blocks(const lres_plan* P, double* d_D, double* d_F) {
+    return guarded([&] {
+        require(P != nullptr, "null plan");
+        const size_t nkk = P->d_Q.n;
+        if (d_D) hip_check(hipMemcpyAsync(d_D, P->d_D.p, nkk * sizeof(double), hipMemcpyDeviceToDevice, P->st), "copy");
+        if (d_F) hip_check(hipMemcpyAsync(d_F, P->d_F.p, nkk * sizeof(double), hipMemcpyDeviceToDevice, P->st), "copy");
+        hip_check(hipStreamSynchronize(P->st), "sync");
+    });
+}
+
 int lres_adjoint(lres_plan* P, double mu, double e_lo, double alpha, double beta, double* d_gH, double* d_gS) {
     return guarded([&] {
         require(P != nullptr, "null plan");

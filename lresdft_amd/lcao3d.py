@@ -222,6 +222,15 @@ class WindowPlan:
             self.h, nat.C.c_void_p(HW.data_ptr()) if HW is not None else None,
             nat.C.c_void_p(SW.data_ptr()) if SW is not None else None))
 
+    def pass_blocks(self):
+        """(D, F) of the last pass: flat device tensors, k_B x k_B per window in order."""
+        torch = self.torch
+        nkk = int((self.k.astype(np.int64) ** 2).sum())
+        D = torch.empty(nkk, dtype=torch.float64, device="cuda")
+        F = torch.empty(nkk, dtype=torch.float64, device="cuda")
+        nat.check(nat.lib().lres_pass_blocks(self.h, nat.C.c_void_p(D.data_ptr()), nat.C.c_void_p(F.data_ptr())))
+        return D, F
+
     def adjoint(self, mu, e_lo, alpha=1.0, beta=0.0):
         """Gradients of sum_B (alpha E_B + beta N_B) (no spin factor) at the fixed circle
         (mu, e_lo) with respect to every window's H_W and S_W: flat device tensors

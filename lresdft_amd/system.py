@@ -102,6 +102,7 @@ class LCAOBasis:
 
     def __init__(self, cr, rcut=5.5, s_exps=(1.0, 0.3, 0.12), p_exps=(0.5,)):
         self.cr, self.rcut = cr, rcut
+        self.s_exps, self.p_exps = tuple(s_exps), tuple(p_exps)
         rad = int(np.ceil(rcut / cr.dx))
         o = np.arange(-rad, rad + 1)
         box = np.stack(np.meshgrid(o, o, o, indexing="ij"), -1).reshape(-1, 3)
@@ -142,3 +143,56 @@ class LCAOBasis:
     def H(self, v):
         Xv = self.Xr.multiply(np.asarray(v).reshape(-1, 1)).tocsc()
         return (self.T + (self.X.T @ Xv) * self.cr.dV).tocsr()
+
+    def derivatives(self):
+        """d x_c(g) / d R_A for every function c (on atom A) at its grid points, as three
+        CSC matrices with X's sparsity and entry order (x, y, z components of R_A).
+
+        x_c = phi_c(g - R_A) / |phi_c|, |phi_c|^2 = sum_g phi_c^2 dV, so
+        dx/dR = -grad phi / |phi| + phi (sum_g phi . grad phi dV) / |phi|^3.  With
+        u = r^2 / rcut^2, e = exp(-al r^2), env = (1 - u)^2 and
+        q = e (-2 al env - 4 (1 - u) / rcut^2): grad phi_s = q d, and for the p
+        function along axis i, grad (d_i e env) = e_i e env + d_i q d.  The
+        envelope's derivative vanishes at rcut, so the changing set of grid points
+        inside the cut-off contributes nothing."""
+        cr, rcut = self.cr, self.rcut
+        rad = int(np.ceil(rcut / cr.dx))
+        o = np.arange(-rad, rad + 1)
+        box = np.stack(np.meshgrid(o, o, o, indexing="ij"), -1).reshape(-1, 3)
+        nat = len(cr.pos)
+        pts = np.round(cr.pos / cr.dx).astype(int)[:, None, :] + box[None]
+        d = pts * cr.dx - cr.pos[:, None, :]
+        d = d - cr.cell * np.round(d / cr.cell)
+        r2 = np.sum(d * d, axis=2)
+        keep = r2 < rcut * rcut
+        u = r2 / (rcut * rcut)
+        env = (1.0 - u) ** 2
+        vals, grads = [], []
+        for al in self.s_exps:
+            e = np.exp(-al * r2)
+            q = e * (-2.0 * al * env - 4.0 * (1.0 - u) / rcut ** 2)
+            vals.append(e * env)
+            grads.append(q[..., None] * d)
+        for al in self.p_exps:
+            e = np.exp(-al * r2)
+            q = e * (-2.0 * al * env - 4.0 * (1.0 - u) / rcut ** 2)
+            for i in range(3):
+                vals.append(d[..., i] * e * env)
+                g = d[..., i][..., None] * q[..., None] * d
+                g[..., i] += e * env
+                grads.append(g)
+        F = np.where(keep[:, None, :], np.stack(vals, 1), 0.0)                         # (nat, nf, nbox)
+        dF = np.where(keep[:, None, :, None], -np.stack(grads, 1), 0.0)                # d/dR = -d/dd
+        norm = np.sqrt(np.sum(F * F, axis=2) * cr.dV)                                   # (nat, nf)
+        dnorm = np.sum(F[..., None] * dF, axis=2) * cr.dV / norm[..., None]             # (nat, nf, 3)
+        dX = dF / norm[..., None, None] - F[..., None] * (dnorm / norm[..., None] ** 2)[:, :, None, :]
+        shape = np.array(cr.shape)
+        gidx = np.ravel_multi_index(tuple(np.moveaxis(pts % shape, -1, 0)), cr.shape)
+        nf = F.shape[1]
+        rows = np.concatenate([np.tile(gidx[a][keep[a]], nf) for a in range(nat)])
+        indptr = np.concatenate([[0], np.cumsum(np.repeat(keep.sum(1), nf))])
+        out = []
+        for i in range(3):
+            data = np.concatenate([dX[a][:, keep[a], i].ravel() for a in range(nat)])
+            out.append(sparse.csc_matrix((data, rows, indptr), shape=self.X.shape))
+        return out

@@ -57,3 +57,26 @@ def test_grid_ops_match_oracle_on_cpu_tensors():
     e1, v1 = g.lda_xc(torch.from_numpy(n))
     e2, v2 = orc.lda_xc(n)
     assert np.max(np.abs(e1.numpy() - e2)) < 1e-14 and np.max(np.abs(v1.numpy() - v2)) < 1e-14
+
+
+def test_basis_derivatives_match_finite_differences():
+    """d x_c(g) / d R_A (analytic, with the grid-quadrature normalisation) vs central
+    differences of the basis itself: 1e-8 absolute (values are O(1))."""
+    from lresdft_amd import system
+    cr = system.h2_crystal((2, 2, 2), a=6.0, pts_per_cell=10, rc=0.6)
+    b = system.LCAOBasis(cr, **system.DEFAULT_BASIS)
+    dX = b.derivatives()
+    h = 1e-5
+    for A in (0, 7):
+        cols = np.where(b.atom_of_fn == A)[0]
+        for i in range(3):
+            pp, pm = cr.pos.copy(), cr.pos.copy()
+            pp[A, i] += h
+            pm[A, i] -= h
+            bp = system.LCAOBasis(system.Crystal(pp, cr.Z, cr.cell, cr.shape, cr.rc), **system.DEFAULT_BASIS)
+            bm = system.LCAOBasis(system.Crystal(pm, cr.Z, cr.cell, cr.shape, cr.rc), **system.DEFAULT_BASIS)
+            fd = ((bp.X - bm.X) / (2 * h)).toarray()
+            an = dX[i].toarray()
+            assert np.max(np.abs(fd[:, cols] - an[:, cols])) < 1e-8
+            others = b.atom_of_fn != A
+            assert np.max(np.abs(fd[:, others])) == 0.0
