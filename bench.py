@@ -42,6 +42,11 @@ def parse():
                          "periodic cube at the density of the a = 6.0 lattice (uneven windows)")
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-forces", action="store_true",
+                    help="skip the adjoint / band-energy forces at the converged potential (after the timed region)")
+    ap.add_argument("--profile-pass", type=int, default=0, metavar="K",
+                    help="profiling mode: K warm window passes per precision at the workload's guess potential "
+                         "and a 256 MiB device copy (the byte-counter calibration), no SCF, no JSON line")
     return ap.parse_args()
 
 
@@ -64,15 +69,40 @@ def run_scf(C, cr, grid, vion, precision, tol):
     return r
 
 
-def roofline(stats, flops, peak):
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pole_kernels_traffic_1024e_rw8_m32.txt")
+
+
+def measured_traffic(tag):
+    """HBM bytes per launch of the pole kernel (FETCH_SIZE + WRITE_SIZE, scaled by a
+    calibration copy) from the committed rocprofv3 summary of this workload."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            line = line.strip()
+            if line.startswith(f"traffic_bytes_per_launch_{tag} ="):
+                return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
+def kernel_name(bucket, fp32):
+    T, G, kind = bucket[5:8] if fp32 else bucket[2:5]
+    if kind == 2:
+        return f"k_pole_mfma<{'float' if fp32 else 'double'}, tiles {T}, waves {G}>"
+    return f"k_pole_gj<T {T}, {'float' if fp32 else 'double'}, grid {G}x{G}>"
+
+
+def roofline(stats, flops, peak, kernel, traffic):
     passes, ms = stats
     if passes == 0 or ms <= 0:
         return None
     t = ms * 1e-3 / passes
     achieved = flops / t
     return {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-            "frac": achieved / peak, "traffic": None, "kernel": "k_pole_gj", "launch_ms": ms / passes,
-            "flops_per_launch": flops}
+            "frac": achieved / peak, "traffic": traffic, "kernel": kernel, "launch_ms": ms / passes,
+            "flops_per_launch": flops,
+            "traffic_note": ("HBM bytes per launch, rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a 256 MiB copy "
+                             "(profiles/pole_kernels_traffic_1024e_rw8_m32.txt)") if traffic else None}
 
 
 def cpu_baseline(cr_p, v_host, n, rw, M, iters, budget_s=40.0):
@@ -119,6 +149,50 @@ def cpu_baseline(cr_p, v_host, n, rw, M, iters, budget_s=40.0):
             "parts_1thread": one, "parts_all_cores": allc}
 
 
+def adjoint_timing(C, grid, vion, res):
+    """The adjoint kernels (fp64) and the fixed-N band-energy forces at the converged
+    input potential of an SCF result, timed after the benchmark's timed region."""
+    import torch
+    from lresdft_amd.forces import band_forces
+    C.set_precision("fp64")
+    v = (vion + grid.hartree(res["dens_in"]) + grid.lda_xc(res["dens_in"])[1]).reshape(-1).contiguous()
+    C.plan.set_potential(v)
+    C.plan.adjoint(C.mu, C.e_lo)            # first call allocates the workspaces
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    C.plan.adjoint(C.mu, C.e_lo)
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t
+    t = time.perf_counter()
+    F, parts = band_forces(C, v.cpu().numpy())
+    t_f = time.perf_counter() - t
+    return {"adjoint_ms": 1e3 * t_adj, "forces_s": t_f, "max_abs_force": float(abs(F).max()),
+            "note": ("adjoint = d(2 sum E_B)/d(H_W, S_W) on the GPU (fp64, all windows); forces = fixed-N "
+                     "band-energy forces with the converged grid potential frozen: adjoint + D_B/F_B on the GPU, "
+                     "chain rule to atoms on the host")}
+
+
+def profile_passes(C, cr, grid, vion, K):
+    """Warm passes at the guess potential (uniform density), both precisions, then a
+    256 MiB device-to-device copy whose FETCH_SIZE / WRITE_SIZE calibrate the counters."""
+    import torch
+    d0 = torch.full(cr.shape, cr.n_elec / (cr.ng * cr.dV), dtype=torch.float64, device="cuda")
+    v = (vion + grid.hartree(d0) + grid.lda_xc(d0)[1]).reshape(-1).contiguous()
+    C.plan.set_potential(v)
+    lmin = C.plan.lambda_min()
+    e_lo, mu = lmin - C.margin, lmin + 0.35
+    for prec in ("mixed", "fp64"):
+        C.set_precision(prec)
+        for _ in range(K + 1):          # the first pass also runs the autotuner
+            C.plan.run(mu, e_lo, want_density=False)
+    src = torch.ones(32 * 2 ** 20, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"profile passes done: {K} per precision, buckets {C.plan.buckets()}", flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -149,6 +223,10 @@ def main():
     C = LRFTLCAOCpp(b, cells, R_w=args.rw, M=args.M)
     flops = C.plan.flops_per_pass()
 
+    if args.profile_pass:
+        profile_passes(C, cr, grid, vion, args.profile_pass)
+        return
+
     for _ in range(args.warmup):
         run_scf(C, cr, grid, vion, "fp64", args.tol)
         run_scf(C, cr, grid, vion, "mixed", args.tol)
@@ -164,6 +242,8 @@ def main():
     st32 = C.plan.stats(reset=True)
 
     t_mixed = t_total / args.steps
+    bk = C.plan.buckets()
+    default_workload = (not args.random and args.n == 8 and args.rw == 8.0 and args.M == 32 and len(bk) == 1)
     t_fp64 = sum(r["wall"] for r in fp64) / args.steps
     dE = (mixed[-1]["energy"] - fp64[-1]["energy"]) / cr.n_elec
     line = {
@@ -180,10 +260,14 @@ def main():
         "passes_per_scf": {"fp64": fp64[-1]["passes"], "mixed": mixed[-1]["passes"]},
         "history": {"fp64": fp64[-1]["history"], "mixed": mixed[-1]["history"]},
         "passes_per_iteration": {"fp64": fp64[-1]["pass_log"], "mixed": mixed[-1]["pass_log"]},
-        "roofline": roofline(st32["fp32"], flops, PEAK_FP32) or roofline(st32["fp64"], flops, PEAK_FP64),
-        "roofline_fp64": roofline(st64["fp64"], flops, PEAK_FP64),
+        "roofline": roofline(st32["fp32"], flops, PEAK_FP32, kernel_name(bk[0], True),
+                             measured_traffic("fp32") if default_workload else None),
+        "roofline_fp64": roofline(st64["fp64"], flops, PEAK_FP64, kernel_name(bk[0], False),
+                                  measured_traffic("fp64") if default_workload else None),
         "cpu_baseline": None,
     }
+    if not args.no_forces:
+        line["adjoint"] = adjoint_timing(C, grid, vion, mixed[-1])
     if not args.no_cpu_baseline and not args.random:
         v_host = (vion + grid.hartree(fp64[-1]["dens_in"]) + grid.lda_xc(fp64[-1]["dens_in"])[1]).cpu().numpy()
         line["cpu_baseline"] = cpu_baseline(cr, v_host, n, args.rw, args.M, fp64[-1]["iterations"])

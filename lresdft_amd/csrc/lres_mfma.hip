@@ -222,7 +222,8 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
                     // masked arithmetically, so no branch per entry; the padding is
                     // 0 off and 1 on the diagonal
                     const bool in = row < nn && col < nn;
-                    const int64_t off = in ? (int64_t)row * nn + col : 0;
+                    // 32-bit element offset (n <= 128): base-register addressing, one VGPR per address
+                    const uint32_t off = in ? (uint32_t)(row * nn + col) : 0u;
                     const double f = in ? 1.0 : 0.0;
                     const double pad = (!in && row == col) ? 1.0 : 0.0;
                     const double s = S[off] * f;
@@ -232,8 +233,9 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
                 }
             }
             // this tile row's entries are formed before the next row's loads issue
+            // (the memory clobber keeps those loads below this point)
 #pragma unroll
-            for (int j = 0; j < TJ; ++j) asm volatile("" : "+v"(are[I][j]), "+v"(aim[I][j]));
+            for (int j = 0; j < TJ; ++j) asm volatile("" : "+v"(are[I][j]), "+v"(aim[I][j])::"memory");
         }
         {
             int ln = lane;
