@@ -137,15 +137,17 @@ def cpu_baseline(cr_p, v_host, n, rw, M, iters, budget_s=40.0):
             t = time.perf_counter()
             orc.ion_potential(crs)
             t_ion = (time.perf_counter() - t) * len(cr.pos) / len(sub)
-        est = t_ion + t_cold + (iters - 1) * t_warm + iters * (t_h + t_pot)
+        # the GPU timing starts from a precomputed ion potential, so the estimate leaves it out
+        est = t_cold + (iters - 1) * t_warm + iters * (t_h + t_pot)
         out[threads] = dict(est=est, t_cold=t_cold, t_warm=t_warm, t_H=t_h, t_pot=t_pot, t_ion=t_ion)
     one, allc = out[1], out[cores]
     return {"value": allc["est"], "unit": "s", "cores": cores, "kind": "port",
             "value_1thread": one["est"],
             "sample": (f"oracle eigen-route LRes (scipy eigh per window, R_w {rw}, M {M}) on the same "
-                       f"{n}^3 crystal: one cold + one warm density build, one H(v), Hartree+LDA, ion "
-                       f"potential on 64 atoms scaled; SCF estimate = ion + cold + (it-1) warm + it (H + pot) "
-                       f"with it = {iters} (the GPU run's iterations)"),
+                       f"{n}^3 crystal: one cold + one warm density build, one H(v), Hartree+LDA; SCF "
+                       f"estimate = cold + (it-1) warm + it (H + pot) with it = {iters} (the GPU run's "
+                       f"iterations); the ion potential, precomputed outside the GPU timing too, is "
+                       f"reported apart (t_ion, timed on 64 atoms and scaled)"),
             "parts_1thread": one, "parts_all_cores": allc}
 
 
