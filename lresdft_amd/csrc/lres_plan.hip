@@ -976,33 +976,41 @@ int lres_lambda_min(lres_plan* P, double tol, double* out) {
             step *= 2.0;
             lo -= step;
         }
-        // multisection with 15 interior shifts per launch.  Once a shift with a
-        // nonzero total is known, only the windows counting an eigenvalue below it
-        // can hold the minimum: later rounds factor those windows alone.
+        // multisection with 15 interior shifts per launch.  Every round keeps only the
+        // windows that count an eigenvalue below its new upper bracket end: only they
+        // can hold the minimum, so the set shrinks as the bracket narrows.
         const int K = 15;
         std::vector<double> sg(K);
         std::vector<int64_t> tot(K);
-        std::vector<int32_t> cand;
-        std::vector<int32_t> cnt_w;
+        std::vector<int32_t> cand, next;
+        std::vector<int32_t> cnt_w((size_t)nwin * K);
         if ((int64_t)P->d_icnt.n < (int64_t)nwin * K) P->d_icnt.alloc((size_t)nwin * K);
-        bool restricted = false;
+        bool all = true;
         while (hi - lo > tol) {
             for (int s = 0; s < K; ++s) sg[s] = lo + (hi - lo) * (s + 1) / (K + 1);
-            inertia(P, sg.data(), K, tot.data(), restricted ? nullptr : P->d_icnt.p, restricted ? &cand : nullptr);
+            inertia(P, sg.data(), K, tot.data(), P->d_icnt.p, all ? nullptr : &cand);
             double nlo = lo, nhi = hi;
             int first_pos = -1;
             for (int s = 0; s < K; ++s) {
                 if (tot[s] == 0) nlo = sg[s];
                 else { nhi = sg[s]; first_pos = s; break; }
             }
-            if (!restricted && first_pos >= 0) {
-                cnt_w.resize((size_t)nwin * K);
+            if (first_pos >= 0) {
                 hip_check(hipMemcpyAsync(cnt_w.data(), P->d_icnt.p, cnt_w.size() * sizeof(int32_t),
                                          hipMemcpyDeviceToHost, P->st), "copy");
                 hip_check(hipStreamSynchronize(P->st), "sync");
-                for (int w = 0; w < nwin; ++w)
-                    if (cnt_w[(size_t)w * K + first_pos] > 0) cand.push_back(w);
-                restricted = !cand.empty();
+                next.clear();
+                if (all) {
+                    for (int w = 0; w < nwin; ++w)
+                        if (cnt_w[(size_t)w * K + first_pos] > 0) next.push_back(w);
+                } else {
+                    for (int32_t w : cand)
+                        if (cnt_w[(size_t)w * K + first_pos] > 0) next.push_back(w);
+                }
+                if (!next.empty()) {
+                    cand.swap(next);
+                    all = false;
+                }
             }
             if (nlo == lo && nhi == hi) break;   // no progress at double resolution
             lo = nlo;
