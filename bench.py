@@ -34,6 +34,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=8, help="lattice cells per axis")
+    ap.add_argument("--a", type=float, default=6.0,
+                    help="lattice constant in bohr (grid round(a / 0.6) points per cell; 5.1 is the 0.85x compressed lattice)")
     ap.add_argument("--rw", type=float, default=8.0)
     ap.add_argument("--M", type=int, default=32)
     ap.add_argument("--tol", type=float, default=1e-6)
@@ -44,6 +46,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forces", action="store_true",
                     help="skip the adjoint / band-energy forces at the converged potential (after the timed region)")
+    ap.add_argument("--forces-fd", type=int, default=0, metavar="K",
+                    help="also check the forces of K seeded random atoms against central differences of the "
+                         "fixed-N band energy (steps 1e-4 / 5e-5 bohr, Richardson)")
     ap.add_argument("--profile-pass", type=int, default=0, metavar="K",
                     help="profiling mode: K warm window passes per precision at the workload's guess potential "
                          "and a 256 MiB device copy (the byte-counter calibration), no SCF, no JSON line")
@@ -151,7 +156,43 @@ def cpu_baseline(cr_p, v_host, n, rw, M, iters, budget_s=40.0):
             "parts_1thread": one, "parts_all_cores": allc}
 
 
-def adjoint_timing(C, grid, vion, res):
+def forces_fd(C, v, F, cells, args, K, seed=11):
+    """Central differences (steps 1e-4 and 5e-5 bohr, Richardson) of the fixed-N band
+    energy along one random unit direction for K random atoms, v frozen, e_lo fixed."""
+    from lresdft_amd import system
+    from lresdft_amd.lcao3d import LRFTLCAOCpp, csr64
+    cr, N, mu, e_lo = C.b.cr, C.b.cr.n_elec, C.mu, C.e_lo
+
+    def e_band(pos):
+        crp = system.Crystal(pos, cr.Z, cr.cell, cr.shape, cr.rc)
+        bp = system.LCAOBasis(crp, **system.DEFAULT_BASIS)
+        Cp = LRFTLCAOCpp(bp, cells, R_w=args.rw, M=args.M, device_h=False)
+        Cp.plan.set_h(csr64(bp.H(v)))
+        m, _ = Cp.plan.mu_solve(N, e_lo, e_lo + 1e-6, mu + 1.0, xtol=1e-14, ftol=1e-13 * N)
+        return 2.0 * Cp.plan.run(m, e_lo, want_density=False)[1]
+
+    rng = np.random.default_rng(seed)
+    out, worst = [], 0.0
+    for A in rng.choice(len(cr.pos), size=K, replace=False):
+        u = rng.standard_normal(3)
+        u /= np.linalg.norm(u)
+
+        def f(h):
+            p = cr.pos.copy()
+            p[A] += h * u
+            return e_band(p)
+
+        h = 1e-4
+        d1 = (f(h) - f(-h)) / (2 * h)
+        d2 = (f(h / 2) - f(-h / 2)) / h
+        fd = -(4 * d2 - d1) / 3
+        an = float(F[A] @ u)
+        worst = max(worst, abs(an - fd) / max(abs(an), 1e-3))
+        out.append({"atom": int(A), "F.u": an, "-dE/dh": fd})
+    return {"atoms": out, "max_rel_err": worst}
+
+
+def adjoint_timing(C, grid, vion, res, cells, args):
     """The adjoint kernels (fp64) and the fixed-N band-energy forces at the converged
     input potential of an SCF result, timed after the benchmark's timed region."""
     import torch
@@ -166,9 +207,11 @@ def adjoint_timing(C, grid, vion, res):
     torch.cuda.synchronize()
     t_adj = time.perf_counter() - t
     t = time.perf_counter()
-    F, parts = band_forces(C, v.cpu().numpy())
+    vh = v.cpu().numpy()
+    F, parts = band_forces(C, vh)
     t_f = time.perf_counter() - t
-    return {"adjoint_ms": 1e3 * t_adj, "forces_s": t_f, "max_abs_force": float(abs(F).max()),
+    fd = forces_fd(C, vh, F, cells, args, args.forces_fd) if args.forces_fd else None
+    return {"adjoint_ms": 1e3 * t_adj, "forces_s": t_f, "max_abs_force": float(abs(F).max()), "fd_check": fd,
             "note": ("adjoint = d(2 sum E_B)/d(H_W, S_W) on the GPU (fp64, all windows); forces = fixed-N "
                      "band-energy forces with the converged grid potential frozen: adjoint + D_B/F_B on the GPU, "
                      "chain rule to atoms on the host")}
@@ -215,10 +258,11 @@ def main():
         workload = (f"{args.random} H2 at random centres (d_min 3.0, seed {args.seed}) in a {box:.2f} bohr cube, "
                     f"{cr.n_elec} e-, R_w {args.rw}, M {args.M}, SCF tol {args.tol}")
     else:
-        cr = system.h2_crystal((n, n, n), a=6.0, pts_per_cell=10, rc=0.6)
-        cells = system.cell_centres((n, n, n), 6.0)
-        workload = (f"h2_crystal {n}x{n}x{n} a=6.0 jitter 0.15 seed 0, {cr.n_elec} e-, "
-                    f"R_w {args.rw}, M {args.M}, SCF tol {args.tol}")
+        pts = int(round(args.a / 0.6))
+        cr = system.h2_crystal((n, n, n), a=args.a, pts_per_cell=pts, rc=0.6)
+        cells = system.cell_centres((n, n, n), args.a)
+        workload = (f"h2_crystal {n}x{n}x{n} a={args.a} ({pts} grid points per cell) jitter 0.15 seed 0, "
+                    f"{cr.n_elec} e-, R_w {args.rw}, M {args.M}, SCF tol {args.tol}")
     b = system.LCAOBasis(cr, **system.DEFAULT_BASIS)
     grid = GridOps(cr)
     vion = grid.ion_potential()
@@ -269,7 +313,7 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_forces:
-        line["adjoint"] = adjoint_timing(C, grid, vion, mixed[-1])
+        line["adjoint"] = adjoint_timing(C, grid, vion, mixed[-1], cells, args)
     if not args.no_cpu_baseline and not args.random:
         v_host = (vion + grid.hartree(fp64[-1]["dens_in"]) + grid.lda_xc(fp64[-1]["dens_in"])[1]).cpu().numpy()
         line["cpu_baseline"] = cpu_baseline(cr, v_host, n, args.rw, args.M, fp64[-1]["iterations"])

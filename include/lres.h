@@ -41,13 +41,22 @@ int lres_device_count(void);
  * Atoms go to their nearest cell centre (minimum image, lowest index on ties);
  * grid point (i,j,k) at (i,j,k)*dx goes to the centre with the smallest squared
  * minimum-image distance, lowest index on ties; W(B) lists, ascending, the
- * functions whose cell lies within R_w of B.  Outputs: h_Woff (ncell+1),
- * h_Ooff (ncell+1), h_Oflat (ng) always; h_Wflat may be NULL on a first call,
- * which then only fills h_Woff so that the caller can size h_Wflat. */
+ * functions whose cell lies within R_w of B.  Outputs: h_Woff (ncell+1), and
+ * with h_Wflat given, h_Wflat, h_Ooff (ncell+1), h_Oflat (ng); h_Wflat NULL
+ * (a first call) fills h_Woff only, so that the caller can size h_Wflat. */
 int lres_build_windows(const double* h_cells, int64_t ncell, const double* h_atom_pos, int64_t natom,
                        const int64_t* h_atom_of_fn, int64_t nb, const double* h_box, const int64_t* h_shape,
                        double dx, double R_w, int64_t* h_Wflat, int64_t* h_Woff, int64_t* h_Oflat,
                        int64_t* h_Ooff);
+
+/* The same tables with the ng x ncell Voronoi distances on the GPU (the bulk of
+ * lcao3d.py:141-151's cost: 2.6e8 distances at 1024 electrons); bitwise the
+ * same output as lres_build_windows (identical operation order, no
+ * contraction).  Needs a HIP device; at most 4096 cells. */
+int lres_build_windows_device(const double* h_cells, int64_t ncell, const double* h_atom_pos, int64_t natom,
+                              const int64_t* h_atom_of_fn, int64_t nb, const double* h_box,
+                              const int64_t* h_shape, double dx, double R_w, int64_t* h_Wflat, int64_t* h_Woff,
+                              int64_t* h_Oflat, int64_t* h_Ooff);
 
 /* Per-geometry plan (replaces LRFTLCAOCpp.__init__'s flattening, lcao3d.py:343-357,
  * plus everything lrft_windows recomputes per call that only depends on the

@@ -26,6 +26,8 @@ SIGNATURES = {
     "lres_device_count": (C.c_int, []),
     "lres_build_windows": (C.c_int, [_f64p, C.c_int64, _f64p, C.c_int64, _i64p, C.c_int64, _f64p, _i64p,
                                      C.c_double, C.c_double, _i64p, _i64p, _i64p, _i64p]),
+    "lres_build_windows_device": (C.c_int, [_f64p, C.c_int64, _f64p, C.c_int64, _i64p, C.c_int64, _f64p, _i64p,
+                                            C.c_double, C.c_double, _i64p, _i64p, _i64p, _i64p]),
     "lres_plan_create": (C.c_int, [C.POINTER(_vp), C.c_int64, C.c_int64, C.c_double, C.c_int, C.c_int,
                                    _i64p, _i64p, C.c_int64, _i64p, _i64p, _i64p, _i64p, _f64p,
                                    _i64p, _i64p, _f64p, _vp]),

@@ -27,6 +27,8 @@ hipError_t launch_density(const lres::WinDesc* wins, int nwin, int kmax, const d
                           const int64_t* xptr, const int32_t* xidx, const double* xval, double* dens,
                           hipStream_t st);
 hipError_t launch_sum2(const double* a, const double* b, int n, double* out, hipStream_t st);
+hipError_t launch_voronoi_owner(const double* cells, int ncell, const double* box, const int64_t* shape, double dx,
+                                int32_t* owner, hipStream_t st);
 hipError_t launch_diag_ratio(const lres::WinDesc* wins, int nwin, const double* SW, const double* HW, double* out,
                              hipStream_t st);
 hipError_t launch_inertia(int T, int G, const lres::WinDesc* wins, const int32_t* ids, int nids, const double* SW,

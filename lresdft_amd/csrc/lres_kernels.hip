@@ -539,6 +539,54 @@ hipError_t launch_density(const WinDesc* wins, int nwin, int kmax, const double*
     return hipGetLastError();
 }
 
+// Voronoi owner of every grid point (lrft/lcao3d.py:141-151): the nearest cell
+// centre under the minimum image d - L rint(d / L) (round half to even), ties to
+// the lowest cell (strict '<' in ascending order).  Every operation is written
+// as a non-contracting intrinsic in the host builder's order, so the distances,
+// and with them the ties, are bitwise those of the host code.
+__global__ void __launch_bounds__(256) k_voronoi_owner(const double* __restrict__ cells, int ncell,
+                                                       double L0, double L1, double L2, int n1, int n2, int n3,
+                                                       double dx, int32_t* __restrict__ owner) {
+    extern __shared__ double cs[];
+    for (int q = threadIdx.x; q < 3 * ncell; q += blockDim.x) cs[q] = cells[q];
+    __syncthreads();
+    const int64_t ng = (int64_t)n1 * n2 * n3;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    const int64_t i = g / ((int64_t)n2 * n3), j = (g / n3) % n2, k = g % n3;
+    const double r0 = __dmul_rn((double)i, dx), r1 = __dmul_rn((double)j, dx), r2 = __dmul_rn((double)k, dx);
+    double best = 1.0 / 0.0;
+    int bi = 0;
+    for (int B = 0; B < ncell; ++B) {
+        double d0 = __dsub_rn(r0, cs[3 * B + 0]);
+        double d1 = __dsub_rn(r1, cs[3 * B + 1]);
+        double d2 = __dsub_rn(r2, cs[3 * B + 2]);
+        d0 = __dsub_rn(d0, __dmul_rn(L0, rint(__ddiv_rn(d0, L0))));
+        d1 = __dsub_rn(d1, __dmul_rn(L1, rint(__ddiv_rn(d1, L1))));
+        d2 = __dsub_rn(d2, __dmul_rn(L2, rint(__ddiv_rn(d2, L2))));
+        const double sq = __dadd_rn(__dadd_rn(__dmul_rn(d0, d0), __dmul_rn(d1, d1)), __dmul_rn(d2, d2));
+        if (sq < best) {
+            best = sq;
+            bi = B;
+        }
+    }
+    owner[g] = bi;
+}
+
+hipError_t launch_voronoi_owner(const double* cells, int ncell, const double* box, const int64_t* shape, double dx,
+                                int32_t* owner, hipStream_t st) {
+    const int64_t ng = shape[0] * shape[1] * shape[2];
+    const size_t lds = (size_t)3 * ncell * sizeof(double);
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_voronoi_owner, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_voronoi_owner, dim3((unsigned)((ng + 255) / 256)), dim3(256), lds, st, cells, ncell, box[0],
+                       box[1], box[2], (int)shape[0], (int)shape[1], (int)shape[2], dx, owner);
+    return hipGetLastError();
+}
+
 // per window the smallest Rayleigh quotient of a basis function, min_a H_aa / S_aa
 // (an upper bound of the window's lowest generalized eigenvalue)
 __global__ void __launch_bounds__(64) k_diag_ratio(const WinDesc* __restrict__ wins, const double* __restrict__ SW,

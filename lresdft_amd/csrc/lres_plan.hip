@@ -431,9 +431,13 @@ int lres_device_count(void) {
     return n;
 }
 
-int lres_build_windows(const double* cells, int64_t ncell, const double* apos, int64_t natom,
-                       const int64_t* atom_of_fn, int64_t nb, const double* box, const int64_t* shape, double dx,
-                       double R_w, int64_t* Wflat, int64_t* Woff, int64_t* Oflat, int64_t* Ooff) {
+#pragma clang fp contract(off)
+// lres_build_windows / lres_build_windows_device: the same tables; the Voronoi owners
+// of the grid points (ng x ncell distances) on the host or by k_voronoi_owner.
+static int build_windows_impl(const double* cells, int64_t ncell, const double* apos, int64_t natom,
+                              const int64_t* atom_of_fn, int64_t nb, const double* box, const int64_t* shape,
+                              double dx, double R_w, int64_t* Wflat, int64_t* Woff, int64_t* Oflat, int64_t* Ooff,
+                              bool on_device) {
     return guarded([&] {
         require(ncell > 0 && natom > 0 && nb > 0, "empty geometry");
         // minimum image exactly as d - L * round(d / L) with round half to even
@@ -481,10 +485,25 @@ int lres_build_windows(const double* cells, int64_t ncell, const double* apos, i
                 }
             Woff[B + 1] = Woff[B] + cnt;
         }
+        if (!Wflat) return;   // sizing call: Woff only
         // Voronoi owners of the grid points, ties to the lowest B (strict '<' in ascending B)
         const int64_t n1 = shape[0], n2 = shape[1], n3 = shape[2], ng = n1 * n2 * n3;
         std::vector<int64_t> owner(ng);
-        for (int64_t g = 0; g < ng; ++g) {
+        if (on_device) {
+            require(ncell <= 4096, "more than 4096 cells for the device Voronoi kernel");
+            int ndev = 0;
+            hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+            require(ndev > 0, "no HIP device");
+            DevBuf<double> d_cells;
+            DevBuf<int32_t> d_owner;
+            d_cells.upload(cells, (size_t)3 * ncell, nullptr);
+            d_owner.alloc((size_t)ng);
+            hip_check(launch_voronoi_owner(d_cells.p, (int)ncell, box, shape, dx, d_owner.p, nullptr), "voronoi kernel");
+            std::vector<int32_t> o32(ng);
+            hip_check(hipMemcpy(o32.data(), d_owner.p, ng * sizeof(int32_t), hipMemcpyDeviceToHost), "copy");
+            for (int64_t g = 0; g < ng; ++g) owner[g] = o32[g];
+        }
+        for (int64_t g = 0; g < ng && !on_device; ++g) {
             const int64_t i = g / (n2 * n3), j = (g / n3) % n2, k = g % n3;
             const double r[3] = {(double)i * dx, (double)j * dx, (double)k * dx};
             double best = std::numeric_limits<double>::infinity();
@@ -508,6 +527,21 @@ int lres_build_windows(const double* cells, int64_t ncell, const double* apos, i
         std::vector<int64_t> fill(Ooff, Ooff + ncell);
         for (int64_t g = 0; g < ng; ++g) Oflat[fill[owner[g]]++] = g;
     });
+}
+#pragma clang fp contract(on)
+
+int lres_build_windows(const double* cells, int64_t ncell, const double* apos, int64_t natom,
+                       const int64_t* atom_of_fn, int64_t nb, const double* box, const int64_t* shape, double dx,
+                       double R_w, int64_t* Wflat, int64_t* Woff, int64_t* Oflat, int64_t* Ooff) {
+    return build_windows_impl(cells, ncell, apos, natom, atom_of_fn, nb, box, shape, dx, R_w, Wflat, Woff, Oflat,
+                              Ooff, false);
+}
+
+int lres_build_windows_device(const double* cells, int64_t ncell, const double* apos, int64_t natom,
+                              const int64_t* atom_of_fn, int64_t nb, const double* box, const int64_t* shape,
+                              double dx, double R_w, int64_t* Wflat, int64_t* Woff, int64_t* Oflat, int64_t* Ooff) {
+    return build_windows_impl(cells, ncell, apos, natom, atom_of_fn, nb, box, shape, dx, R_w, Wflat, Woff, Oflat,
+                              Ooff, true);
 }
 
 int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, int precision,

@@ -45,9 +45,11 @@ def csr64(A):
     return nat.i64(A.indptr), nat.i64(A.indices), nat.f64(A.data)
 
 
-def build_windows(basis, cells, R_w):
+def build_windows(basis, cells, R_w, device=None):
     """Window lists W(B) and Voronoi-owned grid points own(B), natively
-    (lcao3d.py:132-161 semantics; see lres_build_windows in include/lres.h)."""
+    (lcao3d.py:132-161 semantics; see lres_build_windows in include/lres.h).
+    device (default: whether a GPU is present): the grid-point owners by the
+    GPU kernel (lres_build_windows_device, bitwise the same tables)."""
     cr = basis.cr
     L = nat.lib()
     cells = nat.f64(cells).reshape(-1, 3)
@@ -61,9 +63,12 @@ def build_windows(basis, cells, R_w):
     Oflat = np.zeros(ng, np.int64)
     args = (nat.pf(cells), ncell, nat.pf(pos), len(pos), nat.p64(aof), len(aof), nat.pf(box), nat.p64(shape),
             float(cr.dx), float(R_w))
-    nat.check(L.lres_build_windows(*args, None, nat.p64(Woff), nat.p64(Oflat), nat.p64(Ooff)))
+    if device is None:
+        device = L.lres_device_count() > 0
+    fn = L.lres_build_windows_device if device else L.lres_build_windows
+    nat.check(fn(*args, None, nat.p64(Woff), None, None))
     Wflat = np.zeros(int(Woff[-1]), np.int64)
-    nat.check(L.lres_build_windows(*args, nat.p64(Wflat), nat.p64(Woff), nat.p64(Oflat), nat.p64(Ooff)))
+    nat.check(fn(*args, nat.p64(Wflat), nat.p64(Woff), nat.p64(Oflat), nat.p64(Ooff)))
     return Wflat, Woff, Oflat, Ooff
 
 
