@@ -61,11 +61,15 @@ def _richardson(f, h=2e-5):
     return (4 * d2 - d1) / 3
 
 
-@pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (0.0, 1.0)])
-def test_adjoint_matches_finite_differences(alpha, beta):
-    print(f"[gpu] adjoint vs FD n=3 R_w=8 alpha={alpha} beta={beta}", flush=True)
-    cr, b, H, C, e_lo, mu = _solver(3, 8.0)
+@pytest.mark.parametrize("alpha,beta,Rw", [(1.0, 0.0, 8.0), (0.0, 1.0, 8.0), (1.0, 0.0, 10.0)])
+def test_adjoint_matches_finite_differences(alpha, beta, Rw):
+    """R_w 8: every window function touches the owned points (k = n); R_w 10: n_W 228
+    with k < n (functions outside T_B, the blocked out-of-register kernels)."""
+    print(f"[gpu] adjoint vs FD n=3 R_w={Rw} alpha={alpha} beta={beta}", flush=True)
+    cr, b, H, C, e_lo, mu = _solver(3, Rw)
     plan = C.plan
+    if Rw > 8.0:
+        assert (plan.k < plan.n).all() and plan.n.min() > 128
     HW, SW = plan.window_blocks()
     gH, gS = plan.adjoint(mu, e_lo, alpha, beta)
     for which, g in (("H", gH), ("S", gS)):
