@@ -295,13 +295,21 @@ __global__ void __launch_bounds__(256) k_reduce(const WinDesc* __restrict__ wins
 }
 
 // n(g) = x_g^T D_B x_g on the owned points (src/lrft_cpp.cpp:183-191 in D form).
-// One wave per owned point, lanes over the nnz^2 entry pairs of the point's row
-// of X (restricted to T); the row is staged in a per-wave LDS slot and D_B in
-// LDS when it fits (stage != 0, dynamic shared size k*k doubles).  Pair q maps
-// to (q / nnz, q % nnz) by a multiply-shift that is exact for nnz <= 128 and
-// q < nnz^2.  Fixed-shape wave reduction: bitwise reproducible.
+// One wave per owned point: lane a holds entry a of the point's row of X
+// (restricted to T), y_a = sum_b D[a][b] x_b with x_b broadcast by readlane and
+// D_B from LDS (row stride k + 1, odd, so the lanes' different rows fall in
+// different banks), then n = sum_a x_a y_a by a fixed-shape wave reduction
+// (bitwise reproducible).  D_B is staged when k (k + 1) doubles fit 56 KB
+// (stage != 0), else read from global memory; rows above 64 entries take a
+// pair loop.
 constexpr int kDensWaves = 8;
-constexpr int kDensStage = 64;
+
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
 
 __global__ void __launch_bounds__(64 * kDensWaves) k_density(const WinDesc* __restrict__ wins,
                                                              const double* __restrict__ D,
@@ -311,45 +319,43 @@ __global__ void __launch_bounds__(64 * kDensWaves) k_density(const WinDesc* __re
                                                              const double* __restrict__ xval,
                                                              double* __restrict__ dens, int stage) {
     extern __shared__ double Dl[];
-    __shared__ int32_t sidx[kDensWaves][kDensStage];
-    __shared__ double sval[kDensWaves][kDensStage];
     const int w = blockIdx.x;
     const WinDesc d = wins[w];
     const int k = d.k;
+    const int ks = stage ? k + 1 : k;   // row stride of the D_B copy read below
     const double* Dw = D + d.kk_off;
     if (stage) {
-        for (int q = threadIdx.x; q < k * k; q += blockDim.x) Dl[q] = Dw[q];
+        for (int q = threadIdx.x; q < k * k; q += blockDim.x) Dl[(q / k) * ks + q % k] = Dw[q];
         __syncthreads();
     }
+    const double* Dr = stage ? Dl : Dw;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
     for (int r = wave; r < d.nown; r += nwave) {
         const int64_t o = d.own_off + r;
         const int64_t p0 = xptr[o];
         const int nnz = (int)(xptr[o + 1] - p0);
         double acc = 0.0;
-        if (nnz <= kDensStage && nnz > 0) {
-            if (lane < nnz) {
-                sidx[wave][lane] = xidx[p0 + lane];
-                sval[wave][lane] = xval[p0 + lane];
+        if (nnz <= 64) {
+            const int ia = lane < nnz ? xidx[p0 + lane] : 0;
+            const double va = lane < nnz ? xval[p0 + lane] : 0.0;
+            const double* row = Dr + (int64_t)ia * ks;
+            // four partial sums keep four LDS reads in flight; lanes past nnz hold
+            // index 0 and value 0, so the rounded-up tail adds nothing (b + 3 < 64)
+            double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
+            for (int b = 0; b < nnz; b += 4) {
+                const int i0 = __builtin_amdgcn_readlane(ia, b), i1 = __builtin_amdgcn_readlane(ia, b + 1);
+                const int i2 = __builtin_amdgcn_readlane(ia, b + 2), i3 = __builtin_amdgcn_readlane(ia, b + 3);
+                y0 = fma(row[i0], readlane_f64(va, b), y0);
+                y1 = fma(row[i1], readlane_f64(va, b + 1), y1);
+                y2 = fma(row[i2], readlane_f64(va, b + 2), y2);
+                y3 = fma(row[i3], readlane_f64(va, b + 3), y3);
             }
-            // LDS operations of one wave execute in issue order; the wave barrier keeps
-            // the compiler from moving the reads above the writes
-            __builtin_amdgcn_wave_barrier();
-            const uint32_t magic = ((1u << 22) + (uint32_t)nnz - 1u) / (uint32_t)nnz;
-            const int npair = nnz * nnz;
-            for (int q = lane; q < npair; q += 64) {
-                const int ai = (int)(((uint32_t)q * magic) >> 22);
-                const int bi = q - ai * nnz;
-                const int ia = sidx[wave][ai], ib = sidx[wave][bi];
-                const double dv = stage ? Dl[ia * k + ib] : Dw[(int64_t)ia * k + ib];
-                acc = fma(sval[wave][ai] * dv, sval[wave][bi], acc);
-            }
-            __builtin_amdgcn_wave_barrier();     // the slot is rewritten for the next point
+            acc = va * ((y0 + y1) + (y2 + y3));
         } else {
             for (int q = lane; q < nnz * nnz; q += 64) {
                 const int ai = q / nnz, bi = q - (q / nnz) * nnz;
                 const int ia = xidx[p0 + ai], ib = xidx[p0 + bi];
-                acc = fma(xval[p0 + ai] * Dw[(int64_t)ia * k + ib], xval[p0 + bi], acc);
+                acc = fma(xval[p0 + ai] * Dr[(int64_t)ia * ks + ib], xval[p0 + bi], acc);
             }
         }
         for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
@@ -531,8 +537,8 @@ hipError_t launch_reduce(const WinDesc* wins, int nwin, int ngroup, const double
 hipError_t launch_density(const WinDesc* wins, int nwin, int kmax, const double* D, const int64_t* own_g,
                           const int64_t* xptr, const int32_t* xidx, const double* xval, double* dens,
                           hipStream_t st) {
-    const size_t lds = (size_t)kmax * kmax * sizeof(double);
-    // static staging uses 6 KB; the dynamic D block keeps the total within the 64 KB default
+    const size_t lds = (size_t)kmax * (kmax + 1) * sizeof(double);
+    // D_B with its padded row stride, within the 64 KB default of dynamic LDS
     const int stage = lds <= 56 * 1024 ? 1 : 0;
     hipLaunchKernelGGL(k_density, dim3(nwin), dim3(64 * kDensWaves), stage ? lds : 0, st, wins, D, own_g, xptr, xidx,
                        xval, dens, stage);

@@ -86,8 +86,10 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
     passes): complex64 densities differ from fp64 by ~2e-6 in this residual's
     norm, and DIIS steps built from residuals within a few times that noise
     converge slowly (measured at 1024 electrons: 15 iterations with the
-    hand-over at 1e-5, 13 at 1e-4, 12 in pure fp64), so the default floor is 1e-4.  The energy and the returned density are always fp64 results at
-    the converged input density."""
+    hand-over at 1e-5, 13 at 1e-4, 12 in pure fp64), so the default floor is
+    1e-4.  The energy and the returned density are always fp64 results at the
+    converged input density.  One device-to-host transfer per iteration (the
+    residual norm and the new row of the DIIS Gram matrix together)."""
     grid = GridOps(cr) if grid is None else grid
     torch = grid.torch
     t0 = time.perf_counter()
@@ -95,7 +97,12 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
     dV, N = cr.dV, cr.n_elec
     dens = (torch.full(cr.shape, N / (cr.ng * dV), dtype=torch.float64, device=grid.device) if dens0 is None
             else dens0.clone())
-    hist_n, hist_r, hist_g = [], [], []
+    # DIIS history in device rings (slot order oldest first in `slots`), Gram matrix
+    # of the stored residuals on the host; one device-to-host transfer per iteration
+    Rh = torch.zeros((ndiis, cr.ng), dtype=torch.float64, device=grid.device)
+    Nh = torch.zeros((ndiis, cr.ng), dtype=torch.float64, device=grid.device)
+    slots = []
+    gram = np.zeros((ndiis, ndiis))
     converged = False
     handed_over = False
     errs = []
@@ -115,7 +122,14 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
         nout = nout.reshape(cr.shape)
         t_solver += time.perf_counter() - ts
         resid = nout - dens
-        err = float(torch.sum(torch.abs(resid))) * dV / N
+        # the new history slot (the oldest one once the history is full)
+        slot = len(slots) if len(slots) < ndiis else slots[0]
+        Rh[slot] = resid.reshape(-1)
+        Nh[slot] = dens.reshape(-1)
+        live = slots[1:] + [slot] if len(slots) == ndiis else slots + [slot]
+        # |resid| and the dot products with every ring slot (unused slots are zero)
+        stats = torch.cat([torch.sum(torch.abs(resid)).reshape(1), Rh @ resid.reshape(-1)]).cpu().numpy()
+        err = float(stats[0]) * dV / N
         errs.append(err)
         if err < tol:
             converged = True
@@ -125,31 +139,25 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
             # iterations use fp64 passes (the converged step is fp64 anyway)
             solver.set_precision("fp64")
             handed_over = True
-        hist_n.append(dens.clone())
-        hist_r.append(resid.clone())
-        # Gram matrix of the residuals, one new row of dot products per iteration
-        gram = [[float(torch.sum(ri * resid)) for ri in hist_r]] if not hist_g else \
-            [row + [float(torch.sum(ri * resid))] for ri, row in zip(hist_r[:-1], hist_g)] + \
-            [[float(torch.sum(ri * resid)) for ri in hist_r]]
-        hist_g = [row[:] for row in gram]
-        if len(hist_n) > ndiis:
-            hist_n.pop(0)
-            hist_r.pop(0)
-            hist_g = [row[1:] for row in hist_g[1:]]
-        k = len(hist_n)
+        slots = live
+        for sq in slots:
+            gram[slot, sq] = gram[sq, slot] = stats[1 + sq]
+        k = len(slots)
         if k > 1:
             Bm = np.zeros((k + 1, k + 1))
-            Bm[:k, :k] = np.array(hist_g)
+            Bm[:k, :k] = gram[np.ix_(slots, slots)]
             Bm[k, :k] = Bm[:k, k] = 1.0
             rhs = np.zeros(k + 1)
             rhs[k] = 1.0
             coef = np.linalg.lstsq(Bm, rhs, rcond=None)[0][:k]
-            dens = sum(float(c) * h for c, h in zip(coef, hist_n)) + mix * sum(float(c) * r
-                                                                             for c, r in zip(coef, hist_r))
+            cfull = np.zeros(ndiis)
+            cfull[slots] = coef
+            cvec = torch.from_numpy(cfull).to(grid.device)
+            dens = (cvec @ Nh + mix * (cvec @ Rh)).reshape(cr.shape)
         else:
             dens = dens + mix * resid
         dens = torch.clamp(dens, min=1e-12)
-        dens = dens * (N / (float(dens.sum()) * dV))
+        dens = dens * (N / (dens.sum() * dV))
     if adaptive:
         # fp64 finish: mu to the strict count tolerance, density and band energy of
         # the converged input potential in fp64
