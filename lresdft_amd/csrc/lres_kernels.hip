@@ -497,8 +497,11 @@ hipError_t launch_pole_gj(int T, int G, int wide, int fp32, const WinDesc* wins,
 }
 
 int mfma_acc_regs(int fp32, int nt, int nwc) {
+    // the k-th tile column of a wave keeps the tile rows below min(nt, nwc (k + 1))
     const int tj = (nt + nwc - 1) / nwc;
-    return nt * tj * (fp32 ? 8 : 16);
+    int tiles = 0;
+    for (int j = 0; j < tj; ++j) tiles += nwc * (j + 1) < nt ? nwc * (j + 1) : nt;
+    return tiles * (fp32 ? 8 : 16);
 }
 
 static bool mfma_nwc_ok(int nt, int nwc) {

@@ -2,8 +2,7 @@
 //
 // One workgroup of NWC waves handles one (window, pole group).  Per pole it
 // holds A = z S_W - H_W, padded to 16 NT x 16 NT (zero off-diagonal, 1 on the
-// padding diagonal), as NT x NT tiles of 16 x 16 in MFMA accumulator
-// registers; wave w owns the tile columns J = w, w + NWC, ...  The
+// padding diagonal), as 16 x 16 tiles in MFMA accumulator registers.  The
 // Gauss-Jordan sweep runs in blocks of four pivots P = {4p, .., 4p + 3}.  With
 // Pi = A[P,P]^-1 and
 //     C  = A[:,P] - I[:,P],        Rt = Pi (A[P,:] + I[P,:]),
@@ -17,12 +16,24 @@
 // No pivoting: Im A = Im(z) S_W is positive definite for Im z > 0, so every
 // leading block (and every Schur complement block) is nonsingular.
 //
-// Per block the owners of the pivot tile column and row publish A[:,P] and
-// A[P,:] to LDS (double buffered: one barrier per block; real and imaginary
-// parts in separate arrays).  Every wave then
+// Symmetry: A is complex symmetric, so after the pivots of a set D the working
+// matrix is [[A_DD^-1, A_DD^-1 A_DU], [-A_UD A_DD^-1, A_UU - A_UD A_DD^-1 A_DU]]
+// (U the rest): symmetric on D x D and U x U, antisymmetric between them
+// (W[u][d] = -W[d][u]).  Only the tiles on and above the diagonal are needed:
+// the column panel A[:,P] is the row panel A[P,:] with the sign flipped on the
+// rows already pivoted (i < 4p), and the row panel left of the pivot tile is
+// the (negated, transposed) tile column above it.  Wave w owns the tile columns
+// J = w, w + NWC, ...; its k-th column keeps the tile rows I < NWC (k + 1) (all
+// of the upper tiles of that column, plus at most NWC - 1 below the diagonal,
+// which are updated like the others and never read).  The upper tiles see the
+// same arithmetic for every NWC, so every split gives the same bits.
+//
+// Per block the owners of the pivot tile row (and, left of the pivot tile, of
+// the pivot tile column) publish A[P,:] to LDS (double buffered: one barrier
+// per block; real and imaginary parts in separate arrays).  Every wave then
 // inverts the 4 x 4 pivot block on 16 lanes (lane 4i + j holds entry (i, j);
 // the pivot row, column and entry come from ds_swizzle), forms its operands C
-// and Rt from LDS and updates its tiles.
+// and Rt from the panel and updates its tiles.
 //
 // Accumulator layouts (checked on gfx950 with one-hot operands):
 //   v_mfma_f32_16x16x4f32: lane l, register r -> row 4 (l / 16) + r, column l % 16
@@ -137,11 +148,9 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
     constexpr int TJ = (NT + NWC - 1) / NWC;   // tile columns per wave
     constexpr int NR = 16 * NT;                 // padded order
     constexpr bool kEven = NT % NWC == 0;
-    // panels of the current block, planar (real and imaginary parts apart) so that
-    // accumulator registers go to LDS and come back as MFMA operands without
-    // repacking: column panel cp[b][s][i] = A[i][4p + s], row panel rp[b][j][s] = A[4p + s][j]
-    __shared__ __attribute__((aligned(16))) R cpr[2][4][NR];
-    __shared__ __attribute__((aligned(16))) R cpi[2][4][NR];
+    // row panel of the current block, planar (real and imaginary parts apart) so
+    // that accumulator registers go to LDS and come back as MFMA operands without
+    // repacking: rp[b][j][s] = A[4p + s][j]
     __shared__ __attribute__((aligned(16))) R rpr[2][NR][4];
     __shared__ __attribute__((aligned(16))) R rpi[2][NR][4];
     const int w = ids[blockIdx.x];
@@ -156,32 +165,31 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
     double* fp = Fpart + d.kk_off * ngroup + (int64_t)gq * k * k;
     const int nst = (n + 3) >> 2;
 
+    // are[I][j], aim[I][j]: tile (I, wv + NWC j), held for I < held(j) only
     V are[NT][TJ], aim[NT][TJ];
     auto owned = [&](int j) __attribute__((always_inline)) { return kEven || wv + NWC * j < NT; };
+    auto held = [](int j) __attribute__((always_inline)) { return NWC * (j + 1) < NT ? NWC * (j + 1) : NT; };
 
-    // pivot columns and rows of block p = 4 JPc + pq from tile column / row JPc into buffer b
+    // pivot rows of block p = 4 JPc + pq into buffer b: tile row JPc of the owned
+    // columns J >= JPc, and left of it the tile column JPc above the diagonal,
+    // transposed and negated (those rows are pivoted, the pivot rows are not)
     auto publish = [&](auto jpc, int pq, int b, int li, int lg) __attribute__((always_inline)) {
         constexpr int JPc = decltype(jpc)::value;
         constexpr int jc = JPc / NWC;
-        if (wv == JPc % NWC && (li >> 2) == pq) {
+        if (JPc > 0 && wv == JPc % NWC && (li >> 2) == pq) {
             const int sc = li & 3;
 #pragma unroll
-            for (int I = 0; I < NT; ++I) {
-                if constexpr (sizeof(R) == 4) {   // rows 4 lg .. 4 lg + 3: one 16-byte store
-                    *reinterpret_cast<V*>(&cpr[b][sc][16 * I + 4 * lg]) = are[I][jc];
-                    *reinterpret_cast<V*>(&cpi[b][sc][16 * I + 4 * lg]) = aim[I][jc];
-                } else {
+            for (int I = 0; I < JPc; ++I)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        cpr[b][sc][16 * I + MF::row(lg, r)] = are[I][jc][r];
-                        cpi[b][sc][16 * I + MF::row(lg, r)] = aim[I][jc][r];
-                    }
+                for (int r = 0; r < 4; ++r) {
+                    rpr[b][16 * I + MF::row(lg, r)][sc] = -are[I][jc][r];
+                    rpi[b][16 * I + MF::row(lg, r)][sc] = -aim[I][jc][r];
                 }
-            }
         }
 #pragma unroll
         for (int j = 0; j < TJ; ++j) {
-            if (!owned(j)) continue;
+            if (JPc >= held(j) || !owned(j)) continue;
+            if (wv + NWC * j < JPc) continue;
             const int col = 16 * (wv + NWC * j) + li;
             if constexpr (sizeof(R) == 4) {   // lane group pq holds rows 4 pq + s in register s
                 if (lg == pq) {
@@ -214,6 +222,7 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
         for (int I = 0; I < NT; ++I) {
 #pragma unroll
             for (int j = 0; j < TJ; ++j) {
+                if (I >= held(j)) continue;
                 const int col = 16 * (wv + NWC * j) + li;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
@@ -235,7 +244,8 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
             // this tile row's entries are formed before the next row's loads issue
             // (the memory clobber keeps those loads below this point)
 #pragma unroll
-            for (int j = 0; j < TJ; ++j) asm volatile("" : "+v"(are[I][j]), "+v"(aim[I][j])::"memory");
+            for (int j = 0; j < TJ; ++j)
+                if (I < held(j)) asm volatile("" : "+v"(are[I][j]), "+v"(aim[I][j])::"memory");
         }
         {
             int ln = lane;
@@ -264,13 +274,21 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
                 cx<R> prow[4];   // row lg of Pi
 #pragma unroll
                 for (int s = 0; s < 4; ++s) prow[s] = {__shfl(a.re, 4 * lg + s, 64), __shfl(a.im, 4 * lg + s, 64)};
-                // A operands: C[16 I + li][lg], the identity shift only in tile row JP
+                // A operands: C[16 I + li][lg] = A[16 I + li][4p + lg] - d, read from the
+                // row panel with the sign flipped on the pivoted rows (16 I + li < 4p);
+                // the identity shift only in tile row JP
                 R car[NT], cai[NT];
 #pragma unroll
                 for (int I = 0; I < NT; ++I) {
-                    car[I] = cpr[b][lg][16 * I + li];
-                    cai[I] = cpi[b][lg][16 * I + li];
-                    if (I == JP && li == 4 * pp + lg) car[I] -= R(1);
+                    R xr = rpr[b][16 * I + li][lg];
+                    R xi = rpi[b][16 * I + li][lg];
+                    if (I < JP || (I == JP && li < 4 * pp)) {
+                        xr = -xr;
+                        xi = -xi;
+                    }
+                    if (I == JP && li == 4 * pp + lg) xr -= R(1);
+                    car[I] = xr;
+                    cai[I] = xi;
                 }
                 // B operands: Rt[lg][16 J + li] = sum_s Pi[lg][s] (A[4p + s][col] + d), the
                 // identity shift only in tile column JP
@@ -306,7 +324,7 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
                 for (int I = 0; I < NT; ++I)
 #pragma unroll
                     for (int j = 0; j < TJ; ++j) {
-                        if (!owned(j)) continue;
+                        if (I >= held(j) || !owned(j)) continue;
                         are[I][j] = MF::mma(car[I], nre[j], are[I][j]);
                         are[I][j] = MF::mma(cai[I], rim[j], are[I][j]);
                         aim[I][j] = MF::mma(car[I], nim[j], aim[I][j]);
@@ -330,7 +348,7 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
         for (int I = 0; I < NT; ++I)
 #pragma unroll
             for (int j = 0; j < TJ; ++j) {
-                if (!owned(j)) continue;
+                if (I >= held(j) || !owned(j)) continue;
                 const int col = 16 * (wv + NWC * j) + ei - mm;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
@@ -353,11 +371,21 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
     }
 }
 
-// Waves per SIMD the register allocation is held to: two when the accumulators
-// of a wave leave room for the load prologue, else one.
+// Accumulator registers per lane of a variant: 8 (fp32) or 16 (fp64) per held tile.
+template <typename R, int NT, int NWC>
+constexpr int mfma_acc() {
+    constexpr int TJ = (NT + NWC - 1) / NWC;
+    int t = 0;
+    for (int j = 0; j < TJ; ++j) t += NWC * (j + 1) < NT ? NWC * (j + 1) : NT;
+    return t * (sizeof(R) == 4 ? 8 : 16);
+}
+
+// Waves per SIMD the register allocation is held to: as many as leave about 70
+// registers beside the accumulators for operands, addresses and the load prologue.
 template <typename R, int NT, int NWC>
 constexpr int mfma_waves() {
-    return NT * ((NT + NWC - 1) / NWC) * (sizeof(R) == 4 ? 8 : 16) <= 160 ? 2 : 1;
+    constexpr int acc = mfma_acc<R, NT, NWC>();
+    return acc <= 48 ? 4 : acc <= 80 ? 3 : acc <= 168 ? 2 : 1;
 }
 
 template <typename R, int NT, int NWC>

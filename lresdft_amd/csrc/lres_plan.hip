@@ -88,14 +88,16 @@ std::vector<Shape> mfma_shapes(int nmax, int fp32) {
     // group), spills allowed (used to check that every variant gives the same bits)
     const char* e = getenv("LRES_MFMA_WAVES");
     const int force = e ? atoi(e) : 0;
-    // variants without register spills; if every variant spills, the one that
-    // spills least, provided that is a few registers (at most 256 bytes per lane)
+    // variants with at most a few spilled registers (128 bytes per lane: values
+    // kept per pole, such as the pole weights and base pointers, which the
+    // autotuner's timing weighs like everything else); if every variant spills
+    // more, the one that spills least, provided that is at most 256 bytes per lane
     int least = -1, least_bytes = 1 << 30;
     for (int nwc = 1; nwc <= 8; ++nwc) {
         if (nwc > nt || (force > 0 && nwc != force)) continue;
         const int scratch = mfma_scratch_bytes(fp32, nt, nwc);
         if (scratch < 0) continue;
-        if (scratch == 0 || force > 0) out.push_back({nt, nwc, kMfma});
+        if (scratch <= 128 || force > 0) out.push_back({nt, nwc, kMfma});
         if (scratch < least_bytes) least = nwc, least_bytes = scratch;
     }
     if (out.empty() && least > 0 && least_bytes <= 256) out.push_back({nt, least, kMfma});
