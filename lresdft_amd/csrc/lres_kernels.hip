@@ -256,6 +256,12 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
     return t;   // valid in thread 0
 }
 
+// D_B = 2 sum over pole groups of the partials, F_B likewise, N_B = Tr(D_B Q_B),
+// E_B = Tr(F_B Q_B).  The pole kernels write the upper triangle of each partial
+// (G[T,T] is complex symmetric); this walks the upper triangle in row segments of
+// 32 (coalesced reads), writes each entry and its mirror, and forms the traces
+// from the upper triangle with the off-diagonal entries doubled (Q_B is
+// symmetric).
 __global__ void __launch_bounds__(256) k_reduce(const WinDesc* __restrict__ wins, int ngroup,
                                                 const double* __restrict__ Dpart, const double* __restrict__ Fpart,
                                                 const double* __restrict__ Q, double* __restrict__ D,
@@ -264,28 +270,37 @@ __global__ void __launch_bounds__(256) k_reduce(const WinDesc* __restrict__ wins
     __shared__ double red[16];
     const int w = blockIdx.x;
     const WinDesc d = wins[w];
-    const int64_t kk = (int64_t)d.k * d.k;
+    const int k = d.k;
+    const int64_t kk = (int64_t)k * k;
     const double* dp = Dpart + d.kk_off * ngroup;
     const double* fp = Fpart + d.kk_off * ngroup;
+    double* Dw = D + d.kk_off;
+    double* Fw = F + d.kk_off;
+    const double* Qw = Q + d.kk_off;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5, nty = blockDim.x >> 5;
     double sn = 0.0, se = 0.0;
-    for (int64_t q = threadIdx.x; q < kk; q += blockDim.x) {
-        // the pole kernels write the upper triangle of each partial (G[T,T] is
-        // complex symmetric): entry (i, j) is read from (min, max)
-        const int i = (int)(q / d.k), j = (int)(q - (int64_t)i * d.k);
-        const int64_t qs = i <= j ? q : (int64_t)j * d.k + i;
-        double dv = 0.0, fv = 0.0;
-        for (int gq = 0; gq < ngroup; ++gq) {
-            dv += dp[gq * kk + qs];
-            fv += fp[gq * kk + qs];
+    for (int i = ty; i < k; i += nty)
+        for (int j = i + tx; j < k; j += 32) {
+            const int64_t q = (int64_t)i * k + j;
+            double dv = 0.0, fv = 0.0;
+            for (int gq = 0; gq < ngroup; ++gq) {
+                dv += dp[gq * kk + q];
+                fv += fp[gq * kk + q];
+            }
+            dv *= 2.0;
+            fv *= 2.0;
+            Dw[q] = dv;
+            Fw[q] = fv;
+            double qv = Qw[q];
+            if (j != i) {
+                const int64_t qt = (int64_t)j * k + i;
+                Dw[qt] = dv;
+                Fw[qt] = fv;
+                qv *= 2.0;
+            }
+            sn += dv * qv;
+            se += fv * qv;
         }
-        dv *= 2.0;
-        fv *= 2.0;
-        D[d.kk_off + q] = dv;
-        F[d.kk_off + q] = fv;
-        const double qv = Q[d.kk_off + q];
-        sn += dv * qv;
-        se += fv * qv;
-    }
     const double tn = block_sum(sn, red);
     const double te = block_sum(se, red);
     if (threadIdx.x == 0) {
@@ -296,13 +311,15 @@ __global__ void __launch_bounds__(256) k_reduce(const WinDesc* __restrict__ wins
 
 // n(g) = x_g^T D_B x_g on the owned points (src/lrft_cpp.cpp:183-191 in D form).
 // One wave per owned point: lane a holds entry a of the point's row of X
-// (restricted to T), y_a = sum_b D[a][b] x_b with x_b broadcast by readlane and
-// D_B from LDS (row stride k + 1, odd, so the lanes' different rows fall in
-// different banks), then n = sum_a x_a y_a by a fixed-shape wave reduction
-// (bitwise reproducible).  D_B is staged when k (k + 1) doubles fit 56 KB
-// (stage != 0), else read from global memory; rows above 64 entries take a
-// pair loop.
-constexpr int kDensWaves = 8;
+// (restricted to T), y_a = sum_b D[b][a] x_b (D_B is symmetric, so every lane
+// reads the same row b: contiguous addresses) with b, x_b broadcast by
+// readlane, then n = sum_a x_a y_a by a fixed-shape wave reduction (bitwise
+// reproducible).  Each wave takes chunks of 63 consecutive owned points: the
+// chunk's row offsets and grid indices come in one load each, and the next
+// point's row of X is loaded while the current one is summed, so a point waits
+// on no global load of its own.  D_B is staged in LDS when k^2 doubles fit
+// 56 KB, else read from global memory; rows above 64 entries take a pair loop.
+constexpr int kDensWaves = 16;
 
 __device__ __forceinline__ double readlane_f64(double v, int l) {
     const long long b = __double_as_longlong(v);
@@ -311,55 +328,107 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
+__device__ __forceinline__ int64_t readlane_i64(int64_t v, int l) {
+    const int lo = __builtin_amdgcn_readlane((int)v, l);
+    const int hi = __builtin_amdgcn_readlane((int)(v >> 32), l);
+    return ((int64_t)hi << 32) | (int64_t)(unsigned)lo;
+}
+
+// STAGE: D_B read from its LDS copy (a separate instantiation, so that the reads
+// are LDS instructions rather than generic ones)
+template <bool STAGE>
 __global__ void __launch_bounds__(64 * kDensWaves) k_density(const WinDesc* __restrict__ wins,
                                                              const double* __restrict__ D,
                                                              const int64_t* __restrict__ own_g,
                                                              const int64_t* __restrict__ xptr,
                                                              const int32_t* __restrict__ xidx,
                                                              const double* __restrict__ xval,
-                                                             double* __restrict__ dens, int stage) {
+                                                             double* __restrict__ dens) {
     extern __shared__ double Dl[];
     const int w = blockIdx.x;
     const WinDesc d = wins[w];
     const int k = d.k;
-    const int ks = stage ? k + 1 : k;   // row stride of the D_B copy read below
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
     const double* Dw = D + d.kk_off;
-    if (stage) {
-        for (int q = threadIdx.x; q < k * k; q += blockDim.x) Dl[(q / k) * ks + q % k] = Dw[q];
+    if constexpr (STAGE) {
+        for (int i = wave; i < k; i += nwave)
+            for (int j = lane; j < k; j += 64) Dl[i * k + j] = Dw[(int64_t)i * k + j];
         __syncthreads();
     }
-    const double* Dr = stage ? Dl : Dw;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-    for (int r = wave; r < d.nown; r += nwave) {
-        const int64_t o = d.own_off + r;
-        const int64_t p0 = xptr[o];
-        const int nnz = (int)(xptr[o + 1] - p0);
-        double acc = 0.0;
-        if (nnz <= 64) {
-            const int ia = lane < nnz ? xidx[p0 + lane] : 0;
-            const double va = lane < nnz ? xval[p0 + lane] : 0.0;
-            const double* row = Dr + (int64_t)ia * ks;
-            // four partial sums keep four LDS reads in flight; lanes past nnz hold
+    auto Dr = [&](int a, int b) __attribute__((always_inline)) {
+        if constexpr (STAGE)
+            return Dl[a * k + b];
+        else
+            return Dw[(int64_t)a * k + b];
+    };
+    for (int c0 = 63 * wave; c0 < d.nown; c0 += 63 * nwave) {
+        const int cn = min(63, d.nown - c0);   // lanes 0..cn hold the cn + 1 row offsets
+        const int64_t o0 = d.own_off + c0;
+        // lane t: start of point t's row (lane cn: the end of the last row), grid index of point t
+        const int64_t pl = lane <= cn ? xptr[o0 + lane] : 0;
+        const int64_t gl = lane < cn ? own_g[o0 + lane] : 0;
+        const int64_t pn = __shfl_down(pl, 1, 64);
+        if (__ballot(lane < cn && pn - pl > 64) != 0) {
+            // a row above 64 entries in this chunk: pair loop, point by point
+            for (int t = 0; t < cn; ++t) {
+                const int64_t p0 = readlane_i64(pl, t);
+                const int nnz = (int)(readlane_i64(pl, t + 1) - p0);
+                double acc = 0.0;
+                for (int q = lane; q < nnz * nnz; q += 64) {
+                    const int ai = q / nnz, bi = q - (q / nnz) * nnz;
+                    const int ja = xidx[p0 + ai], jb = xidx[p0 + bi];
+                    acc = fma(xval[p0 + ai] * Dr(ja, jb), xval[p0 + bi], acc);
+                }
+                for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+                const int64_t g = readlane_i64(gl, t);
+                if (lane == 0) dens[g] = acc;
+            }
+            continue;
+        }
+        const int64_t pbeg = readlane_i64(pl, 0), pend = readlane_i64(pl, cn);
+        if (pend == pbeg) {   // no entries in the whole chunk
+            if (lane < cn) dens[gl] = 0.0;
+            continue;
+        }
+        // every load below is issued unconditionally at an index clamped into the
+        // chunk's entries [pbeg, pend) and masked afterwards, and the densities are
+        // stored once per chunk: the loop then holds exactly two loads per pass,
+        // so the wait before a point's sum covers its own row and not the next
+        const int64_t plast = pend - 1;
+        int nnz = (int)(readlane_i64(pl, 1) - pbeg);
+        int ia = xidx[min(pbeg + lane, plast)];
+        double va = xval[min(pbeg + lane, plast)];
+        ia = lane < nnz ? ia : 0;
+        va = lane < nnz ? va : 0.0;
+        double res = 0.0;   // lane t: density of point t
+        for (int t = 0; t < cn; ++t) {
+            // next point's row, in flight while this one is summed (after the last
+            // point its length reads as 0)
+            const int t1 = min(t + 1, cn), t2 = min(t + 2, cn);
+            const int64_t p0_n = readlane_i64(pl, t1);
+            const int nnz_n = (int)(readlane_i64(pl, t2) - p0_n);
+            int ia_n = xidx[min(p0_n + lane, plast)];
+            double va_n = xval[min(p0_n + lane, plast)];
+            // four partial sums keep four reads in flight; lanes past nnz hold
             // index 0 and value 0, so the rounded-up tail adds nothing (b + 3 < 64)
             double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
             for (int b = 0; b < nnz; b += 4) {
                 const int i0 = __builtin_amdgcn_readlane(ia, b), i1 = __builtin_amdgcn_readlane(ia, b + 1);
                 const int i2 = __builtin_amdgcn_readlane(ia, b + 2), i3 = __builtin_amdgcn_readlane(ia, b + 3);
-                y0 = fma(row[i0], readlane_f64(va, b), y0);
-                y1 = fma(row[i1], readlane_f64(va, b + 1), y1);
-                y2 = fma(row[i2], readlane_f64(va, b + 2), y2);
-                y3 = fma(row[i3], readlane_f64(va, b + 3), y3);
+                y0 = fma(Dr(i0, ia), readlane_f64(va, b), y0);
+                y1 = fma(Dr(i1, ia), readlane_f64(va, b + 1), y1);
+                y2 = fma(Dr(i2, ia), readlane_f64(va, b + 2), y2);
+                y3 = fma(Dr(i3, ia), readlane_f64(va, b + 3), y3);
             }
-            acc = va * ((y0 + y1) + (y2 + y3));
-        } else {
-            for (int q = lane; q < nnz * nnz; q += 64) {
-                const int ai = q / nnz, bi = q - (q / nnz) * nnz;
-                const int ia = xidx[p0 + ai], ib = xidx[p0 + bi];
-                acc = fma(xval[p0 + ai] * Dr[(int64_t)ia * ks + ib], xval[p0 + bi], acc);
-            }
+            double acc = va * ((y0 + y1) + (y2 + y3));
+            for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+            const double tot = readlane_f64(acc, 0);
+            res = lane == t ? tot : res;
+            nnz = nnz_n;
+            ia = lane < nnz_n ? ia_n : 0;
+            va = lane < nnz_n ? va_n : 0.0;
         }
-        for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-        if (lane == 0) dens[own_g[o]] = acc;
+        if (lane < cn) dens[gl] = res;
     }
 }
 
@@ -540,11 +609,15 @@ hipError_t launch_reduce(const WinDesc* wins, int nwin, int ngroup, const double
 hipError_t launch_density(const WinDesc* wins, int nwin, int kmax, const double* D, const int64_t* own_g,
                           const int64_t* xptr, const int32_t* xidx, const double* xval, double* dens,
                           hipStream_t st) {
-    const size_t lds = (size_t)kmax * (kmax + 1) * sizeof(double);
-    // D_B with its padded row stride, within the 64 KB default of dynamic LDS
+    const size_t lds = (size_t)kmax * kmax * sizeof(double);
+    // D_B in LDS within the 64 KB default of dynamic LDS
     const int stage = lds <= 56 * 1024 ? 1 : 0;
-    hipLaunchKernelGGL(k_density, dim3(nwin), dim3(64 * kDensWaves), stage ? lds : 0, st, wins, D, own_g, xptr, xidx,
-                       xval, dens, stage);
+    if (stage)
+        hipLaunchKernelGGL(k_density<true>, dim3(nwin), dim3(64 * kDensWaves), lds, st, wins, D, own_g, xptr, xidx,
+                           xval, dens);
+    else
+        hipLaunchKernelGGL(k_density<false>, dim3(nwin), dim3(64 * kDensWaves), 0, st, wins, D, own_g, xptr, xidx,
+                           xval, dens);
     return hipGetLastError();
 }
 
