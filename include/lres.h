@@ -81,10 +81,10 @@ int lres_plan_sizes(const lres_plan* plan, int32_t* h_n, int32_t* h_k);
  * the first pass in that precision has run the autotuner).  kind 0 / 1: vector
  * ALU Gauss-Jordan with a T x T tile per thread on a G x G grid and a register
  * budget of 128 / 512; kind 2: matrix cores (MFMA), T = 16 x 16 tiles per
- * dimension, G = waves per (window, pole group).  By default fp32 buckets use
- * kind 2 and fp64 buckets kinds 0 / 1; the environment variable
- * LRES_POLE_KERNEL=mfma or =valu, read at plan creation and tuning, selects one
- * family for both. */
+ * dimension, G = waves per (window, pole group).  By default buckets of up to
+ * 128 functions use kind 2 in both precisions and larger ones the vector-ALU
+ * kernels; the environment variable LRES_POLE_KERNEL=valu, read at plan
+ * creation and tuning, selects the vector-ALU family for both. */
 int lres_plan_buckets(const lres_plan* plan, int32_t* h_out, int cap, int* h_nbucket);
 
 /* Set precision (LRES_FP64 / LRES_MIXED) for later passes. */

@@ -74,15 +74,20 @@ __global__ void __launch_bounds__(256) k_gather_csr(const WinDesc* __restrict__ 
 constexpr int kMaxG = 32;
 
 // Gauss-Jordan sweep without fix-up branches.  For pivot p with value a = A[p][p]
-// the owners publish the pivot row with its pivot entry replaced by a + 1 and
-// the pivot column with its pivot entry replaced by a - 1; then the plain
-// rank-1 update A -= col (row / a) gives exactly the in-place Gauss-Jordan step:
-// row p -> row p / a, column p -> -column p / a, A[p][p] -> 1 / a, all other
-// entries A - col row / a.  (Proof: A[i][p] - A[i][p] (a+1)/a = -A[i][p]/a;
+// take the pivot row with its pivot entry replaced by a + 1 and the pivot
+// column with its pivot entry replaced by a - 1; then the plain rank-1 update
+// A -= col (row / a) gives exactly the in-place Gauss-Jordan step: row p ->
+// row p / a, column p -> -column p / a, A[p][p] -> 1 / a, all other entries
+// A - col row / a.  (Proof: A[i][p] - A[i][p] (a+1)/a = -A[i][p]/a;
 // A[p][j] - (a-1) A[p][j]/a = A[p][j]/a; a - (a-1)(a+1)/a = 1/a.)  The same
 // update serves the Schur pivots 0..m-1: after all n pivots the T x T block
-// holds (A^-1)[T, T] either way.  Rounding matches the branching form
-// (checked against it to 1e-14 relative in fp64 on these pencils).
+// holds (A^-1)[T, T] either way.
+//
+// A is complex symmetric, and after the pivots of a set D the in-place state is
+// symmetric on D x D and on the rest U x U and antisymmetric between them
+// (W[u][d] = -W[d][u]).  So only the entries on and above the diagonal are
+// updated and only the pivot row is published: column p is read from it, with
+// the sign flipped on the rows already pivoted.
 
 // x - c r with the column factor passed pre-negated: nc = -c, ci = c.im
 template <typename R>
@@ -95,26 +100,26 @@ __device__ __forceinline__ cx<R> cupd(cx<R> x, R ncr, R nci, R ci, cx<R> r) {
     return {re, im};
 }
 
-// Publish pivot q (tile slot PQ, compile-time after unrolling) from the owners'
-// registers: row q by thread row q % G, column q by thread column q % G; the
-// owner of (q, q) writes the shifted pivot entries and 1 / A[q][q].
+// Publish the row of pivot q (tile slot PQ, compile-time after unrolling) from the
+// owners' registers: the entries right of the diagonal (columns >= q) from thread
+// row q % G, those left of it from thread column q % G as -A[j][q] (rows j < q are
+// pivoted, q is not, so A[q][j] = -A[j][q]).  Raw values: the +1 / -1 shifts of
+// the pivot entry are applied where the row is read.
 template <int T, typename R>
-__device__ __forceinline__ void publish_pivot(const cx<R> (&A)[T][T], int PQ, bool own_row, bool own_col, int G,
-                                              int tx, int ty, cx<R>* rowq, cx<R>* colq) {
+__device__ __forceinline__ void publish_pivot(const cx<R> (&A)[T][T], int PQ, int q, bool own_row, bool own_col,
+                                              int G, int tx, int ty, cx<R>* rowq) {
     if (own_row) {
 #pragma unroll
         for (int j = 0; j < T; ++j) {
-            cx<R> v = A[PQ][j];
-            if (own_col && j == PQ) v.re += R(1);
-            rowq[tx + G * j] = v;
+            const int col = tx + G * j;
+            if (col >= q) rowq[col] = A[PQ][j];
         }
     }
     if (own_col) {
 #pragma unroll
         for (int i = 0; i < T; ++i) {
-            cx<R> v = A[i][PQ];
-            if (own_row && i == PQ) v.re -= R(1);
-            colq[ty + G * i] = v;
+            const int row = ty + G * i;
+            if (row < q) rowq[row] = {-A[i][PQ].re, -A[i][PQ].im};
         }
     }
 }
@@ -125,7 +130,6 @@ __global__ void __launch_bounds__(MAXT) k_pole_gj(const WinDesc* __restrict__ wi
                                                   int M, int ngroup, int G, double c, double Rr,
                                                   double* __restrict__ Dpart, double* __restrict__ Fpart) {
     __shared__ cx<R> rowbuf[2][kMaxG * T];
-    __shared__ cx<R> colbuf[2][kMaxG * T];
     __shared__ cx<R> invbuf[2];
     const int w = ids[blockIdx.x];
     const int g = blockIdx.y;
@@ -153,6 +157,7 @@ __global__ void __launch_bounds__(MAXT) k_pole_gj(const WinDesc* __restrict__ wi
             const int r = ty + G * i;
 #pragma unroll
             for (int j = 0; j < T; ++j) {
+                if (j < i) continue;   // never used (below the diagonal)
                 const int col = tx + G * j;
                 const bool in = r < n && col < n;
                 const double h = in ? Hq[(int64_t)r * n + col] : (r == col ? -1.0 : 0.0);
@@ -161,7 +166,7 @@ __global__ void __launch_bounds__(MAXT) k_pole_gj(const WinDesc* __restrict__ wi
             }
             asm volatile("" ::: "memory");
         }
-        publish_pivot<T, R>(A, 0, ty == 0, tx == 0, G, tx, ty, rowbuf[0], colbuf[0]);
+        publish_pivot<T, R>(A, 0, 0, ty == 0, tx == 0, G, tx, ty, rowbuf[0]);
         if (ty == 0 && tx == 0) invbuf[0] = crecip(A[0][0]);
         // pivot p = G PI + pr: the tile slot PI is a compile-time constant in each copy
 #pragma unroll
@@ -174,10 +179,19 @@ __global__ void __launch_bounds__(MAXT) k_pole_gj(const WinDesc* __restrict__ wi
                 cx<R> rs[T];
                 R ncr[T], nci[T], ci[T];
 #pragma unroll
-                for (int j = 0; j < T; ++j) rs[j] = cmul(rowbuf[buf][tx + G * j], inv);
+                for (int j = 0; j < T; ++j) {
+                    const int col = tx + G * j;
+                    cx<R> v = rowbuf[buf][col];
+                    if (col == p) v.re += R(1);
+                    rs[j] = cmul(v, inv);
+                }
+                // column p from the row: A[r][p] = -A[p][r] on the pivoted rows r < p
 #pragma unroll
                 for (int i = 0; i < T; ++i) {
-                    const cx<R> cv = colbuf[buf][ty + G * i];
+                    const int r = ty + G * i;
+                    cx<R> cv = rowbuf[buf][r];
+                    if (r < p) cv = {-cv.re, -cv.im};
+                    if (r == p) cv.re -= R(1);
                     ncr[i] = -cv.re;
                     nci[i] = -cv.im;
                     ci[i] = cv.im;
@@ -196,16 +210,19 @@ __global__ void __launch_bounds__(MAXT) k_pole_gj(const WinDesc* __restrict__ wi
                         if (s == PQ) a = cupd(A[s][s], ncr[s], nci[s], ci[s], rs[s]);
                     invbuf[buf ^ 1] = crecip(a);
                 }
-                // rank-1 update of the whole tile
+                // rank-1 update of the tile entries i <= j: every entry on or above the
+                // diagonal that this thread owns is one of them (an entry with i > j has
+                // row - col >= G (i - j) - (G - 1) > 0)
 #pragma unroll
                 for (int i = 0; i < T; ++i)
 #pragma unroll
-                    for (int j = 0; j < T; ++j) A[i][j] = cupd(A[i][j], ncr[i], nci[i], ci[i], rs[j]);
+                    for (int j = 0; j < T; ++j)
+                        if (j >= i) A[i][j] = cupd(A[i][j], ncr[i], nci[i], ci[i], rs[j]);
                 if (pr + 1 < G)
-                    publish_pivot<T, R>(A, PI, own_row, own_col, G, tx, ty, rowbuf[buf ^ 1], colbuf[buf ^ 1]);
+                    publish_pivot<T, R>(A, PI, q, own_row, own_col, G, tx, ty, rowbuf[buf ^ 1]);
                 else
-                    publish_pivot<T, R>(A, PI + 1 < T ? PI + 1 : T - 1, own_row, own_col, G, tx, ty, rowbuf[buf ^ 1],
-                                        colbuf[buf ^ 1]);
+                    publish_pivot<T, R>(A, PI + 1 < T ? PI + 1 : T - 1, q, own_row, own_col, G, tx, ty,
+                                        rowbuf[buf ^ 1]);
             }
         }
         __syncthreads();   // the next pole's prologue rewrites buffer 0
@@ -221,6 +238,7 @@ __global__ void __launch_bounds__(MAXT) k_pole_gj(const WinDesc* __restrict__ wi
             const int r = ty + G * i - mm;
 #pragma unroll
             for (int j = 0; j < T; ++j) {
+                if (j < i) continue;   // below the diagonal
                 const int col = tx + G * j - mm;
                 if (r >= 0 && r <= col && col < kk) {
                     const double gr = (double)A[i][j].re, gi = (double)A[i][j].im;

@@ -67,17 +67,17 @@ struct Shape {
 };
 constexpr int kMfma = 2;
 
-// Which pole kernel family runs: complex64 passes on the matrix cores, fp64
-// passes on the vector ALUs (measured faster there: the fp64 MFMA rate equals
-// the fp64 vector rate, and its accumulators cost twice the registers).
-// LRES_POLE_KERNEL=mfma or =valu selects one family for both precisions.  One
-// family per bucket and precision: the autotuner only chooses among variants
-// that give bitwise identical results.
-bool use_mfma(int fp32) {
+// Which pole kernel family runs: the matrix-core kernel in both precisions for
+// windows of up to 128 functions (in fp64 its four-pivot blocks need a quarter
+// of the vector kernel's barriers: 1.71 against 2.08 ms per pass at 1024
+// electrons, although the fp64 MFMA rate only equals the fp64 vector rate),
+// the vector-ALU kernels beyond.  LRES_POLE_KERNEL=mfma or =valu selects one
+// family for both precisions.  One family per bucket and precision: the
+// autotuner only chooses among variants that give bitwise identical results.
+bool use_mfma(int /*fp32*/) {
     const char* e = getenv("LRES_POLE_KERNEL");
     if (e && std::string(e) == "valu") return false;
-    if (e && std::string(e) == "mfma") return true;
-    return fp32 != 0;
+    return true;
 }
 
 std::vector<Shape> mfma_shapes(int nmax, int fp32) {
