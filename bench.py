@@ -44,6 +44,9 @@ def parse():
                          "periodic cube at the density of the a = 6.0 lattice (uneven windows)")
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every SCF iteration's mixing/potential/H(v) step eagerly instead of "
+                         "replaying it from a captured hipGraph")
     ap.add_argument("--no-forces", action="store_true",
                     help="skip the adjoint / band-energy forces at the converged potential (after the timed region)")
     ap.add_argument("--forces-fd", type=int, default=0, metavar="K",
@@ -55,7 +58,7 @@ def parse():
     return ap.parse_args()
 
 
-def run_scf(C, cr, grid, vion, precision, tol):
+def run_scf(C, cr, grid, vion, precision, tol, graph=True):
     import torch
     from lresdft_amd.scf import scf_lcao
     C.set_precision(precision)
@@ -65,7 +68,7 @@ def run_scf(C, cr, grid, vion, precision, tol):
     p0 = C.passes
     torch.cuda.synchronize()
     t = time.perf_counter()
-    r = scf_lcao(cr, C, tol=tol, grid=grid, vion=vion)
+    r = scf_lcao(cr, C, tol=tol, grid=grid, vion=vion, graph=graph)
     torch.cuda.synchronize()
     r["wall"] = time.perf_counter() - t
     r["passes"] = C.passes - p0
@@ -274,15 +277,15 @@ def main():
         return
 
     for _ in range(args.warmup):
-        run_scf(C, cr, grid, vion, "fp64", args.tol)
-        run_scf(C, cr, grid, vion, "mixed", args.tol)
+        run_scf(C, cr, grid, vion, "fp64", args.tol, not args.no_graph)
+        run_scf(C, cr, grid, vion, "mixed", args.tol, not args.no_graph)
 
     C.plan.stats(reset=True)
-    fp64 = [run_scf(C, cr, grid, vion, "fp64", args.tol) for _ in range(args.steps)]
+    fp64 = [run_scf(C, cr, grid, vion, "fp64", args.tol, not args.no_graph) for _ in range(args.steps)]
     st64 = C.plan.stats(reset=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    mixed = [run_scf(C, cr, grid, vion, "mixed", args.tol) for _ in range(args.steps)]
+    mixed = [run_scf(C, cr, grid, vion, "mixed", args.tol, not args.no_graph) for _ in range(args.steps)]
     torch.cuda.synchronize()
     t_total = time.perf_counter() - t0
     st32 = C.plan.stats(reset=True)
@@ -306,6 +309,8 @@ def main():
         "passes_per_scf": {"fp64": fp64[-1]["passes"], "mixed": mixed[-1]["passes"]},
         "history": {"fp64": fp64[-1]["history"], "mixed": mixed[-1]["history"]},
         "passes_per_iteration": {"fp64": fp64[-1]["pass_log"], "mixed": mixed[-1]["pass_log"]},
+        "graph": {"mixing_potential_H_step_replayed": not args.no_graph,
+                  "replays_per_scf": {"fp64": fp64[-1]["graph_replays"], "mixed": mixed[-1]["graph_replays"]}},
         "roofline": roofline(st32["fp32"], flops, PEAK_FP32, kernel_name(bk[0], True),
                              measured_traffic("fp32") if default_workload else None),
         "roofline_fp64": roofline(st64["fp64"], flops, PEAK_FP64, kernel_name(bk[0], False),

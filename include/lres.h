@@ -110,8 +110,27 @@ int lres_plan_set_basis(lres_plan* plan, const int64_t* h_atom_of_fn, int64_t na
                         const int64_t* h_Wflat, const int64_t* h_Woff);
 
 /* H_W = T_W + [X^T diag(v) X dV]_W for every window from a device potential
- * d_v (ng doubles, C order of the grid); afterwards passes use this H. */
+ * d_v (ng doubles, C order of the grid); afterwards passes use this H.  Only
+ * kernel launches on the plan's stream, so it can be captured into a graph. */
 int lres_set_potential(lres_plan* plan, const double* d_v);
+
+/* Make `stream` (hipStream_t, NULL = the null stream) the plan's stream for the
+ * calls that follow, e.g. a stream being captured into a graph.  Ordering work
+ * already queued on the previous stream is the caller's job.  No counterpart in
+ * the reference (its pass is a blocking host call); for callers that capture
+ * the SCF iteration. */
+int lres_plan_set_stream(lres_plan* plan, void* stream);
+
+/* The SCF driver's density mixing (lcao3d.py:268-281 in DIIS-coefficient form):
+ * d_out = max(sum_h c_h (N_h + mix R_h), lo), then d_out *= ntot / (sum d_out * dV),
+ * with c = d_coef (nhist device doubles), N_h / R_h rows h of d_Nh / d_Rh
+ * (nhist x ng, the stored input densities and residuals).  The sum has a fixed
+ * order; d_part is a device workspace of LRES_MIX_PARTS doubles, fully written,
+ * so nothing needs clearing and the launches can be captured into a graph.
+ * Kernel launches only, on `stream`. */
+#define LRES_MIX_PARTS 1024
+int lres_diis_mix(const double* d_coef, int nhist, const double* d_Nh, const double* d_Rh, int64_t ng, double mix,
+                  double lo, double ntot, double dV, double* d_out, double* d_part, void* stream);
 
 /* One window pass at (mu, e_lo): the pole form of lrft_windows
  * (src/lrft_cpp.cpp:129-233).  Per window B it forms

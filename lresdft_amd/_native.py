@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(_HERE, "liblres_hip.so")
 
 LRES_FP64 = 0
 LRES_MIXED = 1
+LRES_MIX_PARTS = 1024   # workspace doubles of lres_diis_mix (include/lres.h)
 
 _i64p = C.POINTER(C.c_int64)
 _i32p = C.POINTER(C.c_int32)
@@ -40,6 +41,9 @@ SIGNATURES = {
     "lres_plan_set_basis": (C.c_int, [_vp, _i64p, C.c_int64, _i64p, _i64p, _f64p, _i64p, _i64p, _f64p,
                                       _i64p, _i64p]),
     "lres_set_potential": (C.c_int, [_vp, _vp]),
+    "lres_plan_set_stream": (C.c_int, [_vp, _vp]),
+    "lres_diis_mix": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double,
+                                _vp, _vp, _vp]),
     "lres_window_pass": (C.c_int, [_vp, C.c_double, C.c_double, _vp, _vp, _vp, _f64p]),
     "lres_density_last": (C.c_int, [_vp, _vp]),
     "lres_inertia": (C.c_int, [_vp, _f64p, C.c_int, _i64p, _vp]),

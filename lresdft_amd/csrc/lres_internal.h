@@ -29,6 +29,10 @@ hipError_t launch_density(const lres::WinDesc* wins, int nwin, int kmax, const d
 hipError_t launch_sum2(const double* a, const double* b, int n, double* out, hipStream_t st);
 hipError_t launch_voronoi_owner(const double* cells, int ncell, const double* box, const int64_t* shape, double dx,
                                 int32_t* owner, hipStream_t st);
+// SCF density mixing: grid and block counts of its fixed-order sum (kMixBlocks partials)
+constexpr int kMixBlocks = 1024, kMixThreads = 256;
+hipError_t launch_diis_mix(const double* coef, int nh, const double* Nh, const double* Rh, int64_t ng, double mix,
+                           double lo, double ntot, double dV, double* out, double* part, hipStream_t st);
 hipError_t launch_diag_ratio(const lres::WinDesc* wins, int nwin, const double* SW, const double* HW, double* out,
                              hipStream_t st);
 hipError_t launch_inertia(int T, int G, const lres::WinDesc* wins, const int32_t* ids, int nids, const double* SW,

@@ -450,6 +450,44 @@ __global__ void __launch_bounds__(64 * kDensWaves) k_density(const WinDesc* __re
     }
 }
 
+// ---------------------------------------------------------------------------
+// SCF density mixing (the caller's DIIS step): out = max(sum_h c_h (N_h + mix R_h), lo),
+// then out *= ntot / (sum(out) dV).  Fixed grid and block shapes give the sum a fixed
+// order (bitwise reproducible, also when the launches are replayed from a graph), and
+// every partial slot is written, so nothing has to be cleared beforehand.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kMixThreads) k_mix(const double* __restrict__ coef, int nh,
+                                                     const double* __restrict__ Nh, const double* __restrict__ Rh,
+                                                     int64_t ng, double mix, double lo, double* __restrict__ out,
+                                                     double* __restrict__ part) {
+    __shared__ double red[16];
+    double s = 0.0;
+    for (int64_t g = (int64_t)blockIdx.x * kMixThreads + threadIdx.x; g < ng; g += (int64_t)kMixBlocks * kMixThreads) {
+        double v = 0.0;
+        for (int h = 0; h < nh; ++h) v = fma(coef[h], fma(mix, Rh[(int64_t)h * ng + g], Nh[(int64_t)h * ng + g]), v);
+        v = fmax(v, lo);
+        out[g] = v;
+        s += v;
+    }
+    const double t = block_sum(s, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// every block forms the same total from the kMixBlocks partials in the same order
+__global__ void __launch_bounds__(kMixThreads) k_mix_scale(const double* __restrict__ part, double ntot, double dV,
+                                                           int64_t ng, double* __restrict__ out) {
+    __shared__ double red[16];
+    __shared__ double scale;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < kMixBlocks; b += kMixThreads) s += part[b];
+    const double t = block_sum(s, red);
+    if (threadIdx.x == 0) scale = ntot / (t * dV);
+    __syncthreads();
+    const double sc = scale;
+    for (int64_t g = (int64_t)blockIdx.x * kMixThreads + threadIdx.x; g < ng; g += (int64_t)kMixBlocks * kMixThreads)
+        out[g] *= sc;
+}
+
 // sum of nwin values in a fixed order into out[0] (one block)
 __global__ void k_sum2(const double* __restrict__ a, const double* __restrict__ b, int n, double* __restrict__ out) {
     __shared__ double red[16];
@@ -537,6 +575,15 @@ __global__ void __launch_bounds__(1024) k_inertia(const WinDesc* __restrict__ wi
 }  // namespace lres
 
 using namespace lres;
+
+hipError_t launch_diis_mix(const double* coef, int nh, const double* Nh, const double* Rh, int64_t ng, double mix,
+                           double lo, double ntot, double dV, double* out, double* part, hipStream_t st) {
+    hipLaunchKernelGGL(k_mix, dim3(kMixBlocks), dim3(kMixThreads), 0, st, coef, nh, Nh, Rh, ng, mix, lo, out, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_mix_scale, dim3(kMixBlocks), dim3(kMixThreads), 0, st, part, ntot, dV, ng, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_gather_csr(const WinDesc* wins, int nwin, const int64_t* Wflat, const int64_t* Woff,
                              const int32_t* pos, const int64_t* Hp, const int64_t* Hi, const double* Hv,

@@ -886,7 +886,11 @@ int lres_set_potential(lres_plan* P, const double* d_v) {
     return guarded([&] {
         require(P != nullptr, "null plan");
         require(P->have_basis, "no basis: call lres_plan_set_basis first");
-        hip_check(hipEventRecord(P->ev[0], P->st), "event");
+        // no timing events while the stream is being captured into a graph
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        hip_check(hipStreamIsCapturing(P->st, &cs), "capture status");
+        const bool timed = cs == hipStreamCaptureStatusNone;
+        if (timed) hip_check(hipEventRecord(P->ev[0], P->st), "event");
         hip_check(launch_pair_v(P->nf_max, (int)P->npair, P->d_pairA.p, P->d_pairB.p, P->d_pent_off.p, P->d_ent_r.p,
                                 P->d_ent_a.p, P->d_ent_b.p, P->d_xat_off.p, P->d_nf_of.p, P->d_xat.p, P->d_vp_off.p,
                                 d_v, P->dV, P->d_vpair.p, P->st),
@@ -895,8 +899,28 @@ int lres_set_potential(lres_plan* P, const double* d_v) {
                                    P->d_wat_g.p, P->d_wpt_off.p, P->d_pairtab.p, P->d_nf_of.p, P->d_vp_off.p,
                                    P->d_vpair.p, P->d_TW.p, P->d_HW.p, P->st),
                   "window H kernel");
-        hip_check(hipEventRecord(P->ev[1], P->st), "event");
+        if (timed) hip_check(hipEventRecord(P->ev[1], P->st), "event");
         P->have_H = true;
+    });
+}
+
+int lres_plan_set_stream(lres_plan* P, void* stream) {
+    return guarded([&] {
+        require(P != nullptr, "null plan");
+        P->st = (hipStream_t)stream;
+    });
+}
+
+static_assert(kMixBlocks == LRES_MIX_PARTS, "mixing workspace size");
+
+int lres_diis_mix(const double* d_coef, int nhist, const double* d_Nh, const double* d_Rh, int64_t ng, double mix,
+                  double lo, double ntot, double dV, double* d_out, double* d_part, void* stream) {
+    return guarded([&] {
+        require(nhist >= 1 && ng >= 1, "empty history or grid");
+        require(d_coef && d_Nh && d_Rh && d_out && d_part, "null buffer");
+        hip_check(launch_diis_mix(d_coef, nhist, d_Nh, d_Rh, ng, mix, lo, ntot, dV, d_out, d_part,
+                                  (hipStream_t)stream),
+                  "mix kernels");
     });
 }
 

@@ -153,6 +153,12 @@ class WindowPlan:
         self.last = None
         nat.check(nat.lib().lres_set_potential(self.h, v_dev.data_ptr()))
 
+    def set_stream(self, stream):
+        """Launch the plan's later work on `stream` (a torch stream; ordering against
+        work already queued elsewhere is the caller's job), e.g. a capturing stream."""
+        nat.check(nat.lib().lres_plan_set_stream(self.h, nat.C.c_void_p(stream.cuda_stream)))
+        self.stream = stream
+
     def set_h_device(self, Hp, Hi, Hv):
         self.last = None
         nat.check(nat.lib().lres_set_hamiltonian_csr_device(self.h, Hp.data_ptr(), Hi.data_ptr(), Hv.data_ptr()))
@@ -443,15 +449,20 @@ class LRFTLCAOCpp:
         dens, e_band, mu = self._density_loaded(N)
         return dens.cpu().numpy().reshape(cr.shape), e_band, mu
 
-    def density_device(self, v_dev, N=None):
+    def density_device(self, v_dev, N=None, potential_set=False):
         """density() for a device potential: returns (dens tensor [ng] incl. spin 2, e_band, mu).
 
         Needs device_h (H(v) assembled on the GPU); the density tensor is the
-        plan's buffer scaled by 2, valid until the next call."""
+        plan's buffer scaled by 2, valid until the next call.  potential_set: the
+        window blocks of v_dev are already in place (assembled by a replayed graph)."""
         if not self.device_h:
             raise RuntimeError("density_device needs device_h=True")
         N = self.b.cr.n_elec if N is None else N
-        self.plan.set_potential(v_dev)
+        if potential_set:
+            self.plan.last = None
+            self.plan._v_keep = v_dev
+        else:
+            self.plan.set_potential(v_dev)
         p0 = self.plan.passes
         try:
             out = self._density_loaded(N)

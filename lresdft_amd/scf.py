@@ -13,6 +13,8 @@ import time
 
 import numpy as np
 
+from . import _native as nat
+
 
 class GridOps:
     """FFT Hartree, ion potential and PZ81 LDA on the device for one crystal."""
@@ -75,7 +77,75 @@ class GridOps:
         return ex + torch.where(high, ec_h, ec_l), vx + torch.where(high, vc_h, vc_l)
 
 
-def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion=None, dens0=None, fp32_floor=1e-4):
+class PotentialStep:
+    """The device work of one SCF iteration between two host decisions: the DIIS
+    step `dens = max(sum_h c_h (N_h + mix R_h), 1e-12)` renormalised to N (one
+    fixed-order library call, lres_diis_mix), then `v = v_ion + v_H(dens) +
+    v_xc(dens)` (FFT Hartree, PZ81) and the window blocks H_W(v) (lres_set_potential).
+    With graph=True the sequence is captured once into a hipGraph (torch.cuda.CUDAGraph
+    on a side stream, the plan switched to that stream while capturing) and replayed
+    every iteration; the coefficients travel through a pinned host buffer and a fixed
+    device buffer, the outputs (dens, v) live at fixed addresses.  Eager and replayed
+    steps run the same kernels in the same order, so they give the same bits."""
+
+    def __init__(self, cr, grid, solver, vion, Nh, Rh, mix, graph=True):
+        torch = grid.torch
+        self.torch, self.cr, self.grid, self.solver, self.vion = torch, cr, grid, solver, vion
+        self.Nh, self.Rh, self.mix = Nh, Rh, float(mix)
+        self.use_graph = bool(graph)
+        self.key = (id(grid), id(vion), vion.data_ptr(), Nh.shape[0], self.mix, self.use_graph)
+        nh = Nh.shape[0]
+        self.coef_host = torch.zeros(nh, dtype=torch.float64).pin_memory()
+        self.coef = torch.zeros(nh, dtype=torch.float64, device=grid.device)
+        self.dens = torch.empty(cr.ng, dtype=torch.float64, device=grid.device)
+        self.part = torch.empty(nat.LRES_MIX_PARTS, dtype=torch.float64, device=grid.device)
+        self.v = None
+        self.graph = None
+        self.replays = 0
+
+    def _body(self):
+        torch = self.torch
+        nat.check(nat.lib().lres_diis_mix(self.coef.data_ptr(), self.Nh.shape[0], self.Nh.data_ptr(),
+                                          self.Rh.data_ptr(), self.cr.ng, self.mix, 1e-12, float(self.cr.n_elec),
+                                          float(self.cr.dV), self.dens.data_ptr(), self.part.data_ptr(),
+                                          nat.C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        d = self.dens.reshape(self.cr.shape)
+        v = (self.vion + self.grid.hartree(d) + self.grid.lda_xc(d)[1]).reshape(-1)
+        self.solver.plan.set_potential(v)
+        return v
+
+    def _capture(self):
+        torch = self.torch
+        plan = self.solver.plan
+        home = plan.stream
+        side = torch.cuda.Stream()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        plan.set_stream(side)
+        try:
+            with torch.cuda.graph(g, stream=side):
+                self.v = self._body()
+        finally:
+            plan.set_stream(home)
+        self.graph = g
+
+    def run(self, coef):
+        """Mix with the coefficient vector `coef` (numpy, one per history row) and
+        assemble H_W of the new potential; returns (density [ng], potential [ng])."""
+        self.coef_host.numpy()[:] = coef
+        self.coef.copy_(self.coef_host, non_blocking=True)
+        if self.use_graph:
+            if self.graph is None:
+                self._capture()
+            self.graph.replay()
+            self.replays += 1
+        else:
+            self.v = self._body()
+        return self.dens, self.v
+
+
+def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion=None, dens0=None, fp32_floor=1e-4,
+             graph=True):
     """Device SCF; ``solver.density_device(v)`` returns (density tensor [ng], e_band, mu).
 
     A solver with ``count_tol``/``refine`` (``LRFTLCAOCpp``) solves mu each step
@@ -89,7 +159,10 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
     hand-over at 1e-5, 13 at 1e-4, 12 in pure fp64), so the default floor is
     1e-4.  The energy and the returned density are always fp64 results at the
     converged input density.  One device-to-host transfer per iteration (the
-    residual norm and the new row of the DIIS Gram matrix together)."""
+    residual norm and the new row of the DIIS Gram matrix together).  The
+    mixing, potential and H(v) of every iteration after the first form one
+    PotentialStep, replayed from a captured graph when ``graph`` is true (same
+    results as eager launches, bit for bit)."""
     grid = GridOps(cr) if grid is None else grid
     torch = grid.torch
     t0 = time.perf_counter()
@@ -99,8 +172,23 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
             else dens0.clone())
     # DIIS history in device rings (slot order oldest first in `slots`), Gram matrix
     # of the stored residuals on the host; one device-to-host transfer per iteration
-    Rh = torch.zeros((ndiis, cr.ng), dtype=torch.float64, device=grid.device)
-    Nh = torch.zeros((ndiis, cr.ng), dtype=torch.float64, device=grid.device)
+    # the step (its graph and the history rings it reads by address) is kept on the
+    # solver and reused by later SCF runs with the same grid, ion potential and mixing
+    step = None
+    if hasattr(solver, "plan"):
+        step = getattr(solver, "_scf_step", None)
+        key = (id(grid), id(vion), vion.data_ptr(), ndiis, float(mix), bool(graph))
+        if step is None or step.key != key or step.cr is not cr:
+            ring = torch.zeros((2, ndiis, cr.ng), dtype=torch.float64, device=grid.device)
+            step = PotentialStep(cr, grid, solver, vion, ring[0], ring[1], mix, graph)
+            solver._scf_step = step
+        Nh, Rh = step.Nh, step.Rh
+        Nh.zero_()
+        Rh.zero_()
+    else:
+        Rh = torch.zeros((ndiis, cr.ng), dtype=torch.float64, device=grid.device)
+        Nh = torch.zeros((ndiis, cr.ng), dtype=torch.float64, device=grid.device)
+    replays0 = step.replays if step is not None else 0
     slots = []
     gram = np.zeros((ndiis, ndiis))
     converged = False
@@ -109,8 +197,21 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
     t_solver = 0.0
     err = None
     adaptive = hasattr(solver, "count_tol") and hasattr(solver, "refine")
+    cfull = None
     for it in range(1, maxiter + 1):
-        v = vion + grid.hartree(dens) + grid.lda_xc(dens)[1]
+        if cfull is None:
+            v = (vion + grid.hartree(dens) + grid.lda_xc(dens)[1]).reshape(-1).contiguous()
+            preset = False
+        elif step is not None:
+            dflat, v = step.run(cfull)
+            dens = dflat.reshape(cr.shape)
+            preset = True
+        else:
+            cvec = torch.from_numpy(cfull).to(grid.device)
+            dens = torch.clamp((cvec @ Nh + mix * (cvec @ Rh)).reshape(cr.shape), min=1e-12)
+            dens = dens * (N / (dens.sum() * dV))
+            v = (vion + grid.hartree(dens) + grid.lda_xc(dens)[1]).reshape(-1).contiguous()
+            preset = False
         if adaptive:
             # mu to a count accuracy matched to the current density residual: a count
             # error dN moves the residual by about |dN| / N, so |dN| <= 1e-2 err N
@@ -118,7 +219,10 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
             # the strict tolerance below
             solver.count_tol = 1e-4 if err is None else min(1e-4, max(1e-2 * err, 1e-11))
         ts = time.perf_counter()
-        nout, eband, mu = solver.density_device(v.reshape(-1).contiguous())
+        if preset:
+            nout, eband, mu = solver.density_device(v, potential_set=True)
+        else:
+            nout, eband, mu = solver.density_device(v)
         nout = nout.reshape(cr.shape)
         t_solver += time.perf_counter() - ts
         resid = nout - dens
@@ -143,21 +247,16 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
         for sq in slots:
             gram[slot, sq] = gram[sq, slot] = stats[1 + sq]
         k = len(slots)
+        cfull = np.zeros(ndiis)
         if k > 1:
             Bm = np.zeros((k + 1, k + 1))
             Bm[:k, :k] = gram[np.ix_(slots, slots)]
             Bm[k, :k] = Bm[:k, k] = 1.0
             rhs = np.zeros(k + 1)
             rhs[k] = 1.0
-            coef = np.linalg.lstsq(Bm, rhs, rcond=None)[0][:k]
-            cfull = np.zeros(ndiis)
-            cfull[slots] = coef
-            cvec = torch.from_numpy(cfull).to(grid.device)
-            dens = (cvec @ Nh + mix * (cvec @ Rh)).reshape(cr.shape)
+            cfull[slots] = np.linalg.lstsq(Bm, rhs, rcond=None)[0][:k]
         else:
-            dens = dens + mix * resid
-        dens = torch.clamp(dens, min=1e-12)
-        dens = dens * (N / (dens.sum() * dV))
+            cfull[slot] = 1.0      # plain mixing: dens + mix * resid
     if adaptive:
         # fp64 finish: mu to the strict count tolerance, density and band energy of
         # the converged input potential in fp64
@@ -168,6 +267,7 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
         t_solver += time.perf_counter() - ts
     if handed_over:
         solver.set_precision("mixed")
+    dens = dens.clone()
     vh = grid.hartree(dens)
     exc, vxc = grid.lda_xc(dens)
     E_H = 0.5 * float(torch.sum(dens * vh)) * dV
@@ -175,4 +275,5 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
     E = eband - float(torch.sum((vh + vxc) * dens)) * dV + E_H + E_xc
     torch.cuda.synchronize()
     return dict(energy=E, density=nout, iterations=it, converged=converged, dens_in=dens, eband=eband, mu=mu,
-                time=time.perf_counter() - t0, t_solver=t_solver, E_H=E_H, E_xc=E_xc, residuals=errs)
+                time=time.perf_counter() - t0, t_solver=t_solver, E_H=E_H, E_xc=E_xc, residuals=errs,
+                graph_replays=step.replays - replays0 if step is not None else 0)
