@@ -66,6 +66,16 @@ hipError_t launch_inertia_big(int bmax, int nmax, const lres::WinDesc* wins, con
 #define LRES_MFMA_PARTS(X) X(float, 2) X(float, 3) X(float, 4) X(float, 5) X(float, 6) X(float, 7) X(float, 8) \
     X(double, 2) X(double, 3) X(double, 4) X(double, 5) X(double, 6) X(double, 7) X(double, 8)
 LRES_MFMA_PARTS(LRES_MFMA_PART_DECL)
+#define LRES_MFMA_INERTIA_DECL(nt)                                                                              \
+    hipError_t lres_mfma_inertia_##nt(const lres::WinDesc* wins, const int32_t* ids, int nids, const double* SW,   \
+                                      const double* HW, const double* sigma, int nshift, int32_t* counts,       \
+                                      unsigned long long* total, hipStream_t st);
+#define LRES_MFMA_INERTIA_PARTS(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8)
+LRES_MFMA_INERTIA_PARTS(LRES_MFMA_INERTIA_DECL)
+// inertia counts on the matrix cores for windows of 16 nt functions at most (nt in 2..8)
+hipError_t launch_inertia_mfma(int nt, const lres::WinDesc* wins, const int32_t* ids, int nids, const double* SW,
+                               const double* HW, const double* sigma, int nshift, int32_t* counts,
+                               unsigned long long* total, hipStream_t st);
 hipError_t launch_pole_mfma(int fp32, int nt, int nwc, const lres::WinDesc* wins, const int32_t* ids, int nids,
                             const double* SW, const double* HW, int M, int ngroup, double c, double Rr,
                             double* Dpart, double* Fpart, hipStream_t st);

@@ -654,6 +654,16 @@ hipError_t launch_pole_mfma(int fp32, int nt, int nwc, const WinDesc* wins, cons
     return hipErrorInvalidValue;
 }
 
+hipError_t launch_inertia_mfma(int nt, const WinDesc* wins, const int32_t* ids, int nids, const double* SW,
+                               const double* HW, const double* sigma, int nshift, int32_t* counts,
+                               unsigned long long* total, hipStream_t st) {
+#define LRES_INERTIA_CALL(NT) \
+    if (nt == NT) return lres_mfma_inertia_##NT(wins, ids, nids, SW, HW, sigma, nshift, counts, total, st);
+    LRES_MFMA_INERTIA_PARTS(LRES_INERTIA_CALL)
+#undef LRES_INERTIA_CALL
+    return hipErrorInvalidValue;
+}
+
 int mfma_scratch_bytes(int fp32, int nt, int nwc) {
     if (!mfma_nwc_ok(nt, nwc)) return -1;
 #define LRES_MFMA_CALL(r, NT)                                                                                  \

@@ -371,6 +371,159 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
     }
 }
 
+// One real Gauss-Jordan pivot S of the 4 x 4 block held as lane 4i + j -> entry
+// (i, j) (the shifted form of pivot4); returns the pivot value, which is the
+// S-th pivot of the block's unpivoted LDL^T.
+template <int S>
+__device__ __forceinline__ double rpivot4(double& a, int i, int j) {
+    constexpr int kRowS = 0x1C | (S << 5);
+    constexpr int kColS = 0x13 | ((4 * S) << 5);
+    constexpr int kPiv = 0x10 | ((5 * S) << 5);
+    double ci = swz<kRowS>(a);
+    double rj = swz<kColS>(a);
+    const double piv = swz<kPiv>(a);
+    if (i == S) ci -= 1.0;
+    if (j == S) rj += 1.0;
+    a = fma(-ci, rj / piv, a);
+    return piv;
+}
+
+// Inertia of sigma S_W - H_W (number of positive pivots of its unpivoted LDL^T,
+// the same count as the vector-ALU k_inertia) on the matrix cores: block Gaussian
+// elimination in blocks of four pivots, one wave per (window, shift).  Only the
+// trailing Schur complement matters and it stays symmetric, so the wave holds the
+// tiles on and above the diagonal, updates at block p only those at or below the
+// pivot tile row (one v_mfma_f64_16x16x4f64 each: A -= A[:,P] (A_PP^-1 A[P,:])),
+// and takes the column panel from the row panel.  The four pivots of a block come
+// out of the 4 x 4 inverse of its diagonal block (rpivot4, in order); pivots on the
+// identity padding are not counted.
+template <int NT>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NT <= 6 ? 2 : 1)))
+k_inertia_mfma(const WinDesc* __restrict__ wins, const int32_t* __restrict__ ids, const double* __restrict__ SW,
+               const double* __restrict__ HW, const double* __restrict__ sigma, int nshift,
+               int32_t* __restrict__ counts, unsigned long long* __restrict__ total) {
+    using MF = Mfma<double>;
+    using V = typename MF::V;
+    constexpr int NR = 16 * NT;
+    // row panel of the current block: rp[b][j][s] = A[4p + s][j]
+    __shared__ __attribute__((aligned(16))) double rp[2][NR][4];
+    const int w = ids[blockIdx.x];
+    const int s_id = blockIdx.y;
+    const WinDesc d = wins[w];
+    const int n = d.n;
+    const int lane = threadIdx.x & 63;
+    const double sg = sigma[s_id];
+    const double* S = SW + d.mat_off;
+    const double* H = HW + d.mat_off;
+    const int nst = (n + 3) >> 2;
+
+    V a[NT][NT];   // tile (I, J) for I <= J only
+    {
+        int nn = n, ln0 = lane;
+        asm volatile("" : "+s"(nn), "+v"(ln0));
+        const int li = ln0 & 15, lg = ln0 >> 4;
+#pragma unroll
+        for (int I = 0; I < NT; ++I) {
+#pragma unroll
+            for (int J = 0; J < NT; ++J) {
+                if (J < I) continue;
+                const int col = 16 * J + li;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = 16 * I + MF::row(lg, r);
+                    const bool in = row < nn && col < nn;
+                    const uint32_t off = in ? (uint32_t)(row * nn + col) : 0u;
+                    const double sv = S[off], hv = H[off];
+                    a[I][J][r] = in ? sg * sv - hv : (row == col ? 1.0 : 0.0);
+                }
+            }
+#pragma unroll
+            for (int J = 0; J < NT; ++J)
+                if (J >= I) asm volatile("" : "+v"(a[I][J])::"memory");
+        }
+    }
+    // tile row JPc (columns of tiles J >= JPc) of block 4 JPc + pq into buffer b;
+    // register pq of lane group s holds row 4 pq + s
+    auto publish = [&](auto jpc, int pq, int b, int li, int lg) __attribute__((always_inline)) {
+        constexpr int JPc = decltype(jpc)::value;
+#pragma unroll
+        for (int J = 0; J < NT; ++J) {
+            if (J < JPc) continue;
+            const V& x = a[JPc][J];
+            rp[b][16 * J + li][lg] = pq == 0 ? x[0] : pq == 1 ? x[1] : pq == 2 ? x[2] : x[3];
+        }
+        asm volatile("; inertia publish tile %0" ::"n"(JPc));
+    };
+    {
+        int ln = lane;
+        asm volatile("" : "+v"(ln));
+        publish(std::integral_constant<int, 0>{}, 0, 0, ln & 15, ln >> 4);
+    }
+    int npos = 0;
+    static_for<0, NT>([&](auto jpc) __attribute__((always_inline)) {
+        constexpr int JP = decltype(jpc)::value;
+        const int ppend = min(4, nst - 4 * JP);
+        for (int pp = 0; pp < ppend; ++pp) {
+            const int p = 4 * JP + pp, b = p & 1;
+            __syncthreads();
+            int ln = lane;
+            asm volatile("" : "+v"(ln));
+            const int li = ln & 15, lg = ln >> 4;
+            const int pi = li >> 2, pj = li & 3;
+            double x = rp[b][4 * p + pj][pi];
+            const double v0 = rpivot4<0>(x, pi, pj);
+            const double v1 = rpivot4<1>(x, pi, pj);
+            const double v2 = rpivot4<2>(x, pi, pj);
+            const double v3 = rpivot4<3>(x, pi, pj);
+            npos += (v0 > 0.0 && 4 * p < n ? 1 : 0) + (v1 > 0.0 && 4 * p + 1 < n ? 1 : 0) +
+                    (v2 > 0.0 && 4 * p + 2 < n ? 1 : 0) + (v3 > 0.0 && 4 * p + 3 < n ? 1 : 0);
+            double prow[4];   // row lg of A_PP^-1
+#pragma unroll
+            for (int s2 = 0; s2 < 4; ++s2) prow[s2] = __shfl(x, 4 * lg + s2, 64);
+            // A operands: C[16 I + li][lg] = A[4p + lg][16 I + li] on the trailing rows, 0 on
+            // the rows already eliminated and the pivot rows
+            double car[NT];
+#pragma unroll
+            for (int I = 0; I < NT; ++I) {
+                car[I] = 0.0;
+                if (I < JP) continue;
+                const int row = 16 * I + li;
+                const double v = rp[b][row][lg];
+                car[I] = row >= 4 * p + 4 ? v : 0.0;
+            }
+            // B operands: -(A_PP^-1 A[P,:])[lg][16 J + li]
+            double nb[NT];
+#pragma unroll
+            for (int J = 0; J < NT; ++J) {
+                nb[J] = 0.0;
+                if (J < JP) continue;
+                const int col = 16 * J + li;
+                double t = 0.0;
+#pragma unroll
+                for (int s2 = 0; s2 < 4; ++s2) t = fma(prow[s2], rp[b][col][s2], t);
+                nb[J] = -t;
+            }
+#pragma unroll
+            for (int I = 0; I < NT; ++I)
+#pragma unroll
+                for (int J = 0; J < NT; ++J) {
+                    if (I < JP || J < I) continue;
+                    a[I][J] = MF::mma(car[I], nb[J], a[I][J]);
+                }
+            if (p + 1 < nst) {
+                if (pp < 3)
+                    publish(jpc, pp + 1, b ^ 1, li, lg);
+                else
+                    publish(std::integral_constant<int, (JP + 1 < NT ? JP + 1 : JP)>{}, 0, b ^ 1, li, lg);
+            }
+        }
+    });
+    if (lane == 0) {
+        if (counts) counts[(int64_t)w * nshift + s_id] = npos;
+        atomicAdd(&total[s_id], (unsigned long long)npos);
+    }
+}
+
 // Accumulator registers per lane of a variant: 8 (fp32) or 16 (fp64) per held tile.
 template <typename R, int NT, int NWC>
 constexpr int mfma_acc() {
@@ -434,4 +587,19 @@ hipError_t LRES_MFMA_NAME(LRES_MFMA_R, LRES_MFMA_NT)(int nwc, const lres::WinDes
 #undef LRES_MFMA_CASE
     return nwc < 0 ? (hipError_t)-1 : hipErrorInvalidValue;
 }
+
+// the inertia kernel: fp64 parts only
+#define LRES_MFMA_CAT_(a, b) a##b
+#define LRES_MFMA_CAT(a, b) LRES_MFMA_CAT_(a, b)
+#define LRES_MFMA_IS_DOUBLE_double 1
+#if LRES_MFMA_CAT(LRES_MFMA_IS_DOUBLE_, LRES_MFMA_R)
+hipError_t LRES_MFMA_CAT(lres_mfma_inertia_, LRES_MFMA_NT)(const lres::WinDesc* wins, const int32_t* ids, int nids,
+                                                          const double* SW, const double* HW, const double* sigma,
+                                                          int nshift, int32_t* counts, unsigned long long* total,
+                                                          hipStream_t st) {
+    hipLaunchKernelGGL((lres::k_inertia_mfma<LRES_MFMA_NT>), dim3(nids, nshift), dim3(64), 0, st, wins, ids, SW, HW,
+                       sigma, nshift, counts, total);
+    return hipGetLastError();
+}
+#endif
 #endif

@@ -325,6 +325,16 @@ void run_pass(lres_plan* P, double mu, double e_lo, double* d_dens, double* d_NB
 // Sylvester inertia counts #{eigenvalues of (H_W, S_W) below sigma} summed over
 // windows, at ns shifts; per-window counts into d_counts[w * ns + s] if given.
 // cand (optional): only these windows (host list of window indices).
+// Inertia counts of windows of 17..128 functions on the matrix cores (block
+// elimination, a quarter of the vector kernel's barriers); LRES_INERTIA_KERNEL=valu
+// selects the vector-ALU kernel.  Both count the positive pivots of the same
+// unpivoted LDL^T.
+bool inertia_on_mfma(int nmax) {
+    const char* e = getenv("LRES_INERTIA_KERNEL");
+    if (e && std::string(e) == "valu") return false;
+    return nmax > 16 && nmax <= kMaxRegisterN;
+}
+
 void inertia(lres_plan* P, const double* sig, int ns, int64_t* h_total, int32_t* d_counts,
              const std::vector<int32_t>* cand = nullptr) {
     require(P->have_H, "no Hamiltonian loaded");
@@ -353,6 +363,10 @@ void inertia(lres_plan* P, const double* sig, int ns, int64_t* h_total, int32_t*
             hip_check(launch_inertia_big(b.bmax, b.nmax, P->d_wins.p, ids, nids, P->d_SW.p, P->d_HW.p, P->d_sigma.p,
                                          ns, P->d_work.p, P->d_ws_off.p, P->ngroup, d_counts, P->d_itot.p, P->st),
                       "blocked inertia kernel");
+        else if (inertia_on_mfma(b.nmax))
+            hip_check(launch_inertia_mfma((b.nmax + 15) / 16, P->d_wins.p, ids, nids, P->d_SW.p, P->d_HW.p,
+                                          P->d_sigma.p, ns, d_counts, P->d_itot.p, P->st),
+                      "matrix-core inertia kernel");
         else
             hip_check(launch_inertia(b.ishape.T, b.ishape.G, P->d_wins.p, ids, nids, P->d_SW.p, P->d_HW.p,
                                      P->d_sigma.p, ns, d_counts, P->d_itot.p, P->st),

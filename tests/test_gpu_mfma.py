@@ -11,8 +11,12 @@ Bars:
   oracle at 1e-12 in fp64 (at most one bucket may fall back to the vector-ALU
   kernel when every MFMA variant of its size spills) and at the fp32 bar in
   complex64 (all buckets on the matrix cores).
-fp64 passes run on the vector ALUs by default; these tests force the matrix
-cores with LRES_POLE_KERNEL=mfma.
+Windows of up to 128 functions run on the matrix cores in both precisions by
+default; the tests still force the family they check (LRES_POLE_KERNEL=mfma /
+=valu).  Inertia counts: the matrix-core block elimination equals the
+vector-ALU kernel and the eigenvalue counts of the oracle's dense solves
+exactly, and lambda_min from both agrees with the dense lowest eigenvalue
+to 1e-10.
 """
 import os
 
@@ -130,3 +134,31 @@ def test_mfma_uneven_windows_match_oracle():
     d32, _, _, n32, _, bk32 = _pass(b, centres, H, mu, e_lo, Rw=6.0, precision="mixed")
     assert all(row[7] == 2 for row in bk32), bk32
     assert np.max(np.abs(d32 - dp)) < 2e-5 and abs(n32 - NBo.sum()) < 1e-5 * cr.n_elec
+
+
+def test_mfma_inertia_counts_match_valu_and_dense(sys3):
+    cr, b, v = sys3
+    print("[gpu] inertia counts: matrix cores vs vector ALU vs dense, n=3 R_w=8", flush=True)
+    from lresdft_amd.lcao3d import LRFTLCAOCpp, csr64
+    cells = cell_centres((3, 3, 3), 6.0)
+    H = b.H(v)
+    W, _ = build_windows(b, cells, 8.0)
+    C = LRFTLCAOCpp(b, cells, R_w=8.0, M=32)
+    C.plan.set_h(csr64(H))
+    assert all(row[0] == 84 for row in C.plan.buckets())
+    lam = [linalg.eigh(H[w][:, w].toarray(), b.S[w][:, w].toarray(), eigvals_only=True) for w in W]
+    lo = min(l[0] for l in lam)
+    sig = np.linspace(lo - 0.2, lo + 1.5, 24)
+    # positive pivots of sigma S_W - H_W = eigenvalues of (H_W, S_W) below sigma (Sylvester)
+    ref = np.array([sum(int(np.sum(l < s)) for l in lam) for s in sig])
+    got_m = C.plan.inertia(sig)
+    lm_m = C.plan.lambda_min()
+    os.environ["LRES_INERTIA_KERNEL"] = "valu"
+    try:
+        got_v = C.plan.inertia(sig)
+        lm_v = C.plan.lambda_min()
+    finally:
+        os.environ.pop("LRES_INERTIA_KERNEL", None)
+    assert np.array_equal(got_m, ref), (got_m, ref)
+    assert np.array_equal(got_v, ref), (got_v, ref)
+    assert abs(lm_m - lo) < 1e-10 and abs(lm_v - lo) < 1e-10
