@@ -14,7 +14,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 SOURCES = ["csrc/lres_kernels.hip", "csrc/lres_plan.hip"]
 MFMA_SOURCE = "csrc/lres_mfma.hip"
-MFMA_PARTS = [(r, nt) for r in ("float", "double") for nt in range(2, 9)]
+MFMA_PARTS = [("float", nt) for nt in range(2, 17)] + [("double", nt) for nt in range(2, 9)]
 LIB = os.path.join(HERE, "liblres_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -64,6 +64,7 @@ hipError_t launch_inertia_big(int bmax, int nmax, const lres::WinDesc* wins, con
                                          const double* SW, const double* HW, int M, int ngroup, double c,     \
                                          double Rr, double* Dpart, double* Fpart, hipStream_t st);
 #define LRES_MFMA_PARTS(X) X(float, 2) X(float, 3) X(float, 4) X(float, 5) X(float, 6) X(float, 7) X(float, 8) \
+    X(float, 9) X(float, 10) X(float, 11) X(float, 12) X(float, 13) X(float, 14) X(float, 15) X(float, 16)     \
     X(double, 2) X(double, 3) X(double, 4) X(double, 5) X(double, 6) X(double, 7) X(double, 8)
 LRES_MFMA_PARTS(LRES_MFMA_PART_DECL)
 #define LRES_MFMA_INERTIA_DECL(nt)                                                                              \

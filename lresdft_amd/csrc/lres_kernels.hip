@@ -639,7 +639,8 @@ int mfma_acc_regs(int fp32, int nt, int nwc) {
 }
 
 static bool mfma_nwc_ok(int nt, int nwc) {
-    return nt >= 2 && nt <= 8 && nwc >= 1 && nwc <= nt;
+    if (nt > 8) return nt <= 16 && nwc == 8;
+    return nt >= 2 && nwc >= 1 && nwc <= nt;
 }
 
 hipError_t launch_pole_mfma(int fp32, int nt, int nwc, const WinDesc* wins, const int32_t* ids, int nids,

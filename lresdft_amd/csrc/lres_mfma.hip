@@ -1,4 +1,5 @@
-// Pole solves on the matrix cores (MFMA) for windows of up to 128 functions.
+// Pole solves on the matrix cores (MFMA) for windows of up to 128 functions (both
+// precisions) and, in fp32, up to 256.
 //
 // One workgroup of NWC waves handles one (window, pole group).  Per pole it
 // holds A = z S_W - H_W, padded to 16 NT x 16 NT (zero off-diagonal, 1 on the
@@ -567,7 +568,13 @@ int scratch_one() {
 #if defined(LRES_MFMA_R) && defined(LRES_MFMA_NT)
 #define LRES_MFMA_NAME3(a, b, c) a##b##_##c
 #define LRES_MFMA_NAME(r, nt) LRES_MFMA_NAME3(lres_mfma_part_, r, nt)
+// up to 8 tile columns: every split of 1..8 waves; beyond (fp32 windows of 129..256
+// functions, whose tiles fill the register file of eight waves): 8 waves
+#if LRES_MFMA_NT <= 8
 #define LRES_MFMA_FOR_NWC(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8)
+#else
+#define LRES_MFMA_FOR_NWC(X) X(8)
+#endif
 // nwc > 0: launch with nwc waves per (window, pole group); nwc < 0: return the
 // scratch bytes per lane of the -nwc variant (-1 if it does not exist)
 hipError_t LRES_MFMA_NAME(LRES_MFMA_R, LRES_MFMA_NT)(int nwc, const lres::WinDesc* wins, const int32_t* ids, int nids,

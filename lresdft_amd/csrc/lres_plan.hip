@@ -83,7 +83,7 @@ bool use_mfma(int /*fp32*/) {
 std::vector<Shape> mfma_shapes(int nmax, int fp32) {
     std::vector<Shape> out;
     const int nt = (nmax + 15) / 16;
-    if (nt < 2 || nt > 8) return out;
+    if (nt < 2 || nt > (fp32 ? 16 : 8)) return out;
     // LRES_MFMA_WAVES=w restricts the candidates to w waves per (window, pole
     // group), spills allowed (used to check that every variant gives the same bits)
     const char* e = getenv("LRES_MFMA_WAVES");
@@ -94,7 +94,7 @@ std::vector<Shape> mfma_shapes(int nmax, int fp32) {
     // more, the one that spills least, provided that is at most 256 bytes per lane
     int least = -1, least_bytes = 1 << 30;
     for (int nwc = 1; nwc <= 8; ++nwc) {
-        if (nwc > nt || (force > 0 && nwc != force)) continue;
+        if (nwc > nt || (nt > 8 && nwc != 8) || (force > 0 && nwc != force)) continue;
         const int scratch = mfma_scratch_bytes(fp32, nt, nwc);
         if (scratch < 0) continue;
         if (scratch <= 128 || force > 0) out.push_back({nt, nwc, kMfma});
@@ -238,6 +238,13 @@ void gather_h(lres_plan* P, const int64_t* Hp, const int64_t* Hi, const double* 
     P->have_H = true;
 }
 
+// Windows above the register-resident limit of the vector kernels: in fp32 the
+// matrix-core kernel holds up to 256 functions (16 x 16 tiles over eight waves);
+// otherwise (fp64, or larger) the blocked sweep in a workspace.
+bool big_on_mfma(int nmax, int fp32) {
+    return fp32 && nmax <= 256 && use_mfma(1) && !mfma_shapes(nmax, 1).empty();
+}
+
 // Autotuner: time every valid shape of a bucket on the loaded H (two launches
 // after one warm-up, HIP events) and keep the fastest.  All shapes apply the
 // same operations to every matrix entry, so the choice never changes results.
@@ -286,7 +293,7 @@ void run_pass(lres_plan* P, double mu, double e_lo, double* d_dens, double* d_NB
     hip_check(hipEventRecord(P->ev[2], P->st), "event");
     const int fp32 = P->precision == LRES_MIXED ? 1 : 0;
     for (auto& b : P->buckets) {
-        if (b.big) {
+        if (b.big && !big_on_mfma(b.nmax, fp32)) {
             hip_check(launch_pole_big(fp32, b.bmax, b.nmax, P->d_wins.p, P->d_bucket_ids.p + b.id_off,
                                       (int)b.ids.size(), P->d_SW.p, P->d_HW.p, P->M, P->ngroup, c, R, P->d_work.p,
                                       P->d_ws_off.p, P->d_Dpart.p, P->d_Fpart.p, P->st),
