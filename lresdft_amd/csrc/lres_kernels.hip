@@ -1006,6 +1006,85 @@ __device__ int big_block_step(E* __restrict__ A, int n, int p0, int b, E* Cl, E*
     return npos;
 }
 
+// big_block_step on the upper triangle only, for complex-symmetric A (the pole
+// solves).  After the pivots of a set D the in-place state is symmetric on D x D
+// and on the rest U x U and antisymmetric between them (W[u][d] = -W[d][u]), so
+// entries (i, j) with i <= j carry everything: the pivot rows are read from the
+// upper triangle (A[p][j] = -A[j][p] for pivoted j < p0, the mirror inside the
+// pivot block), the pivot columns are the pivot rows with the sign flipped on the
+// pivoted rows, and the update touches i <= j only (half the workspace traffic).
+// On return the upper triangle holds that of the Gauss-Jordan state.
+template <typename E>
+__device__ void big_block_step_sym(E* __restrict__ A, int n, int p0, int b, E* Cl, E* Rl, E* Pl) {
+    using Fo = Fld<E>;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int idx = tid; idx < b * n; idx += nt) {
+        const int q = idx / n, j = idx - (idx / n) * n;
+        const int pr = p0 + q;
+        E v;
+        if (j >= pr) {
+            v = A[(int64_t)pr * n + j];
+        } else {
+            v = A[(int64_t)j * n + pr];
+            if (j < p0) v = Fo::mul(v, Fo::add_one(Fo::zero(), -1));
+        }
+        Rl[idx] = v;
+        Cl[(int64_t)j * b + q] = j < p0 ? Fo::mul(v, Fo::add_one(Fo::zero(), -1)) : v;
+    }
+    __syncthreads();
+    if (tid < 64) {   // wave 0: Pi = A_PP^-1 by the scalar shifted-pivot sweep
+        for (int e = tid; e < b * b; e += 64) Pl[e] = Rl[(e / b) * n + p0 + (e - (e / b) * b)];
+        __builtin_amdgcn_wave_barrier();
+        for (int p = 0; p < b; ++p) {
+            const E piv = Pl[p * b + p];
+            const E inv = Fo::recip(piv);
+            E colv[4], rowv[4];
+            int cnt = 0;
+            for (int e = tid; e < b * b; e += 64, ++cnt) {
+                const int i = e / b, j = e - (e / b) * b;
+                colv[cnt] = (i == p) ? Fo::add_one(piv, -1) : Pl[i * b + p];
+                rowv[cnt] = Fo::mul((j == p) ? Fo::add_one(piv, 1) : Pl[p * b + j], inv);
+            }
+            __builtin_amdgcn_wave_barrier();
+            cnt = 0;
+            for (int e = tid; e < b * b; e += 64, ++cnt) Pl[e] = Fo::fms(Pl[e], colv[cnt], rowv[cnt]);
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    __syncthreads();
+    // R~ = Pi (R + [I on P]) in place, one column per thread; C~ = C - [I on P]
+    for (int j = tid; j < n; j += nt) {
+        E v[32];
+        for (int q = 0; q < b; ++q) v[q] = (j - p0 == q) ? Fo::add_one(Rl[q * n + j], 1) : Rl[q * n + j];
+        for (int q = 0; q < b; ++q) {
+            E acc = Fo::zero();
+            for (int r = 0; r < b; ++r) acc = Fo::fma_(acc, Pl[q * b + r], v[r]);
+            Rl[q * n + j] = acc;
+        }
+    }
+    for (int q = tid; q < b; q += nt) Cl[(p0 + q) * b + q] = Fo::add_one(Cl[(p0 + q) * b + q], -1);
+    __syncthreads();
+    // A -= C~ R~ on i <= j: each thread four rows of one column per round
+    const int rows4 = (n + 3) / 4;
+    for (int idx = tid; idx < rows4 * n; idx += nt) {
+        const int i0 = (idx / n) * 4, j = idx - (idx / n) * n;
+        if (i0 > j) continue;
+        E a[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) a[s] = (i0 + s <= j && i0 + s < n) ? A[(int64_t)(i0 + s) * n + j] : Fo::zero();
+        for (int q = 0; q < b; ++q) {
+            const E r = Rl[q * n + j];
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+                if (i0 + s <= j && i0 + s < n) a[s] = Fo::fms(a[s], Cl[(i0 + s) * b + q], r);
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+            if (i0 + s <= j && i0 + s < n) A[(int64_t)(i0 + s) * n + j] = a[s];
+    }
+    __syncthreads();
+}
+
 constexpr int kBigThreads = 1024;
 
 template <typename R>
@@ -1032,10 +1111,12 @@ __global__ void __launch_bounds__(kBigThreads) k_pole_big(const WinDesc* __restr
     for (int jp = g; jp < M / 2; jp += ngroup) {
         cx<double> z, wt, wz;
         pole(jp, M, c, Rr, z, wt, wz);
-        for (int64_t q = threadIdx.x; q < (int64_t)n * n; q += blockDim.x)
+        for (int64_t q = threadIdx.x; q < (int64_t)n * n; q += blockDim.x) {
+            if (q / n > q % n) continue;   // upper triangle only
             A[q] = {(R)(z.re * S[q] - H[q]), (R)(z.im * S[q])};
+        }
         __syncthreads();
-        for (int p0 = 0; p0 < n; p0 += bmax) big_block_step<E>(A, n, p0, min(bmax, n - p0), Cl, Rl, Pl);
+        for (int p0 = 0; p0 < n; p0 += bmax) big_block_step_sym<E>(A, n, p0, min(bmax, n - p0), Cl, Rl, Pl);
         for (int q = threadIdx.x; q < k * k; q += blockDim.x) {
             const int r = q / k, col = q - (q / k) * k;
             if (r > col) continue;   // upper triangle (the partial sums are symmetric)
