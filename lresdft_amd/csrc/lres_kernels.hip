@@ -1018,18 +1018,25 @@ template <typename E>
 __device__ void big_block_step_sym(E* __restrict__ A, int n, int p0, int b, E* Cl, E* Rl, E* Pl) {
     using Fo = Fld<E>;
     const int tid = threadIdx.x, nt = blockDim.x;
+    // pivot rows: right of the diagonal from the rows themselves (coalesced along j),
+    // left of it from the columns, read as rows j of the b pivot columns (contiguous)
     for (int idx = tid; idx < b * n; idx += nt) {
         const int q = idx / n, j = idx - (idx / n) * n;
-        const int pr = p0 + q;
-        E v;
-        if (j >= pr) {
-            v = A[(int64_t)pr * n + j];
-        } else {
-            v = A[(int64_t)j * n + pr];
-            if (j < p0) v = Fo::mul(v, Fo::add_one(Fo::zero(), -1));
+        if (j >= p0 + q) Rl[idx] = A[(int64_t)(p0 + q) * n + j];
+    }
+    for (int idx = tid; idx < (p0 + b) * b; idx += nt) {
+        const int j = idx / b, q = idx - (idx / b) * b;
+        if (j < p0 + q) {
+            const E v = A[(int64_t)j * n + p0 + q];
+            Rl[q * n + j] = j < p0 ? Fo::mul(v, Fo::add_one(Fo::zero(), -1)) : v;
         }
-        Rl[idx] = v;
-        Cl[(int64_t)j * b + q] = j < p0 ? Fo::mul(v, Fo::add_one(Fo::zero(), -1)) : v;
+    }
+    __syncthreads();
+    // pivot columns: the pivot rows with the sign flipped on the pivoted rows
+    for (int idx = tid; idx < n * b; idx += nt) {
+        const int i = idx / b, q = idx - (idx / b) * b;
+        const E v = Rl[q * n + i];
+        Cl[idx] = i < p0 ? Fo::mul(v, Fo::add_one(Fo::zero(), -1)) : v;
     }
     __syncthreads();
     if (tid < 64) {   // wave 0: Pi = A_PP^-1 by the scalar shifted-pivot sweep
@@ -1054,7 +1061,7 @@ __device__ void big_block_step_sym(E* __restrict__ A, int n, int p0, int b, E* C
     __syncthreads();
     // R~ = Pi (R + [I on P]) in place, one column per thread; C~ = C - [I on P]
     for (int j = tid; j < n; j += nt) {
-        E v[32];
+        E v[16];
         for (int q = 0; q < b; ++q) v[q] = (j - p0 == q) ? Fo::add_one(Rl[q * n + j], 1) : Rl[q * n + j];
         for (int q = 0; q < b; ++q) {
             E acc = Fo::zero();
