@@ -317,8 +317,9 @@ class LRFTLCAOCpp:
     def set_precision(self, precision):
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
+        if precision != self.precision:
+            self.plan.set_precision(precision)
         self.precision = precision
-        self.plan.set_precision(precision)
 
     # -- mu search -----------------------------------------------------------
     # The root of count(mu) = 2 sum_B N_B(mu) - N is accepted at |count| <= ftol:
@@ -409,7 +410,11 @@ class LRFTLCAOCpp:
         self.count_tol = None
         try:
             mu, e_lo = self.mu, self.e_lo
-            nsum, _ = self.plan.run(mu, e_lo, want_density=False)
+            last = self.plan.last
+            if last is not None and last[0] == mu and last[1] == e_lo:
+                nsum = last[2]      # an fp64 pass at this (mu, e_lo) on this H has run already
+            else:
+                nsum, _ = self.plan.run(mu, e_lo, want_density=False)
             cnt = 2.0 * nsum - N
             if abs(cnt) > self._ftol(N):
                 mu = self._solve_mu_warm(mu, cnt, e_lo, N)
