@@ -144,7 +144,7 @@ class PotentialStep:
         return self.dens, self.v
 
 
-def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion=None, dens0=None, fp32_floor=1e-4,
+def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion=None, dens0=None, fp32_floor=3e-5,
              graph=True):
     """Device SCF; ``solver.density_device(v)`` returns (density tensor [ng], e_band, mu).
 
@@ -155,9 +155,9 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
     complex64 passes until the residual falls below ``fp32_floor`` (then fp64
     passes): complex64 densities differ from fp64 by ~2e-6 in this residual's
     norm, and DIIS steps built from residuals within a few times that noise
-    converge slowly (measured at 1024 electrons: 15 iterations with the
-    hand-over at 1e-5, 13 at 1e-4, 12 in pure fp64), so the default floor is
-    1e-4.  The energy and the returned density are always fp64 results at the
+    converge slowly.  Measured at 1024 electrons (13 iterations in each case,
+    14 at 1e-5): 45.7 ms with the hand-over at 1e-4, 41.2 ms at 3e-5, so the
+    default floor is 3e-5.  The energy and the returned density are always fp64 results at the
     converged input density.  One device-to-host transfer per iteration (the
     residual norm and the new row of the DIIS Gram matrix together).  The
     mixing, potential and H(v) of every iteration after the first form one
@@ -227,7 +227,8 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
         t_solver += time.perf_counter() - ts
         resid = nout - dens
         # the new history slot (the oldest one once the history is full)
-        slot = len(slots) if len(slots) < ndiis else slots[0]
+        free = [q for q in range(ndiis) if q not in slots]
+        slot = free[0] if free else slots[0]
         Rh[slot] = resid.reshape(-1)
         Nh[slot] = dens.reshape(-1)
         live = slots[1:] + [slot] if len(slots) == ndiis else slots + [slot]
