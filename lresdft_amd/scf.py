@@ -145,13 +145,15 @@ class PotentialStep:
 
 
 def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion=None, dens0=None, fp32_floor=3e-5,
-             graph=True):
+             graph=True, count_rel=1e-2):
     """Device SCF; ``solver.density_device(v)`` returns (density tensor [ng], e_band, mu).
 
     A solver with ``count_tol``/``refine`` (``LRFTLCAOCpp``) solves mu each step
-    to a count accuracy 1e-2 x the current density residual (clamped to
+    to a count accuracy ``count_rel`` x the current density residual (clamped to
     [1e-11, 1e-4] N), and the converged step is finished in fp64 to the strict
-    tolerance (``solver.refine()``).  With mixed precision the iterations run on
+    tolerance (``solver.refine()``).  The default 1e-2 is the measured best at
+    432 and 1024 electrons: 3e-2 and 1e-1 add one or two iterations (the count
+    error enters the residual) and more passes.  With mixed precision the iterations run on
     complex64 passes until the residual falls below ``fp32_floor`` (then fp64
     passes): complex64 densities differ from fp64 by ~2e-6 in this residual's
     norm, and DIIS steps built from residuals within a few times that noise
@@ -217,7 +219,7 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
             # error dN moves the residual by about |dN| / N, so |dN| <= 1e-2 err N
             # costs the iteration nothing; the converged step is polished in fp64 to
             # the strict tolerance below
-            solver.count_tol = 1e-4 if err is None else min(1e-4, max(1e-2 * err, 1e-11))
+            solver.count_tol = 1e-4 if err is None else min(1e-4, max(count_rel * err, 1e-11))
         ts = time.perf_counter()
         if preset:
             nout, eband, mu = solver.density_device(v, potential_set=True)
