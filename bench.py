@@ -201,7 +201,7 @@ def adjoint_timing(C, grid, vion, res, cells, args):
     import torch
     from lresdft_amd.forces import band_forces
     C.set_precision("fp64")
-    v = (vion + grid.hartree(res["dens_in"]) + grid.lda_xc(res["dens_in"])[1]).reshape(-1).contiguous()
+    v = grid.potential(res["dens_in"], vion)
     C.plan.set_potential(v)
     C.plan.adjoint(C.mu, C.e_lo)            # first call allocates the workspaces
     torch.cuda.synchronize()
@@ -225,7 +225,7 @@ def profile_passes(C, cr, grid, vion, K):
     256 MiB device-to-device copy whose FETCH_SIZE / WRITE_SIZE calibrate the counters."""
     import torch
     d0 = torch.full(cr.shape, cr.n_elec / (cr.ng * cr.dV), dtype=torch.float64, device="cuda")
-    v = (vion + grid.hartree(d0) + grid.lda_xc(d0)[1]).reshape(-1).contiguous()
+    v = grid.potential(d0, vion)
     C.plan.set_potential(v)
     lmin = C.plan.lambda_min()
     e_lo, mu = lmin - C.margin, lmin + 0.35
@@ -320,7 +320,7 @@ def main():
     if not args.no_forces:
         line["adjoint"] = adjoint_timing(C, grid, vion, mixed[-1], cells, args)
     if not args.no_cpu_baseline and not args.random:
-        v_host = (vion + grid.hartree(fp64[-1]["dens_in"]) + grid.lda_xc(fp64[-1]["dens_in"])[1]).cpu().numpy()
+        v_host = grid.potential(fp64[-1]["dens_in"], vion).cpu().numpy().reshape(cr.shape)
         line["cpu_baseline"] = cpu_baseline(cr, v_host, n, args.rw, args.M, fp64[-1]["iterations"])
     print(json.dumps(line), flush=True)
 

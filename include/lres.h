@@ -132,6 +132,30 @@ int lres_plan_set_stream(lres_plan* plan, void* stream);
 int lres_diis_mix(const double* d_coef, int nhist, const double* d_Nh, const double* d_Rh, int64_t ng, double mix,
                   double lo, double ntot, double dV, double* d_out, double* d_part, void* stream);
 
+/* The SCF driver's potential v = v_ion + v_H + v_xc(n) (lcao3d.py:244-245) with
+ * v_xc the Slater + Perdew-Zunger 1981 LDA potential of lda_xc (three_d.py:132-148,
+ * density clamped at 1e-14); d_vh is the Hartree potential of d_n.  All device
+ * arrays of ng doubles.  One kernel launch on `stream` (capturable). */
+int lres_lda_potential(const double* d_n, const double* d_vion, const double* d_vh, int64_t ng, double* d_v,
+                       void* stream);
+
+/* The SCF driver's residual step (lcao3d.py:250-251, 258-265): r = scale * d_nout - d_nin,
+ * stored as history row `slot` (d_Rh[slot] = r, d_Nh[slot] = d_nin; d_Nh / d_Rh are
+ * nhist x ng), and d_stats[0] = sum |r|, d_stats[1 + q] = R_q . r for q < nhist (the
+ * new row of DIIS's Gram matrix; rows not in use hold zeros).  nhist <= LRES_MAX_HIST.
+ * Fixed-order sums; d_part is a device workspace of (LRES_MAX_HIST + 1) x LRES_MIX_PARTS
+ * doubles.  Kernel launches only, on `stream`. */
+#define LRES_MAX_HIST 8
+int lres_scf_residual(const double* d_nout, double scale, const double* d_nin, int64_t ng, int nhist, int slot,
+                      double* d_Nh, double* d_Rh, double* d_part, double* d_stats, void* stream);
+
+/* The integrals of the Harris-Foulkes energy at density d_n with Hartree potential
+ * d_vh (lcao3d.py:276-279): d_out[0] = sum n v_H, d_out[1] = sum n e_xc(n),
+ * d_out[2] = sum n (v_H + v_xc(n)) (grid sums, no dV).  d_part: 3 x LRES_MIX_PARTS
+ * doubles of workspace.  Kernel launches only, on `stream`. */
+int lres_harris_terms(const double* d_n, const double* d_vh, int64_t ng, double* d_part, double* d_out,
+                      void* stream);
+
 /* One window pass at (mu, e_lo): the pole form of lrft_windows
  * (src/lrft_cpp.cpp:129-233).  Per window B it forms
  *   D_B = 2 Re sum_{j<M/2} w_j G_j[T,T],  F_B = 2 Re sum_j w_j z_j G_j[T,T],

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "liblres_hip.so")
 LRES_FP64 = 0
 LRES_MIXED = 1
 LRES_MIX_PARTS = 1024   # workspace doubles of lres_diis_mix (include/lres.h)
+LRES_MAX_HIST = 8       # history rows of lres_scf_residual
 
 _i64p = C.POINTER(C.c_int64)
 _i32p = C.POINTER(C.c_int32)
@@ -44,6 +45,9 @@ SIGNATURES = {
     "lres_plan_set_stream": (C.c_int, [_vp, _vp]),
     "lres_diis_mix": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double,
                                 _vp, _vp, _vp]),
+    "lres_lda_potential": (C.c_int, [_vp, _vp, _vp, C.c_int64, _vp, _vp]),
+    "lres_scf_residual": (C.c_int, [_vp, C.c_double, _vp, C.c_int64, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp]),
+    "lres_harris_terms": (C.c_int, [_vp, _vp, C.c_int64, _vp, _vp, _vp]),
     "lres_window_pass": (C.c_int, [_vp, C.c_double, C.c_double, _vp, _vp, _vp, _f64p]),
     "lres_density_last": (C.c_int, [_vp, _vp]),
     "lres_inertia": (C.c_int, [_vp, _f64p, C.c_int, _i64p, _vp]),

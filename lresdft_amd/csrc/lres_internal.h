@@ -33,6 +33,14 @@ hipError_t launch_voronoi_owner(const double* cells, int ncell, const double* bo
 constexpr int kMixBlocks = 1024, kMixThreads = 256;
 hipError_t launch_diis_mix(const double* coef, int nh, const double* Nh, const double* Rh, int64_t ng, double mix,
                            double lo, double ntot, double dV, double* out, double* part, hipStream_t st);
+// SCF grid passes: LDA potential, residual + DIIS dot products (history rows <= kMaxHist),
+// Harris-Foulkes integrals; sums over kMixBlocks partials per quantity
+constexpr int kMaxHist = 8;
+hipError_t launch_lda_potential(const double* n, const double* vion, const double* vh, int64_t ng, double* v,
+                                hipStream_t st);
+hipError_t launch_scf_residual(const double* nout, double scale, const double* nin, int64_t ng, int nh, int slot,
+                               double* Nh, double* Rh, double* part, double* stats, hipStream_t st);
+hipError_t launch_harris(const double* n, const double* vh, int64_t ng, double* part, double* out, hipStream_t st);
 hipError_t launch_diag_ratio(const lres::WinDesc* wins, int nwin, const double* SW, const double* HW, double* out,
                              hipStream_t st);
 hipError_t launch_inertia(int T, int G, const lres::WinDesc* wins, const int32_t* ids, int nids, const double* SW,

@@ -488,6 +488,111 @@ __global__ void __launch_bounds__(kMixThreads) k_mix_scale(const double* __restr
         out[g] *= sc;
 }
 
+// ---------------------------------------------------------------------------
+// SCF grid work around the density step (lcao3d.py:244-281, three_d.py:132-148):
+// the LDA potential, the residual with its DIIS dot products and the
+// Harris-Foulkes integrals, each one pass over the grid.  Sums go through
+// kMixBlocks fixed partials per quantity and one finishing block (fixed order).
+// ---------------------------------------------------------------------------
+struct LdaXc {
+    double exc, vxc;
+};
+
+// Slater exchange + Perdew-Zunger 1981 correlation of one density value, with
+// the clamp at 1e-14 (three_d.py:132-148)
+__device__ __forceinline__ LdaXc lda_pz81(double n) {
+    constexpr double kPi = 3.14159265358979323846;
+    constexpr double gam = -0.1423, b1 = 1.0529, b2 = 0.3334;
+    constexpr double A = 0.0311, B = -0.048, C = 0.0020, D = -0.0116;
+    n = fmax(n, 1e-14);
+    const double n13 = cbrt(n);
+    const double rs = cbrt(3.0 / (4.0 * kPi)) / n13;
+    const double ex = -0.75 * cbrt(3.0 / kPi) * n13;
+    const double vx = (4.0 / 3.0) * ex;
+    double ec, vc;
+    if (rs >= 1.0) {
+        const double srs = sqrt(rs);
+        const double den = 1.0 + b1 * srs + b2 * rs;
+        ec = gam / den;
+        vc = ec * (1.0 + (7.0 / 6.0) * b1 * srs + (4.0 / 3.0) * b2 * rs) / den;
+    } else {
+        const double lnr = log(rs);
+        ec = A * lnr + B + C * rs * lnr + D * rs;
+        vc = A * lnr + (B - A / 3.0) + (2.0 / 3.0) * C * rs * lnr + (2.0 * D - C) / 3.0 * rs;
+    }
+    return {ex + ec, vx + vc};
+}
+
+// v = v_ion + v_H + v_xc(n)
+__global__ void __launch_bounds__(kMixThreads) k_lda_potential(const double* __restrict__ n,
+                                                               const double* __restrict__ vion,
+                                                               const double* __restrict__ vh, int64_t ng,
+                                                               double* __restrict__ v) {
+    for (int64_t g = (int64_t)blockIdx.x * kMixThreads + threadIdx.x; g < ng; g += (int64_t)kMixBlocks * kMixThreads)
+        v[g] = vion[g] + vh[g] + lda_pz81(n[g]).vxc;
+}
+
+// r = scale nout - nin; Rh[slot] = r, Nh[slot] = nin; partials of sum |r| (row 0)
+// and of R_q . r for every history row q < nh (rows 1..nh; row slot is r . r)
+__global__ void __launch_bounds__(kMixThreads) k_scf_residual(const double* __restrict__ nout, double scale,
+                                                              const double* __restrict__ nin, int64_t ng, int nh,
+                                                              int slot, double* __restrict__ Nh,
+                                                              double* __restrict__ Rh, double* __restrict__ part) {
+    __shared__ double red[16];
+    double s[kMaxHist + 1];
+#pragma unroll
+    for (int q = 0; q <= kMaxHist; ++q) s[q] = 0.0;
+    double* Rs = Rh + (int64_t)slot * ng;
+    for (int64_t g = (int64_t)blockIdx.x * kMixThreads + threadIdx.x; g < ng; g += (int64_t)kMixBlocks * kMixThreads) {
+        const double x = nin[g];
+        const double r = scale * nout[g] - x;
+        Rs[g] = r;
+        Nh[(int64_t)slot * ng + g] = x;
+        s[0] += fabs(r);
+#pragma unroll
+        for (int q = 0; q < kMaxHist; ++q)
+            if (q < nh) s[1 + q] = fma(q == slot ? r : Rh[(int64_t)q * ng + g], r, s[1 + q]);
+    }
+#pragma unroll
+    for (int q = 0; q <= kMaxHist; ++q) {
+        if (q > nh) break;
+        const double t = block_sum(s[q], red);
+        if (threadIdx.x == 0) part[(int64_t)q * kMixBlocks + blockIdx.x] = t;
+    }
+}
+
+// partials of sum n v_H (row 0), sum n e_xc(n) (row 1), sum n (v_H + v_xc(n)) (row 2)
+__global__ void __launch_bounds__(kMixThreads) k_harris(const double* __restrict__ n, const double* __restrict__ vh,
+                                                        int64_t ng, double* __restrict__ part) {
+    __shared__ double red[16];
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int64_t g = (int64_t)blockIdx.x * kMixThreads + threadIdx.x; g < ng; g += (int64_t)kMixBlocks * kMixThreads) {
+        const double x = n[g], h = vh[g];
+        const LdaXc l = lda_pz81(x);
+        s0 = fma(x, h, s0);
+        s1 = fma(x, l.exc, s1);
+        s2 = fma(h + l.vxc, x, s2);
+    }
+    double t = block_sum(s0, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+    t = block_sum(s1, red);
+    if (threadIdx.x == 0) part[kMixBlocks + blockIdx.x] = t;
+    t = block_sum(s2, red);
+    if (threadIdx.x == 0) part[2 * kMixBlocks + blockIdx.x] = t;
+}
+
+// out[q] = sum_b part[q kMixBlocks + b] for q < nq (one block, fixed order)
+__global__ void __launch_bounds__(kMixThreads) k_rows_total(const double* __restrict__ part, int nq,
+                                                            double* __restrict__ out) {
+    __shared__ double red[16];
+    for (int q = 0; q < nq; ++q) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < kMixBlocks; b += kMixThreads) s += part[(int64_t)q * kMixBlocks + b];
+        const double t = block_sum(s, red);
+        if (threadIdx.x == 0) out[q] = t;
+    }
+}
+
 // sum of nwin values in a fixed order into out[0] (one block)
 __global__ void k_sum2(const double* __restrict__ a, const double* __restrict__ b, int n, double* __restrict__ out) {
     __shared__ double red[16];
@@ -582,6 +687,30 @@ hipError_t launch_diis_mix(const double* coef, int nh, const double* Nh, const d
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_mix_scale, dim3(kMixBlocks), dim3(kMixThreads), 0, st, part, ntot, dV, ng, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_lda_potential(const double* n, const double* vion, const double* vh, int64_t ng, double* v,
+                                hipStream_t st) {
+    hipLaunchKernelGGL(k_lda_potential, dim3(kMixBlocks), dim3(kMixThreads), 0, st, n, vion, vh, ng, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_scf_residual(const double* nout, double scale, const double* nin, int64_t ng, int nh, int slot,
+                               double* Nh, double* Rh, double* part, double* stats, hipStream_t st) {
+    hipLaunchKernelGGL(k_scf_residual, dim3(kMixBlocks), dim3(kMixThreads), 0, st, nout, scale, nin, ng, nh, slot,
+                       Nh, Rh, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_rows_total, dim3(1), dim3(kMixThreads), 0, st, part, nh + 1, stats);
+    return hipGetLastError();
+}
+
+hipError_t launch_harris(const double* n, const double* vh, int64_t ng, double* part, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_harris, dim3(kMixBlocks), dim3(kMixThreads), 0, st, n, vh, ng, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_rows_total, dim3(1), dim3(kMixThreads), 0, st, part, 3, out);
     return hipGetLastError();
 }
 

@@ -945,6 +945,39 @@ int lres_diis_mix(const double* d_coef, int nhist, const double* d_Nh, const dou
     });
 }
 
+static_assert(kMaxHist == LRES_MAX_HIST, "history rows of the residual workspace");
+
+int lres_lda_potential(const double* d_n, const double* d_vion, const double* d_vh, int64_t ng, double* d_v,
+                       void* stream) {
+    return guarded([&] {
+        require(ng >= 1, "empty grid");
+        require(d_n && d_vion && d_vh && d_v, "null buffer");
+        hip_check(launch_lda_potential(d_n, d_vion, d_vh, ng, d_v, (hipStream_t)stream), "potential kernel");
+    });
+}
+
+int lres_scf_residual(const double* d_nout, double scale, const double* d_nin, int64_t ng, int nhist, int slot,
+                      double* d_Nh, double* d_Rh, double* d_part, double* d_stats, void* stream) {
+    return guarded([&] {
+        require(ng >= 1, "empty grid");
+        require(nhist >= 1 && nhist <= LRES_MAX_HIST, "history rows must be 1..LRES_MAX_HIST");
+        require(slot >= 0 && slot < nhist, "history slot out of range");
+        require(d_nout && d_nin && d_Nh && d_Rh && d_part && d_stats, "null buffer");
+        hip_check(launch_scf_residual(d_nout, scale, d_nin, ng, nhist, slot, d_Nh, d_Rh, d_part, d_stats,
+                                      (hipStream_t)stream),
+                  "residual kernels");
+    });
+}
+
+int lres_harris_terms(const double* d_n, const double* d_vh, int64_t ng, double* d_part, double* d_out,
+                      void* stream) {
+    return guarded([&] {
+        require(ng >= 1, "empty grid");
+        require(d_n && d_vh && d_part && d_out, "null buffer");
+        hip_check(launch_harris(d_n, d_vh, ng, d_part, d_out, (hipStream_t)stream), "Harris kernels");
+    });
+}
+
 int lres_plan_destroy(lres_plan* P) {
     return guarded([&] {
         if (!P) return;

@@ -420,7 +420,7 @@ class LRFTLCAOCpp:
                 mu = self._solve_mu_warm(mu, cnt, e_lo, N)
                 self.mu = mu
             dens, e_band, _ = self._finish(mu, e_lo)
-            return dens, e_band, mu
+            return 2.0 * dens, e_band, mu
         finally:
             self.set_precision(prec)
             self.count_tol = ctol
@@ -441,7 +441,8 @@ class LRFTLCAOCpp:
                 self.plan.density_last()
         else:
             n, e = self.plan.run(mu, e_lo, want_density=True)
-        return 2.0 * self.plan.dens, 2.0 * e, n
+        # the plan's density buffer (no spin factor) and the band energy with it
+        return self.plan.dens, 2.0 * e, n
 
     def _density(self, v, N=None):
         cr = self.b.cr
@@ -452,14 +453,15 @@ class LRFTLCAOCpp:
         else:
             self.plan.set_h(csr64(self.b.H(v)))
         dens, e_band, mu = self._density_loaded(N)
-        return dens.cpu().numpy().reshape(cr.shape), e_band, mu
+        return 2.0 * dens.cpu().numpy().reshape(cr.shape), e_band, mu
 
-    def density_device(self, v_dev, N=None, potential_set=False):
+    def density_device(self, v_dev, N=None, potential_set=False, spin_scaled=True):
         """density() for a device potential: returns (dens tensor [ng] incl. spin 2, e_band, mu).
 
-        Needs device_h (H(v) assembled on the GPU); the density tensor is the
-        plan's buffer scaled by 2, valid until the next call.  potential_set: the
-        window blocks of v_dev are already in place (assembled by a replayed graph)."""
+        Needs device_h (H(v) assembled on the GPU).  potential_set: the window
+        blocks of v_dev are already in place (assembled by a replayed graph).
+        spin_scaled=False returns the plan's own density buffer without the spin
+        factor 2 (valid until the next pass) instead of a scaled copy."""
         if not self.device_h:
             raise RuntimeError("density_device needs device_h=True")
         N = self.b.cr.n_elec if N is None else N
@@ -475,7 +477,7 @@ class LRFTLCAOCpp:
             self.mu = None
             out = self._density_loaded(N)
         self.pass_log.append(self.plan.passes - p0)
-        return out
+        return (2.0 * out[0],) + out[1:] if spin_scaled else out
 
     def _density_loaded(self, N):
         tol = max(1e-8 * N, self._ftol(N))   # the reference's keep-mu rule, floored at the pass precision

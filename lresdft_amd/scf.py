@@ -31,6 +31,9 @@ class GridOps:
         self.inv_G2[0, 0, 0] = 0.0
         self.G2 = G2
         self.G2[0, 0, 0] = 0.0
+        # 4 pi / G^2 on the half spectrum of a real-to-complex transform
+        self.w_half = (4 * math.pi * self.inv_G2[:, :, : cr.shape[2] // 2 + 1]).contiguous()
+        self.part = None
 
     def ion_potential(self, chunk=256):
         """Periodic sum of -Z erf(r/rc)/r, G = 0 dropped (three_d.py:102-119).
@@ -53,9 +56,46 @@ class GridOps:
         return torch.fft.ifftn(vG).real * cr.ng
 
     def hartree(self, n):
+        """v_H = IFFT(4 pi FFT(n) / G^2), G = 0 dropped (three_d.py:122-129); the
+        density is real, so real-to-complex transforms over the half spectrum."""
         torch = self.torch
-        nG = torch.fft.fftn(n)
-        return torch.fft.ifftn(4 * math.pi * nG * self.inv_G2).real
+        n = n.reshape(self.cr.shape)
+        return torch.fft.irfftn(torch.fft.rfftn(n) * self.w_half, s=tuple(self.cr.shape))
+
+    def _workspace(self):
+        if self.part is None:
+            rows = max(3, nat.LRES_MAX_HIST + 1)
+            self.part = self.torch.empty(rows * nat.LRES_MIX_PARTS, dtype=self.torch.float64, device=self.device)
+        return self.part
+
+    def potential(self, n, vion):
+        """v = v_ion + v_H(n) + v_xc(n) as a flat device tensor: the FFT Hartree
+        potential, then one fused kernel for the LDA potential and the sum
+        (lres_lda_potential)."""
+        torch = self.torch
+        n = n.reshape(-1).contiguous()
+        vh = self.hartree(n).reshape(-1)
+        v = torch.empty_like(n)
+        nat.check(nat.lib().lres_lda_potential(n.data_ptr(), vion.reshape(-1).contiguous().data_ptr(), vh.data_ptr(),
+                                               n.numel(), v.data_ptr(),
+                                               nat.C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        return v
+
+    def harris(self, n, eband):
+        """Harris-Foulkes energy at input density n (lcao3d.py:275-280):
+        (E, E_H, E_xc) with E = e_band - int (v_H + v_xc) n + E_H + E_xc.  One
+        pass over the grid (lres_harris_terms) and one transfer of three sums."""
+        torch = self.torch
+        n = n.reshape(-1).contiguous()
+        vh = self.hartree(n).reshape(-1)
+        out = torch.empty(3, dtype=torch.float64, device=self.device)
+        nat.check(nat.lib().lres_harris_terms(n.data_ptr(), vh.data_ptr(), n.numel(), self._workspace().data_ptr(),
+                                              out.data_ptr(), nat.C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        s_vh, s_exc, s_v = out.cpu().tolist()
+        dV = self.cr.dV
+        E_H = 0.5 * s_vh * dV
+        E_xc = s_exc * dV
+        return eband - s_v * dV + E_H + E_xc, E_H, E_xc
 
     def lda_xc(self, n):
         """Slater exchange + Perdew-Zunger 1981 correlation (three_d.py:132-148)."""
@@ -109,8 +149,7 @@ class PotentialStep:
                                           self.Rh.data_ptr(), self.cr.ng, self.mix, 1e-12, float(self.cr.n_elec),
                                           float(self.cr.dV), self.dens.data_ptr(), self.part.data_ptr(),
                                           nat.C.c_void_p(torch.cuda.current_stream().cuda_stream)))
-        d = self.dens.reshape(self.cr.shape)
-        v = (self.vion + self.grid.hartree(d) + self.grid.lda_xc(d)[1]).reshape(-1)
+        v = self.grid.potential(self.dens, self.vion)
         self.solver.plan.set_potential(v)
         return v
 
@@ -200,9 +239,13 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
     err = None
     adaptive = hasattr(solver, "count_tol") and hasattr(solver, "refine")
     cfull = None
+    if step is not None:
+        stats_dev = torch.empty(ndiis + 1, dtype=torch.float64, device=grid.device)
+        if ndiis > nat.LRES_MAX_HIST:
+            raise ValueError(f"ndiis must be at most {nat.LRES_MAX_HIST}")
     for it in range(1, maxiter + 1):
         if cfull is None:
-            v = (vion + grid.hartree(dens) + grid.lda_xc(dens)[1]).reshape(-1).contiguous()
+            v = grid.potential(dens, vion)
             preset = False
         elif step is not None:
             dflat, v = step.run(cfull)
@@ -212,7 +255,7 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
             cvec = torch.from_numpy(cfull).to(grid.device)
             dens = torch.clamp((cvec @ Nh + mix * (cvec @ Rh)).reshape(cr.shape), min=1e-12)
             dens = dens * (N / (dens.sum() * dV))
-            v = (vion + grid.hartree(dens) + grid.lda_xc(dens)[1]).reshape(-1).contiguous()
+            v = grid.potential(dens, vion)
             preset = False
         if adaptive:
             # mu to a count accuracy matched to the current density residual: a count
@@ -221,21 +264,30 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
             # the strict tolerance below
             solver.count_tol = 1e-4 if err is None else min(1e-4, max(count_rel * err, 1e-11))
         ts = time.perf_counter()
-        if preset:
-            nout, eband, mu = solver.density_device(v, potential_set=True)
-        else:
-            nout, eband, mu = solver.density_device(v)
-        nout = nout.reshape(cr.shape)
-        t_solver += time.perf_counter() - ts
-        resid = nout - dens
         # the new history slot (the oldest one once the history is full)
         free = [q for q in range(ndiis) if q not in slots]
         slot = free[0] if free else slots[0]
-        Rh[slot] = resid.reshape(-1)
-        Nh[slot] = dens.reshape(-1)
         live = slots[1:] + [slot] if len(slots) == ndiis else slots + [slot]
-        # |resid| and the dot products with every ring slot (unused slots are zero)
-        stats = torch.cat([torch.sum(torch.abs(resid)).reshape(1), Rh @ resid.reshape(-1)]).cpu().numpy()
+        if step is not None:
+            # density without the spin factor; one fused pass forms the residual
+            # 2 n_out - n_in, stores the history row and the Gram row
+            nraw, eband, mu = solver.density_device(v, potential_set=preset, spin_scaled=False)
+            t_solver += time.perf_counter() - ts
+            nat.check(nat.lib().lres_scf_residual(
+                nraw.data_ptr(), 2.0, dens.reshape(-1).contiguous().data_ptr(), cr.ng, ndiis, slot, Nh.data_ptr(),
+                Rh.data_ptr(), grid._workspace().data_ptr(), stats_dev.data_ptr(),
+                nat.C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+            stats = stats_dev.cpu().numpy()
+            nout = None
+        else:
+            nout, eband, mu = solver.density_device(v)
+            nout = nout.reshape(cr.shape)
+            t_solver += time.perf_counter() - ts
+            resid = nout - dens
+            Rh[slot] = resid.reshape(-1)
+            Nh[slot] = dens.reshape(-1)
+            # |resid| and the dot products with every ring slot (unused slots are zero)
+            stats = torch.cat([torch.sum(torch.abs(resid)).reshape(1), Rh @ resid.reshape(-1)]).cpu().numpy()
         err = float(stats[0]) * dV / N
         errs.append(err)
         if err < tol:
@@ -268,14 +320,12 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
         nout, eband, mu = solver.refine()
         nout = nout.reshape(cr.shape)
         t_solver += time.perf_counter() - ts
+    elif nout is None:
+        nout = 2.0 * nraw.reshape(cr.shape)
     if handed_over:
         solver.set_precision("mixed")
     dens = dens.clone()
-    vh = grid.hartree(dens)
-    exc, vxc = grid.lda_xc(dens)
-    E_H = 0.5 * float(torch.sum(dens * vh)) * dV
-    E_xc = float(torch.sum(dens * exc)) * dV
-    E = eband - float(torch.sum((vh + vxc) * dens)) * dV + E_H + E_xc
+    E, E_H, E_xc = grid.harris(dens, eband)
     torch.cuda.synchronize()
     return dict(energy=E, density=nout, iterations=it, converged=converged, dens_in=dens, eband=eband, mu=mu,
                 time=time.perf_counter() - t0, t_solver=t_solver, E_H=E_H, E_xc=E_xc, residuals=errs,
