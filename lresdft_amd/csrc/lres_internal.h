@@ -21,8 +21,11 @@ hipError_t launch_gather_csr(const lres::WinDesc* wins, int nwin, const int64_t*
 hipError_t launch_pole_gj(int T, int G, int wide, int fp32, const lres::WinDesc* wins, const int32_t* ids, int nids,
                           const double* SW, const double* HW, int M, int ngroup, double c, double Rr,
                           double* Dpart, double* Fpart, hipStream_t st);
+// D, F, N_B, E_B from the pole kernels' partials: group sums (Dpart, Fpart) or,
+// for windows with by_pole[w] != 0, nslot per-pole float pairs in `slots`
 hipError_t launch_reduce(const lres::WinDesc* wins, int nwin, int ngroup, const double* Dpart, const double* Fpart,
-                         const double* Q, double* D, double* F, double* NB, double* EB, hipStream_t st);
+                         const float* slots, int nslot, const int8_t* by_pole, const double* Q, double* D, double* F,
+                         double* NB, double* EB, hipStream_t st);
 hipError_t launch_density(const lres::WinDesc* wins, int nwin, int kmax, const double* D, const int64_t* own_g,
                           const int64_t* xptr, const int32_t* xidx, const double* xval, double* dens,
                           hipStream_t st);

@@ -279,9 +279,12 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // (G[T,T] is complex symmetric); this walks the upper triangle in row segments of
 // 32 (coalesced reads), writes each entry and its mirror, and forms the traces
 // from the upper triangle with the off-diagonal entries doubled (Q_B is
-// symmetric).
+// symmetric).  Windows with by_pole[w] != 0 (complex64 matrix-core passes) hold
+// one float pair (Re w G, Re w z G) per pole instead, nslot poles in order.
 __global__ void __launch_bounds__(256) k_reduce(const WinDesc* __restrict__ wins, int ngroup,
                                                 const double* __restrict__ Dpart, const double* __restrict__ Fpart,
+                                                const float2* __restrict__ slots, int nslot,
+                                                const int8_t* __restrict__ by_pole,
                                                 const double* __restrict__ Q, double* __restrict__ D,
                                                 double* __restrict__ F, double* __restrict__ NB,
                                                 double* __restrict__ EB) {
@@ -292,6 +295,8 @@ __global__ void __launch_bounds__(256) k_reduce(const WinDesc* __restrict__ wins
     const int64_t kk = (int64_t)k * k;
     const double* dp = Dpart + d.kk_off * ngroup;
     const double* fp = Fpart + d.kk_off * ngroup;
+    const bool poles = by_pole != nullptr && by_pole[w] != 0;
+    const float2* gs = slots + d.kk_off * nslot;
     double* Dw = D + d.kk_off;
     double* Fw = F + d.kk_off;
     const double* Qw = Q + d.kk_off;
@@ -301,9 +306,17 @@ __global__ void __launch_bounds__(256) k_reduce(const WinDesc* __restrict__ wins
         for (int j = i + tx; j < k; j += 32) {
             const int64_t q = (int64_t)i * k + j;
             double dv = 0.0, fv = 0.0;
-            for (int gq = 0; gq < ngroup; ++gq) {
-                dv += dp[gq * kk + q];
-                fv += fp[gq * kk + q];
+            if (poles) {
+                for (int j = 0; j < nslot; ++j) {
+                    const float2 g = gs[j * kk + q];
+                    dv += (double)g.x;
+                    fv += (double)g.y;
+                }
+            } else {
+                for (int gq = 0; gq < ngroup; ++gq) {
+                    dv += dp[gq * kk + q];
+                    fv += fp[gq * kk + q];
+                }
             }
             dv *= 2.0;
             fv *= 2.0;
@@ -806,8 +819,10 @@ int mfma_scratch_bytes(int fp32, int nt, int nwc) {
 }
 
 hipError_t launch_reduce(const WinDesc* wins, int nwin, int ngroup, const double* Dpart, const double* Fpart,
-                         const double* Q, double* D, double* F, double* NB, double* EB, hipStream_t st) {
-    hipLaunchKernelGGL(k_reduce, dim3(nwin), dim3(256), 0, st, wins, ngroup, Dpart, Fpart, Q, D, F, NB, EB);
+                         const float* slots, int nslot, const int8_t* by_pole, const double* Q, double* D, double* F,
+                         double* NB, double* EB, hipStream_t st) {
+    hipLaunchKernelGGL(k_reduce, dim3(nwin), dim3(256), 0, st, wins, ngroup, Dpart, Fpart,
+                       reinterpret_cast<const float2*>(slots), nslot, by_pole, Q, D, F, NB, EB);
     return hipGetLastError();
 }
 

@@ -162,8 +162,11 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const double* S = SW + d.mat_off;
     const double* H = HW + d.mat_off;
+    // fp64: this group's partial sums (Dpart, Fpart); fp32: Dpart is the per-pole
+    // slot array, (M / 2) k x k float pairs per window
     double* dp = Dpart + d.kk_off * ngroup + (int64_t)gq * k * k;
     double* fp = Fpart + d.kk_off * ngroup + (int64_t)gq * k * k;
+    float2* gs = reinterpret_cast<float2*>(Dpart) + d.kk_off * (M / 2);
     const int nst = (n + 3) >> 2;
 
     // are[I][j], aim[I][j]: tile (I, wv + NWC j), held for I < held(j) only
@@ -341,7 +344,9 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
         });
         __syncthreads();   // the next pole's first publish rewrites buffer 0
 
-        // accumulate Re(w G) and Re(w z G) on T x T into this group's partial slot
+        // Re(w G) and Re(w z G) on T x T: fp64 adds them to this group's partial
+        // sums; fp32 stores them once, as a float pair, in this pole's own slot (no
+        // read-modify-write; the reduction sums the poles in order)
         int kk = k, mm = m, ln1 = lane;
         asm volatile("" : "+s"(kk), "+s"(mm), "+v"(ln1));
         const int ei = ln1 & 15, eg = ln1 >> 4;
@@ -358,7 +363,9 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
                         const double gr = (double)are[I][j][r], gi = (double)aim[I][j][r];
                         const double dv = fma(wt.re, gr, -(wt.im * gi)), fv = fma(wz.re, gr, -(wz.im * gi));
                         const int64_t q = (int64_t)row * kk + col;
-                        if (first) {
+                        if constexpr (sizeof(R) == 4) {
+                            gs[(int64_t)jp * kk * kk + q] = make_float2((float)dv, (float)fv);
+                        } else if (first) {
                             dp[q] = dv;
                             fp[q] = fv;
                         } else {

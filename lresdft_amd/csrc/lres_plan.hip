@@ -175,6 +175,11 @@ struct lres_plan {
     DevBuf<int64_t> d_ws_off;                   // workspace offset per window (big windows), cx<double> units
     DevBuf<double> d_work;                      // workspace of the blocked sweep (2 doubles per entry)
     DevBuf<double> d_SW, d_HW, d_Q, d_D, d_F, d_Dpart, d_Fpart, d_NB, d_EB, d_sums;
+    // complex64 matrix-core passes: one float pair per pole and upper-triangle entry
+    // ((M / 2) k x k pairs per window), and which windows' partials are in that form
+    DevBuf<float> d_slots;
+    DevBuf<int8_t> d_by_pole;
+    std::vector<int8_t> by_pole;
     DevBuf<int64_t> d_own_g, d_xptr;
     DevBuf<int32_t> d_xidx;
     DevBuf<double> d_xval;
@@ -245,6 +250,15 @@ bool big_on_mfma(int nmax, int fp32) {
     return fp32 && nmax <= 256 && use_mfma(1) && !mfma_shapes(nmax, 1).empty();
 }
 
+// The partial-sum buffer a pole kernel of shape sh writes: the per-pole slots for
+// the complex64 matrix-core kernel (allocated on first use), else the group sums.
+double* pole_partials(lres_plan* P, const Shape& sh, int fp32) {
+    if (!(fp32 && sh.wide == kMfma)) return P->d_Dpart.p;
+    const size_t need = P->d_Q.n * (size_t)(P->M / 2) * 2;
+    if (P->d_slots.n < need) P->d_slots.alloc(need);
+    return reinterpret_cast<double*>(P->d_slots.p);
+}
+
 // Autotuner: time every valid shape of a bucket on the loaded H (two launches
 // after one warm-up, HIP events) and keep the fastest.  All shapes apply the
 // same operations to every matrix entry, so the choice never changes results.
@@ -261,7 +275,8 @@ void tune_bucket(lres_plan* P, lres_plan::Bucket& b, int fp32, double c, double 
     for (const auto& sh : cands) {
         auto launch = [&] {
             hip_check(launch_pole(sh, fp32, P->d_wins.p, P->d_bucket_ids.p + b.id_off, (int)b.ids.size(), P->d_SW.p,
-                                  P->d_HW.p, P->M, P->ngroup, c, R, P->d_Dpart.p, P->d_Fpart.p, P->st),
+                                  P->d_HW.p, P->M, P->ngroup, c, R, pole_partials(P, sh, fp32), P->d_Fpart.p,
+                                  P->st),
                       "pole kernel (tuning)");
         };
         launch();
@@ -292,6 +307,7 @@ void run_pass(lres_plan* P, double mu, double e_lo, double* d_dens, double* d_NB
     const double c = 0.5 * (mu + e_lo), R = 0.5 * (mu - e_lo);
     hip_check(hipEventRecord(P->ev[2], P->st), "event");
     const int fp32 = P->precision == LRES_MIXED ? 1 : 0;
+    std::vector<int8_t> by_pole(P->nwin, 0);
     for (auto& b : P->buckets) {
         if (b.big && !big_on_mfma(b.nmax, fp32)) {
             hip_check(launch_pole_big(fp32, b.bmax, b.nmax, P->d_wins.p, P->d_bucket_ids.p + b.id_off,
@@ -302,13 +318,24 @@ void run_pass(lres_plan* P, double mu, double e_lo, double* d_dens, double* d_NB
         }
         if (!b.tuned[fp32]) tune_bucket(P, b, fp32, c, R);
         hip_check(launch_pole(b.shape[fp32], fp32, P->d_wins.p, P->d_bucket_ids.p + b.id_off, (int)b.ids.size(),
-                              P->d_SW.p, P->d_HW.p, P->M, P->ngroup, c, R, P->d_Dpart.p, P->d_Fpart.p, P->st),
+                              P->d_SW.p, P->d_HW.p, P->M, P->ngroup, c, R, pole_partials(P, b.shape[fp32], fp32),
+                              P->d_Fpart.p, P->st),
                   "pole kernel");
+        if (fp32 && b.shape[fp32].wide == kMfma)
+            for (int32_t w : b.ids) by_pole[w] = 1;
+    }
+    // complex64 passes only (fp64 partials are always group sums); changes only
+    // when a bucket's kernel family does
+    if (fp32 && by_pole != P->by_pole) {
+        P->d_by_pole.upload(by_pole.data(), by_pole.size(), P->st);
+        hip_check(hipStreamSynchronize(P->st), "sync");
+        P->by_pole = by_pole;
     }
     hip_check(hipEventRecord(P->ev[3], P->st), "event");
     double* NB = d_NB ? d_NB : P->d_NB.p;
     double* EB = d_EB ? d_EB : P->d_EB.p;
-    hip_check(launch_reduce(P->d_wins.p, (int)P->nwin, P->ngroup, P->d_Dpart.p, P->d_Fpart.p, P->d_Q.p, P->d_D.p,
+    hip_check(launch_reduce(P->d_wins.p, (int)P->nwin, P->ngroup, P->d_Dpart.p, P->d_Fpart.p, P->d_slots.p, P->M / 2,
+                            fp32 ? P->d_by_pole.p : nullptr, P->d_Q.p, P->d_D.p,
                             P->d_F.p, NB, EB, P->st),
               "reduce kernel");
     if (d_dens)
