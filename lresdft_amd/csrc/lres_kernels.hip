@@ -281,7 +281,7 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // from the upper triangle with the off-diagonal entries doubled (Q_B is
 // symmetric).  Windows with by_pole[w] != 0 (complex64 matrix-core passes) hold
 // one float pair (Re w G, Re w z G) per pole instead, nslot poles in order.
-__global__ void __launch_bounds__(256) k_reduce(const WinDesc* __restrict__ wins, int ngroup,
+__global__ void __launch_bounds__(1024) k_reduce(const WinDesc* __restrict__ wins, int ngroup,
                                                 const double* __restrict__ Dpart, const double* __restrict__ Fpart,
                                                 const float2* __restrict__ slots, int nslot,
                                                 const int8_t* __restrict__ by_pole,
@@ -307,10 +307,17 @@ __global__ void __launch_bounds__(256) k_reduce(const WinDesc* __restrict__ wins
             const int64_t q = (int64_t)i * k + j;
             double dv = 0.0, fv = 0.0;
             if (poles) {
-                for (int j = 0; j < nslot; ++j) {
-                    const float2 g = gs[j * kk + q];
-                    dv += (double)g.x;
-                    fv += (double)g.y;
+                // eight loads in flight, summed in pole order
+                for (int j0 = 0; j0 < nslot; j0 += 8) {
+                    float2 g[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+                        g[u] = j0 + u < nslot ? gs[(j0 + u) * kk + q] : make_float2(0.0f, 0.0f);
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        dv += (double)g[u].x;
+                        fv += (double)g[u].y;
+                    }
                 }
             } else {
                 for (int gq = 0; gq < ngroup; ++gq) {
@@ -821,7 +828,9 @@ int mfma_scratch_bytes(int fp32, int nt, int nwc) {
 hipError_t launch_reduce(const WinDesc* wins, int nwin, int ngroup, const double* Dpart, const double* Fpart,
                          const float* slots, int nslot, const int8_t* by_pole, const double* Q, double* D, double* F,
                          double* NB, double* EB, hipStream_t st) {
-    hipLaunchKernelGGL(k_reduce, dim3(nwin), dim3(256), 0, st, wins, ngroup, Dpart, Fpart,
+    // 16 waves per window: enough loads in flight for the per-pole partials (16 float
+    // pairs per entry)
+    hipLaunchKernelGGL(k_reduce, dim3(nwin), dim3(1024), 0, st, wins, ngroup, Dpart, Fpart,
                        reinterpret_cast<const float2*>(slots), nslot, by_pole, Q, D, F, NB, EB);
     return hipGetLastError();
 }
