@@ -26,8 +26,11 @@ hipError_t launch_pole_gj(int T, int G, int wide, int fp32, const lres::WinDesc*
 hipError_t launch_reduce(const lres::WinDesc* wins, int nwin, int ngroup, const double* Dpart, const double* Fpart,
                          const float* slots, int nslot, const int8_t* by_pole, const double* Q, double* D, double* F,
                          double* NB, double* EB, hipStream_t st);
+// blocks6: every owned row of X is whole atom blocks (6 consecutive T-local columns
+// each, at most kMaxAtomBlocks blocks), checked on the host; then one lane per point
+constexpr int kMaxAtomBlocks = 8;
 hipError_t launch_density(const lres::WinDesc* wins, int nwin, int kmax, const double* D, const int64_t* own_g,
-                          const int64_t* xptr, const int32_t* xidx, const double* xval, double* dens,
+                          const int64_t* xptr, const int32_t* xidx, const double* xval, double* dens, int blocks6,
                           hipStream_t st);
 hipError_t launch_sum2(const double* a, const double* b, int n, double* out, hipStream_t st);
 hipError_t launch_voronoi_owner(const double* cells, int ncell, const double* box, const int64_t* shape, double dx,

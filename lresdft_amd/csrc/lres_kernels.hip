@@ -470,6 +470,81 @@ __global__ void __launch_bounds__(64 * kDensWaves) k_density(const WinDesc* __re
     }
 }
 
+// n(g) = x_g^T D_B x_g when every owned row of X is made of whole atom blocks: a
+// multiple of 6 entries, each 6 in consecutive T-local columns (the 6 functions of
+// one atom share its support), at most kMaxAtomBlocks blocks.  One lane per owned
+// point sums x_A^T D[A, B] x_B over the atom blocks A <= B of its row (pairs
+// A < B doubled: D_B is symmetric), reading the 6 x 6 blocks of D_B from LDS, so
+// only the entries of D_B the point touches are read; each lane adds in a fixed
+// order (bitwise reproducible).
+constexpr int kDensBlkThreads = 512;
+
+template <bool STAGE>
+__global__ void __launch_bounds__(kDensBlkThreads) k_density_blk(const WinDesc* __restrict__ wins,
+                                                                 const double* __restrict__ D,
+                                                                 const int64_t* __restrict__ own_g,
+                                                                 const int64_t* __restrict__ xptr,
+                                                                 const int32_t* __restrict__ xidx,
+                                                                 const double* __restrict__ xval,
+                                                                 double* __restrict__ dens) {
+    extern __shared__ double Dl[];
+    const int w = blockIdx.x;
+    const WinDesc d = wins[w];
+    const int k = d.k;
+    const double* Dw = D + d.kk_off;
+    if constexpr (STAGE) {
+        for (int e = threadIdx.x; e < k * k; e += blockDim.x) Dl[e] = Dw[e];
+        __syncthreads();
+    }
+    auto Dr = [&](int a, int b) __attribute__((always_inline)) {
+        if constexpr (STAGE)
+            return Dl[a * k + b];
+        else
+            return Dw[(int64_t)a * k + b];
+    };
+    for (int p = threadIdx.x; p < d.nown; p += blockDim.x) {
+        const int64_t o = d.own_off + p;
+        const int64_t r0 = xptr[o];
+        const int nb = (int)((xptr[o + 1] - r0) / 6);
+        double acc = 0.0;
+#pragma unroll
+        for (int A = 0; A < kMaxAtomBlocks; ++A) {
+            if (A >= nb) break;
+            const int sa = xidx[r0 + 6 * A];
+            double xa[6];
+#pragma unroll
+            for (int i = 0; i < 6; ++i) xa[i] = xval[r0 + 6 * A + i];
+            double t = 0.0;
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                double y = 0.0;
+#pragma unroll
+                for (int j = 0; j < 6; ++j) y = fma(Dr(sa + i, sa + j), xa[j], y);
+                t = fma(xa[i], y, t);
+            }
+            acc += t;
+#pragma unroll
+            for (int B = A + 1; B < kMaxAtomBlocks; ++B) {
+                if (B >= nb) break;
+                const int sb = xidx[r0 + 6 * B];
+                double xb[6];
+#pragma unroll
+                for (int j = 0; j < 6; ++j) xb[j] = xval[r0 + 6 * B + j];
+                double u = 0.0;
+#pragma unroll
+                for (int i = 0; i < 6; ++i) {
+                    double y = 0.0;
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) y = fma(Dr(sa + i, sb + j), xb[j], y);
+                    u = fma(xa[i], y, u);
+                }
+                acc = fma(2.0, u, acc);
+            }
+        }
+        dens[own_g[o]] = acc;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // SCF density mixing (the caller's DIIS step): out = max(sum_h c_h (N_h + mix R_h), lo),
 // then out *= ntot / (sum(out) dV).  Fixed grid and block shapes give the sum a fixed
@@ -836,11 +911,20 @@ hipError_t launch_reduce(const WinDesc* wins, int nwin, int ngroup, const double
 }
 
 hipError_t launch_density(const WinDesc* wins, int nwin, int kmax, const double* D, const int64_t* own_g,
-                          const int64_t* xptr, const int32_t* xidx, const double* xval, double* dens,
+                          const int64_t* xptr, const int32_t* xidx, const double* xval, double* dens, int blocks6,
                           hipStream_t st) {
     const size_t lds = (size_t)kmax * kmax * sizeof(double);
     // D_B in LDS within the 64 KB default of dynamic LDS
     const int stage = lds <= 56 * 1024 ? 1 : 0;
+    if (blocks6) {
+        if (stage)
+            hipLaunchKernelGGL(k_density_blk<true>, dim3(nwin), dim3(kDensBlkThreads), lds, st, wins, D, own_g, xptr,
+                               xidx, xval, dens);
+        else
+            hipLaunchKernelGGL(k_density_blk<false>, dim3(nwin), dim3(kDensBlkThreads), 0, st, wins, D, own_g, xptr,
+                               xidx, xval, dens);
+        return hipGetLastError();
+    }
     if (stage)
         hipLaunchKernelGGL(k_density<true>, dim3(nwin), dim3(64 * kDensWaves), lds, st, wins, D, own_g, xptr, xidx,
                            xval, dens);

@@ -181,6 +181,7 @@ struct lres_plan {
     DevBuf<int8_t> d_by_pole;
     std::vector<int8_t> by_pole;
     DevBuf<int64_t> d_own_g, d_xptr;
+    int dens_blocks6 = 0;                        // owned rows of X are whole 6-function atom blocks
     DevBuf<int32_t> d_xidx;
     DevBuf<double> d_xval;
     DevBuf<int64_t> d_Hp, d_Hi;
@@ -248,6 +249,13 @@ void gather_h(lres_plan* P, const int64_t* Hp, const int64_t* Hi, const double* 
 // otherwise (fp64, or larger) the blocked sweep in a workspace.
 bool big_on_mfma(int nmax, int fp32) {
     return fp32 && nmax <= 256 && use_mfma(1) && !mfma_shapes(nmax, 1).empty();
+}
+
+// Density kernel form: one lane per point over atom blocks when the owned rows of X
+// allow it, unless LRES_DENSITY_KERNEL=rows (the row-per-wave kernel, any row shape)
+int density_blocks(const lres_plan* P) {
+    const char* e = std::getenv("LRES_DENSITY_KERNEL");
+    return P->dens_blocks6 && !(e && std::strcmp(e, "rows") == 0);
 }
 
 // The partial-sum buffer a pole kernel of shape sh writes: the per-pole slots for
@@ -340,7 +348,7 @@ void run_pass(lres_plan* P, double mu, double e_lo, double* d_dens, double* d_NB
               "reduce kernel");
     if (d_dens)
         hip_check(launch_density(P->d_wins.p, (int)P->nwin, P->kmax, P->d_D.p, P->d_own_g.p, P->d_xptr.p, P->d_xidx.p,
-                                 P->d_xval.p, d_dens, P->st),
+                                 P->d_xval.p, d_dens, density_blocks(P), P->st),
                   "density kernel");
     hip_check(launch_sum2(NB, EB, (int)P->nwin, P->d_sums.p, P->st), "sum kernel");
     if (h_sums) {
@@ -760,6 +768,19 @@ int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, 
         P->d_NB.alloc(nwin);
         P->d_EB.alloc(nwin);
         P->d_sums.alloc(2);
+        // whole atom blocks in every owned row: 6 consecutive T-local columns per
+        // block, at most kMaxAtomBlocks blocks (the density kernel's lane-per-point form)
+        P->dens_blocks6 = 1;
+        for (size_t o = 0; o + 1 < xptr.size() && P->dens_blocks6; ++o) {
+            const int64_t r0 = xptr[o], nnz = xptr[o + 1] - r0;
+            if (nnz % 6 != 0 || nnz > 6 * kMaxAtomBlocks) {
+                P->dens_blocks6 = 0;
+                break;
+            }
+            for (int64_t q = r0; q < r0 + nnz; q += 6)
+                for (int i = 1; i < 6; ++i)
+                    if (xidx[q + i] != xidx[q] + i) P->dens_blocks6 = 0;
+        }
         P->d_own_g.upload(own_g.data(), own_g.size(), st);
         P->d_xptr.upload(xptr.data(), xptr.size(), st);
         P->d_xidx.upload(xidx.data(), std::max<size_t>(xidx.size(), 0), st);
@@ -1079,7 +1100,7 @@ int lres_density_last(lres_plan* P, double* d_dens) {
         require(P != nullptr, "null plan");
         require(d_dens != nullptr, "null density buffer");
         hip_check(launch_density(P->d_wins.p, (int)P->nwin, P->kmax, P->d_D.p, P->d_own_g.p, P->d_xptr.p, P->d_xidx.p,
-                                 P->d_xval.p, d_dens, P->st),
+                                 P->d_xval.p, d_dens, density_blocks(P), P->st),
                   "density kernel");
     });
 }

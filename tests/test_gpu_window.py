@@ -64,6 +64,37 @@ def test_fixed_mu_pass_matches_oracle(sys3, M):
     assert np.array_equal(C.plan.dens.cpu().numpy(), dens)
 
 
+def test_density_kernels_agree(sys3):
+    """The lane-per-point atom-block density kernel (the default for this basis,
+    whose rows are whole 6-function atom blocks) and the row-per-wave kernel
+    (LRES_DENSITY_KERNEL=rows) both match the oracle pass to 1e-12 and each
+    other to 1e-13 (different summation orders)."""
+    import os
+    cr, b, v = sys3
+    print("[gpu] density kernels: atom blocks vs rows, n=3 R_w=8 M=32", flush=True)
+    from lresdft_amd.lcao3d import csr64
+    cells = cell_centres((3, 3, 3), 6.0)
+    H = b.H(v)
+    W, own = build_windows(b, cells, 8.0)
+    lam_min = min(linalg.eigh(H[w][:, w].toarray(), b.S[w][:, w].toarray(), eigvals_only=True)[0] for w in W)
+    e_lo, mu = lam_min - 0.3, lam_min + 0.35
+    out = {}
+    for kind in ("blocks", "rows"):
+        if kind == "rows":
+            os.environ["LRES_DENSITY_KERNEL"] = "rows"
+        try:
+            C = _plan(b, cells, 8.0, 32)
+            C.plan.set_h(csr64(H))
+            C.plan.run(mu, e_lo, want_density=True)
+            out[kind] = C.plan.dens.cpu().numpy().copy()
+        finally:
+            os.environ.pop("LRES_DENSITY_KERNEL", None)
+    dp, _, _ = pole_window_pass(H, b.S, b.Xr, W, own, cr.dV, mu, e_lo, 32, True)
+    assert np.max(np.abs(out["blocks"] - dp)) < 1e-12
+    assert np.max(np.abs(out["rows"] - dp)) < 1e-12
+    assert np.max(np.abs(out["blocks"] - out["rows"])) < 1e-13
+
+
 def test_inertia_counts_match_library_eigenvalues(sys3):
     cr, b, v = sys3
     print("[gpu] inertia n=3 R_w=8", flush=True)
