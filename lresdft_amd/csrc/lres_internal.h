@@ -26,6 +26,8 @@ hipError_t launch_pole_gj(int T, int G, int wide, int fp32, const lres::WinDesc*
 hipError_t launch_reduce(const lres::WinDesc* wins, int nwin, int ngroup, const double* Dpart, const double* Fpart,
                          const float* slots, int nslot, const int8_t* by_pole, const double* Q, double* D, double* F,
                          double* NB, double* EB, hipStream_t st);
+// lower triangles of D_B (and F_B if F != NULL) from the upper ones the reduction writes
+hipError_t launch_mirror(const lres::WinDesc* wins, int nwin, double* D, double* F, hipStream_t st);
 // blocks6: every owned row of X is whole atom blocks (6 consecutive T-local columns
 // each, at most kMaxAtomBlocks blocks), checked on the host; then one lane per point
 constexpr int kMaxAtomBlocks = 8;

@@ -277,9 +277,9 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // D_B = 2 sum over pole groups of the partials, F_B likewise, N_B = Tr(D_B Q_B),
 // E_B = Tr(F_B Q_B).  The pole kernels write the upper triangle of each partial
 // (G[T,T] is complex symmetric); this walks the upper triangle in row segments of
-// 32 (coalesced reads), writes each entry and its mirror, and forms the traces
-// from the upper triangle with the off-diagonal entries doubled (Q_B is
-// symmetric).  Windows with by_pole[w] != 0 (complex64 matrix-core passes) hold
+// 32 (coalesced reads and writes), writes the upper triangle of D_B and F_B
+// (k_mirror fills the lower one for the consumers that read it) and forms the
+// traces with the off-diagonal entries doubled (Q_B is symmetric).  Windows with by_pole[w] != 0 (complex64 matrix-core passes) hold
 // one float pair (Re w G, Re w z G) per pole instead, nslot poles in order.
 __global__ void __launch_bounds__(1024) k_reduce(const WinDesc* __restrict__ wins, int ngroup,
                                                 const double* __restrict__ Dpart, const double* __restrict__ Fpart,
@@ -307,22 +307,36 @@ __global__ void __launch_bounds__(1024) k_reduce(const WinDesc* __restrict__ win
             const int64_t q = (int64_t)i * k + j;
             double dv = 0.0, fv = 0.0;
             if (poles) {
-                // eight loads in flight, summed in pole order
+                // eight loads in flight, summed in pole order: every load is issued
+                // (index clamped to the last pole) and the values past the last
+                // pole are masked, so no load sits behind a branch of its own
                 for (int j0 = 0; j0 < nslot; j0 += 8) {
                     float2 g[8];
 #pragma unroll
-                    for (int u = 0; u < 8; ++u)
-                        g[u] = j0 + u < nslot ? gs[(j0 + u) * kk + q] : make_float2(0.0f, 0.0f);
+                    for (int u = 0; u < 8; ++u) g[u] = gs[(int64_t)min(j0 + u, nslot - 1) * kk + q];
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
-                        dv += (double)g[u].x;
-                        fv += (double)g[u].y;
+                        const bool in = j0 + u < nslot;
+                        dv += in ? (double)g[u].x : 0.0;
+                        fv += in ? (double)g[u].y : 0.0;
                     }
                 }
             } else {
-                for (int gq = 0; gq < ngroup; ++gq) {
-                    dv += dp[gq * kk + q];
-                    fv += fp[gq * kk + q];
+                // four groups' loads in flight (clamped and masked as above)
+                for (int g0 = 0; g0 < ngroup; g0 += 4) {
+                    double a[4], c[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int64_t o = (int64_t)min(g0 + u, ngroup - 1) * kk + q;
+                        a[u] = dp[o];
+                        c[u] = fp[o];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const bool in = g0 + u < ngroup;
+                        dv += in ? a[u] : 0.0;
+                        fv += in ? c[u] : 0.0;
+                    }
                 }
             }
             dv *= 2.0;
@@ -330,12 +344,7 @@ __global__ void __launch_bounds__(1024) k_reduce(const WinDesc* __restrict__ win
             Dw[q] = dv;
             Fw[q] = fv;
             double qv = Qw[q];
-            if (j != i) {
-                const int64_t qt = (int64_t)j * k + i;
-                Dw[qt] = dv;
-                Fw[qt] = fv;
-                qv *= 2.0;
-            }
+            if (j != i) qv *= 2.0;
             sn += dv * qv;
             se += fv * qv;
         }
@@ -345,6 +354,23 @@ __global__ void __launch_bounds__(1024) k_reduce(const WinDesc* __restrict__ win
         if (NB) NB[w] = tn;
         if (EB) EB[w] = te;
     }
+}
+
+// Lower triangle of D_B (and F_B if given) from the upper one, per window.
+__global__ void __launch_bounds__(256) k_mirror(const WinDesc* __restrict__ wins, double* __restrict__ D,
+                                                double* __restrict__ F) {
+    const WinDesc d = wins[blockIdx.x];
+    const int k = d.k;
+    double* Dw = D + d.kk_off;
+    double* Fw = F ? F + d.kk_off : nullptr;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5, nty = blockDim.x >> 5;
+    // column segments of the lower triangle: rows j > i read from row i (coalesced
+    // writes along each lower row)
+    for (int j = ty; j < k; j += nty)
+        for (int i = tx; i < j; i += 32) {
+            Dw[(int64_t)j * k + i] = Dw[(int64_t)i * k + j];
+            if (Fw) Fw[(int64_t)j * k + i] = Fw[(int64_t)i * k + j];
+        }
 }
 
 // n(g) = x_g^T D_B x_g on the owned points (src/lrft_cpp.cpp:183-191 in D form).
@@ -472,10 +498,11 @@ __global__ void __launch_bounds__(64 * kDensWaves) k_density(const WinDesc* __re
 
 // n(g) = x_g^T D_B x_g when every owned row of X is made of whole atom blocks: a
 // multiple of 6 entries, each 6 in consecutive T-local columns (the 6 functions of
-// one atom share its support), at most kMaxAtomBlocks blocks.  One lane per owned
-// point sums x_A^T D[A, B] x_B over the atom blocks A <= B of its row (pairs
-// A < B doubled: D_B is symmetric), reading the 6 x 6 blocks of D_B from LDS, so
-// only the entries of D_B the point touches are read; each lane adds in a fixed
+// one atom share its support), blocks in ascending column order, at most
+// kMaxAtomBlocks blocks.  One lane per owned point sums x_A^T D[A, B] x_B over the
+// atom blocks A <= B of its row (pairs A < B doubled: D_B is symmetric), reading
+// the 6 x 6 blocks of D_B from LDS, so only the entries of D_B the point touches
+// are read, and all of them on or above the diagonal; each lane adds in a fixed
 // order (bitwise reproducible).
 constexpr int kDensBlkThreads = 512;
 
@@ -519,7 +546,7 @@ __global__ void __launch_bounds__(kDensBlkThreads) k_density_blk(const WinDesc* 
             for (int i = 0; i < 6; ++i) {
                 double y = 0.0;
 #pragma unroll
-                for (int j = 0; j < 6; ++j) y = fma(Dr(sa + i, sa + j), xa[j], y);
+                for (int j = 0; j < 6; ++j) y = fma(Dr(sa + (i < j ? i : j), sa + (i < j ? j : i)), xa[j], y);
                 t = fma(xa[i], y, t);
             }
             acc += t;
@@ -910,6 +937,11 @@ hipError_t launch_reduce(const WinDesc* wins, int nwin, int ngroup, const double
     return hipGetLastError();
 }
 
+hipError_t launch_mirror(const WinDesc* wins, int nwin, double* D, double* F, hipStream_t st) {
+    hipLaunchKernelGGL(k_mirror, dim3(nwin), dim3(256), 0, st, wins, D, F);
+    return hipGetLastError();
+}
+
 hipError_t launch_density(const WinDesc* wins, int nwin, int kmax, const double* D, const int64_t* own_g,
                           const int64_t* xptr, const int32_t* xidx, const double* xval, double* dens, int blocks6,
                           hipStream_t st) {
@@ -925,6 +957,7 @@ hipError_t launch_density(const WinDesc* wins, int nwin, int kmax, const double*
                                xidx, xval, dens);
         return hipGetLastError();
     }
+    hipLaunchKernelGGL(k_mirror, dim3(nwin), dim3(256), 0, st, wins, const_cast<double*>(D), nullptr);
     if (stage)
         hipLaunchKernelGGL(k_density<true>, dim3(nwin), dim3(64 * kDensWaves), lds, st, wins, D, own_g, xptr, xidx,
                            xval, dens);

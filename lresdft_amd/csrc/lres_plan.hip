@@ -769,7 +769,8 @@ int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, 
         P->d_EB.alloc(nwin);
         P->d_sums.alloc(2);
         // whole atom blocks in every owned row: 6 consecutive T-local columns per
-        // block, at most kMaxAtomBlocks blocks (the density kernel's lane-per-point form)
+        // block, blocks ascending, at most kMaxAtomBlocks blocks (the density
+        // kernel's lane-per-point form)
         P->dens_blocks6 = 1;
         for (size_t o = 0; o + 1 < xptr.size() && P->dens_blocks6; ++o) {
             const int64_t r0 = xptr[o], nnz = xptr[o + 1] - r0;
@@ -777,9 +778,11 @@ int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, 
                 P->dens_blocks6 = 0;
                 break;
             }
-            for (int64_t q = r0; q < r0 + nnz; q += 6)
+            for (int64_t q = r0; q < r0 + nnz; q += 6) {
                 for (int i = 1; i < 6; ++i)
                     if (xidx[q + i] != xidx[q] + i) P->dens_blocks6 = 0;
+                if (q > r0 && xidx[q] < xidx[q - 6] + 6) P->dens_blocks6 = 0;   // ascending blocks
+            }
         }
         P->d_own_g.upload(own_g.data(), own_g.size(), st);
         P->d_xptr.upload(xptr.data(), xptr.size(), st);
@@ -1281,6 +1284,7 @@ int lres_pass_blocks(const lres_plan* P, double* d_D, double* d_F) {
     return guarded([&] {
         require(P != nullptr, "null plan");
         const size_t nkk = P->d_Q.n;
+        hip_check(launch_mirror(P->d_wins.p, (int)P->nwin, P->d_D.p, P->d_F.p, P->st), "mirror kernel");
         if (d_D) hip_check(hipMemcpyAsync(d_D, P->d_D.p, nkk * sizeof(double), hipMemcpyDeviceToDevice, P->st), "copy");
         if (d_F) hip_check(hipMemcpyAsync(d_F, P->d_F.p, nkk * sizeof(double), hipMemcpyDeviceToDevice, P->st), "copy");
         hip_check(hipStreamSynchronize(P->st), "sync");
