@@ -300,10 +300,20 @@ __global__ void __launch_bounds__(1024) k_reduce(const WinDesc* __restrict__ win
     double* Dw = D + d.kk_off;
     double* Fw = F + d.kk_off;
     const double* Qw = Q + d.kk_off;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5, nty = blockDim.x >> 5;
     double sn = 0.0, se = 0.0;
-    for (int i = ty; i < k; i += nty)
-        for (int j = i + tx; j < k; j += 32) {
+    // the upper-triangle entries in row order, dealt to the threads in turn (equal
+    // work per thread, consecutive threads on consecutive columns): entry e is
+    // (i, j) with i the last row whose start s(i) = i k - i (i - 1) / 2 is <= e
+    const int nup = k * (k + 1) / 2;
+    auto row_start = [k](int r) { return r * k - (r * (r - 1)) / 2; };
+    for (int e = threadIdx.x; e < nup; e += blockDim.x) {
+        const float b2 = 2.0f * (float)k + 1.0f;
+        int i = (int)(0.5f * (b2 - sqrtf(fmaxf(b2 * b2 - 8.0f * (float)e, 0.0f))));
+        i = min(max(i, 0), k - 1);
+        while (i > 0 && row_start(i) > e) --i;
+        while (i + 1 < k && row_start(i + 1) <= e) ++i;
+        const int j = i + (e - row_start(i));
+        {
             const int64_t q = (int64_t)i * k + j;
             double dv = 0.0, fv = 0.0;
             if (poles) {
@@ -348,6 +358,7 @@ __global__ void __launch_bounds__(1024) k_reduce(const WinDesc* __restrict__ win
             sn += dv * qv;
             se += fv * qv;
         }
+    }
     const double tn = block_sum(sn, red);
     const double te = block_sum(se, red);
     if (threadIdx.x == 0) {
