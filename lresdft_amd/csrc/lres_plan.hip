@@ -378,7 +378,7 @@ bool inertia_on_mfma(int nmax) {
 }
 
 void inertia(lres_plan* P, const double* sig, int ns, int64_t* h_total, int32_t* d_counts,
-             const std::vector<int32_t>* cand = nullptr) {
+             const std::vector<int32_t>* cand = nullptr, int32_t* h_counts = nullptr) {
     require(P->have_H, "no Hamiltonian loaded");
     require(ns > 0 && ns <= 1024, "nshift must be in 1..1024");
     P->d_sigma.upload(sig, ns, P->st);
@@ -417,6 +417,11 @@ void inertia(lres_plan* P, const double* sig, int ns, int64_t* h_total, int32_t*
     std::vector<unsigned long long> tot(ns);
     hip_check(hipMemcpyAsync(tot.data(), P->d_itot.p, ns * sizeof(unsigned long long), hipMemcpyDeviceToHost, P->st),
               "copy");
+    // per-window counts (counts[w * ns + s]) with the totals, under one synchronisation
+    if (h_counts && d_counts)
+        hip_check(hipMemcpyAsync(h_counts, d_counts, (size_t)P->nwin * ns * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                 P->st),
+                  "copy");
     hip_check(hipStreamSynchronize(P->st), "sync");
     for (int s = 0; s < ns; ++s) h_total[s] = (int64_t)tot[s];
 }
@@ -1141,29 +1146,33 @@ int lres_lambda_min(lres_plan* P, double tol, double* out) {
             step *= 2.0;
             lo -= step;
         }
-        // multisection with 15 interior shifts per launch.  Every round keeps only the
-        // windows that count an eigenvalue below its new upper bracket end: only they
-        // can hold the minimum, so the set shrinks as the bracket narrows.
-        const int K = 15;
-        std::vector<double> sg(K);
-        std::vector<int64_t> tot(K);
+        // multisection: K interior shifts per launch, one wave per (window, shift).
+        // Every round keeps only the windows that count an eigenvalue below its new
+        // upper bracket end: only they can hold the minimum, so the set shrinks as the
+        // bracket narrows, and K grows as it does (about kWaves waves per launch, at
+        // most kMaxK shifts: log2(K + 1) bits of the bracket per round).
+        constexpr int kWaves = 8192, kMaxK = 255;
+        std::vector<double> sg(kMaxK);
+        std::vector<int64_t> tot(kMaxK);
         std::vector<int32_t> cand, next;
-        std::vector<int32_t> cnt_w((size_t)nwin * K);
-        if ((int64_t)P->d_icnt.n < (int64_t)nwin * K) P->d_icnt.alloc((size_t)nwin * K);
+        std::vector<int32_t> cnt_w((size_t)nwin * kMaxK);
+        if ((int64_t)P->d_icnt.n < (int64_t)nwin * kMaxK) P->d_icnt.alloc((size_t)nwin * kMaxK);
         bool all = true;
         while (hi - lo > tol) {
+            const int ncand = all ? nwin : (int)cand.size();
+            const int K = std::max(15, std::min(kMaxK, kWaves / std::max(1, ncand)));
             for (int s = 0; s < K; ++s) sg[s] = lo + (hi - lo) * (s + 1) / (K + 1);
-            inertia(P, sg.data(), K, tot.data(), P->d_icnt.p, all ? nullptr : &cand);
+            // the per-window counts come back only while the set is large enough for
+            // shrinking it to pay (small sets run at up to kMaxK shifts anyway)
+            const bool shrink = ncand > 64;
+            inertia(P, sg.data(), K, tot.data(), P->d_icnt.p, all ? nullptr : &cand, shrink ? cnt_w.data() : nullptr);
             double nlo = lo, nhi = hi;
             int first_pos = -1;
             for (int s = 0; s < K; ++s) {
                 if (tot[s] == 0) nlo = sg[s];
                 else { nhi = sg[s]; first_pos = s; break; }
             }
-            if (first_pos >= 0) {
-                hip_check(hipMemcpyAsync(cnt_w.data(), P->d_icnt.p, cnt_w.size() * sizeof(int32_t),
-                                         hipMemcpyDeviceToHost, P->st), "copy");
-                hip_check(hipStreamSynchronize(P->st), "sync");
+            if (first_pos >= 0 && shrink) {
                 next.clear();
                 if (all) {
                     for (int w = 0; w < nwin; ++w)
