@@ -29,8 +29,7 @@ hipError_t launch_reduce(const lres::WinDesc* wins, int nwin, int ngroup, const 
 // lower triangles of D_B (and F_B if F != NULL) from the upper ones the reduction writes
 hipError_t launch_mirror(const lres::WinDesc* wins, int nwin, double* D, double* F, hipStream_t st);
 // blocks6: every owned row of X is whole atom blocks (6 consecutive T-local columns
-// each, at most kMaxAtomBlocks blocks), checked on the host; then one lane per point
-constexpr int kMaxAtomBlocks = 8;
+// each, ascending), checked on the host; then one lane per point
 hipError_t launch_density(const lres::WinDesc* wins, int nwin, int kmax, const double* D, const int64_t* own_g,
                           const int64_t* xptr, const int32_t* xidx, const double* xval, double* dens, int blocks6,
                           hipStream_t st);

@@ -509,12 +509,12 @@ __global__ void __launch_bounds__(64 * kDensWaves) k_density(const WinDesc* __re
 
 // n(g) = x_g^T D_B x_g when every owned row of X is made of whole atom blocks: a
 // multiple of 6 entries, each 6 in consecutive T-local columns (the 6 functions of
-// one atom share its support), blocks in ascending column order, at most
-// kMaxAtomBlocks blocks.  One lane per owned point sums x_A^T D[A, B] x_B over the
-// atom blocks A <= B of its row (pairs A < B doubled: D_B is symmetric), reading
-// the 6 x 6 blocks of D_B from LDS, so only the entries of D_B the point touches
-// are read, and all of them on or above the diagonal; each lane adds in a fixed
-// order (bitwise reproducible).
+// one atom share its support), blocks in ascending column order.  One lane per
+// owned point sums x_A^T D[A, B] x_B over the atom blocks A <= B of its row (pairs
+// A < B doubled: D_B is symmetric), reading the 6 x 6 blocks of D_B from LDS (or
+// global memory when D_B does not fit), so only the entries of D_B the point
+// touches are read, all of them on or above the diagonal; each lane adds in a
+// fixed order (bitwise reproducible).
 constexpr int kDensBlkThreads = 512;
 
 template <bool STAGE>
@@ -545,9 +545,8 @@ __global__ void __launch_bounds__(kDensBlkThreads) k_density_blk(const WinDesc* 
         const int64_t r0 = xptr[o];
         const int nb = (int)((xptr[o + 1] - r0) / 6);
         double acc = 0.0;
-#pragma unroll
-        for (int A = 0; A < kMaxAtomBlocks; ++A) {
-            if (A >= nb) break;
+#pragma unroll 1
+        for (int A = 0; A < nb; ++A) {
             const int sa = xidx[r0 + 6 * A];
             double xa[6];
 #pragma unroll
@@ -561,9 +560,8 @@ __global__ void __launch_bounds__(kDensBlkThreads) k_density_blk(const WinDesc* 
                 t = fma(xa[i], y, t);
             }
             acc += t;
-#pragma unroll
-            for (int B = A + 1; B < kMaxAtomBlocks; ++B) {
-                if (B >= nb) break;
+#pragma unroll 1
+            for (int B = A + 1; B < nb; ++B) {
                 const int sb = xidx[r0 + 6 * B];
                 double xb[6];
 #pragma unroll

@@ -774,12 +774,11 @@ int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, 
         P->d_EB.alloc(nwin);
         P->d_sums.alloc(2);
         // whole atom blocks in every owned row: 6 consecutive T-local columns per
-        // block, blocks ascending, at most kMaxAtomBlocks blocks (the density
-        // kernel's lane-per-point form)
+        // block, blocks ascending (the density kernel's lane-per-point form)
         P->dens_blocks6 = 1;
         for (size_t o = 0; o + 1 < xptr.size() && P->dens_blocks6; ++o) {
             const int64_t r0 = xptr[o], nnz = xptr[o + 1] - r0;
-            if (nnz % 6 != 0 || nnz > 6 * kMaxAtomBlocks) {
+            if (nnz % 6 != 0) {
                 P->dens_blocks6 = 0;
                 break;
             }
@@ -1150,8 +1149,13 @@ int lres_lambda_min(lres_plan* P, double tol, double* out) {
         // Every round keeps only the windows that count an eigenvalue below its new
         // upper bracket end: only they can hold the minimum, so the set shrinks as the
         // bracket narrows, and K grows as it does (about kWaves waves per launch, at
-        // most kMaxK shifts: log2(K + 1) bits of the bracket per round).
+        // most kMaxK shifts: log2(K + 1) bits of the bracket per round) on the
+        // one-wave-per-shift kernels.
         constexpr int kWaves = 8192, kMaxK = 255;
+        // the blocked inertia kernel of windows above 128 functions factors its shifts
+        // one after another within a few blocks per window: 15 shifts per launch there
+        const bool blocked = std::any_of(P->buckets.begin(), P->buckets.end(),
+                                         [](const lres_plan::Bucket& b) { return b.big; });
         std::vector<double> sg(kMaxK);
         std::vector<int64_t> tot(kMaxK);
         std::vector<int32_t> cand, next;
@@ -1160,11 +1164,12 @@ int lres_lambda_min(lres_plan* P, double tol, double* out) {
         bool all = true;
         while (hi - lo > tol) {
             const int ncand = all ? nwin : (int)cand.size();
-            const int K = std::max(15, std::min(kMaxK, kWaves / std::max(1, ncand)));
+            const int K = blocked ? 15 : std::max(15, std::min(kMaxK, kWaves / std::max(1, ncand)));
             for (int s = 0; s < K; ++s) sg[s] = lo + (hi - lo) * (s + 1) / (K + 1);
-            // the per-window counts come back only while the set is large enough for
-            // shrinking it to pay (small sets run at up to kMaxK shifts anyway)
-            const bool shrink = ncand > 64;
+            // the per-window counts come back while the set is large enough for
+            // shrinking it to pay (small sets run at up to kMaxK shifts anyway), and
+            // always for blocked windows (each costs a factorisation per shift)
+            const bool shrink = blocked || ncand > 64;
             inertia(P, sg.data(), K, tot.data(), P->d_icnt.p, all ? nullptr : &cand, shrink ? cnt_w.data() : nullptr);
             double nlo = lo, nhi = hi;
             int first_pos = -1;
