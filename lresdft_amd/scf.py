@@ -34,6 +34,9 @@ class GridOps:
         # 4 pi / G^2 on the half spectrum of a real-to-complex transform
         self.w_half = (4 * math.pi * self.inv_G2[:, :, : cr.shape[2] // 2 + 1]).contiguous()
         self.part = None
+        # the FFT library builds (and on the GPU compiles) its transform plans on first
+        # use: do that here, at set-up, rather than inside the first SCF step
+        self.hartree(torch.zeros(cr.shape, dtype=torch.float64, device=device))
 
     def ion_potential(self, chunk=256):
         """Periodic sum of -Z erf(r/rc)/r, G = 0 dropped (three_d.py:102-119).
