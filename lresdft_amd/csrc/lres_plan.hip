@@ -208,9 +208,15 @@ struct lres_plan {
     // cumulative pole-kernel statistics per precision: passes and milliseconds (HIP events)
     int64_t stat_passes[2] = {0, 0};
     double stat_ms[2] = {0.0, 0.0};
+    // pinned host staging: a pass's sums come back in slots 0, 1 (an asynchronous
+    // copy, then one wait); the block from kPinSlots stages inertia shifts on the way
+    // in (every call that fills it waits for its stream before returning)
+    static constexpr int kPinSlots = 2 + 1024;
+    double* h_pin = nullptr;
     ~lres_plan() {
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
+        if (h_pin) (void)hipHostFree(h_pin);
     }
 };
 
@@ -352,8 +358,10 @@ void run_pass(lres_plan* P, double mu, double e_lo, double* d_dens, double* d_NB
                   "density kernel");
     hip_check(launch_sum2(NB, EB, (int)P->nwin, P->d_sums.p, P->st), "sum kernel");
     if (h_sums) {
-        hip_check(hipMemcpyAsync(h_sums, P->d_sums.p, 2 * sizeof(double), hipMemcpyDeviceToHost, P->st), "copy");
+        hip_check(hipMemcpyAsync(P->h_pin, P->d_sums.p, 2 * sizeof(double), hipMemcpyDeviceToHost, P->st), "copy");
         hip_check(hipStreamSynchronize(P->st), "sync");
+        h_sums[0] = P->h_pin[0];
+        h_sums[1] = P->h_pin[1];
         float t0 = 0, t1 = 0;
         if (hipEventElapsedTime(&t0, P->ev[0], P->ev[1]) == hipSuccess) P->timing[0] = t0;
         if (hipEventElapsedTime(&t1, P->ev[2], P->ev[3]) == hipSuccess) {
@@ -377,11 +385,17 @@ bool inertia_on_mfma(int nmax) {
     return nmax > 16 && nmax <= kMaxRegisterN;
 }
 
-void inertia(lres_plan* P, const double* sig, int ns, int64_t* h_total, int32_t* d_counts,
-             const std::vector<int32_t>* cand = nullptr, int32_t* h_counts = nullptr) {
+// queue the inertia counts at ns shifts (totals into d_itot, per window into d_counts
+// if given) without waiting for them
+void inertia_launch(lres_plan* P, const double* sig, int ns, int32_t* d_counts,
+                    const std::vector<int32_t>* cand = nullptr) {
     require(P->have_H, "no Hamiltonian loaded");
     require(ns > 0 && ns <= 1024, "nshift must be in 1..1024");
-    P->d_sigma.upload(sig, ns, P->st);
+    require(ns <= lres_plan::kPinSlots, "too many shifts");
+    if ((int)P->d_sigma.n < ns) P->d_sigma.alloc(ns);
+    double* stage = P->h_pin + lres_plan::kPinSlots;
+    std::memcpy(stage, sig, ns * sizeof(double));
+    hip_check(hipMemcpyAsync(P->d_sigma.p, stage, ns * sizeof(double), hipMemcpyHostToDevice, P->st), "upload");
     if ((int)P->d_itot.n < ns) P->d_itot.alloc(ns);
     hip_check(hipMemsetAsync(P->d_itot.p, 0, ns * sizeof(unsigned long long), P->st), "memset");
     std::vector<int32_t> sub;        // candidate ids grouped by bucket
@@ -414,6 +428,11 @@ void inertia(lres_plan* P, const double* sig, int ns, int64_t* h_total, int32_t*
                                      P->d_sigma.p, ns, d_counts, P->d_itot.p, P->st),
                       "inertia kernel");
     }
+}
+
+void inertia(lres_plan* P, const double* sig, int ns, int64_t* h_total, int32_t* d_counts,
+             const std::vector<int32_t>* cand = nullptr, int32_t* h_counts = nullptr) {
+    inertia_launch(P, sig, ns, d_counts, cand);
     std::vector<unsigned long long> tot(ns);
     hip_check(hipMemcpyAsync(tot.data(), P->d_itot.p, ns * sizeof(unsigned long long), hipMemcpyDeviceToHost, P->st),
               "copy");
@@ -629,6 +648,8 @@ int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, 
         P->precision = precision;
         P->st = (hipStream_t)stream;
         for (auto& e : P->ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+        hip_check(hipHostMalloc((void**)&P->h_pin, 2 * lres_plan::kPinSlots * sizeof(double), hipHostMallocDefault),
+                  "hipHostMalloc");
         P->wins.resize(nwin);
         P->n_of.resize(nwin);
         P->k_of.resize(nwin);
