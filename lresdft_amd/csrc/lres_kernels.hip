@@ -276,11 +276,12 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 
 // D_B = 2 sum over pole groups of the partials, F_B likewise, N_B = Tr(D_B Q_B),
 // E_B = Tr(F_B Q_B).  The pole kernels write the upper triangle of each partial
-// (G[T,T] is complex symmetric); this walks the upper triangle in row segments of
-// 32 (coalesced reads and writes), writes the upper triangle of D_B and F_B
-// (k_mirror fills the lower one for the consumers that read it) and forms the
-// traces with the off-diagonal entries doubled (Q_B is symmetric).  Windows with by_pole[w] != 0 (complex64 matrix-core passes) hold
-// one float pair (Re w G, Re w z G) per pole instead, nslot poles in order.
+// (G[T,T] is complex symmetric); this deals the upper-triangle entries to the
+// threads in row order (coalesced reads and writes), writes the upper triangles
+// of D_B and F_B (k_mirror fills the lower ones for the consumers that read them)
+// and forms the traces with the off-diagonal entries doubled (Q_B is symmetric).
+// Windows with by_pole[w] != 0 (complex64 matrix-core passes) hold one float pair
+// (Re w G, Re w z G) per pole instead, nslot poles in order.
 __global__ void __launch_bounds__(1024) k_reduce(const WinDesc* __restrict__ wins, int ngroup,
                                                 const double* __restrict__ Dpart, const double* __restrict__ Fpart,
                                                 const float2* __restrict__ slots, int nslot,
@@ -551,15 +552,17 @@ __global__ void __launch_bounds__(kDensBlkThreads) k_density_blk(const WinDesc* 
             double xa[6];
 #pragma unroll
             for (int i = 0; i < 6; ++i) xa[i] = xval[r0 + 6 * A + i];
+            // diagonal block from its upper triangle: x^T D x = 2 sum_i x_i (D_ii x_i / 2
+            // + sum_{j > i} D_ij x_j)
             double t = 0.0;
 #pragma unroll
             for (int i = 0; i < 6; ++i) {
-                double y = 0.0;
+                double y = 0.5 * Dr(sa + i, sa + i) * xa[i];
 #pragma unroll
-                for (int j = 0; j < 6; ++j) y = fma(Dr(sa + (i < j ? i : j), sa + (i < j ? j : i)), xa[j], y);
+                for (int j = i + 1; j < 6; ++j) y = fma(Dr(sa + i, sa + j), xa[j], y);
                 t = fma(xa[i], y, t);
             }
-            acc += t;
+            acc = fma(2.0, t, acc);
 #pragma unroll 1
             for (int B = A + 1; B < nb; ++B) {
                 const int sb = xidx[r0 + 6 * B];
