@@ -7,6 +7,8 @@ Bars:
   (complex64 arithmetic over a window of 84 functions);
   every waves-per-matrix variant gives bitwise the same pass (the autotuner's
   choice never changes results);
+  M = 20 (10 poles: partial load batches in the reduction) vs the oracle in both
+  precisions;
   uneven windows (random centres, n_W 36..96 in one plan, six buckets) vs the
   oracle at 1e-12 in fp64 (at most one bucket may fall back to the vector-ALU
   kernel when every MFMA variant of its size spills) and at the fp32 bar in
@@ -91,6 +93,25 @@ def test_mfma_fp32_error_is_fp32_level(sys3):
     assert np.max(np.abs(d32 - dp)) < 2e-5
     assert abs(n32 - NBo.sum()) < 1e-5 * cr.n_elec
     assert np.max(np.abs(d32 - dp)) > 0.0      # really the complex64 path
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pole_count_not_a_multiple_of_the_load_batch(sys3, precision):
+    """M = 20: 10 poles, so the reduction's batches of 8 per-pole slots (complex64)
+    and of 4 pole-group partials (fp64, 10 groups at 27 windows) end in a partial
+    batch; fp64 vs the oracle at 1e-12, complex64 at the fp32 bar."""
+    cr, b, v = sys3
+    print(f"[gpu] M=20 pass {precision} vs oracle, n=3 R_w=8", flush=True)
+    cells = cell_centres((3, 3, 3), 6.0)
+    H = b.H(v)
+    W, own = build_windows(b, cells, 8.0)
+    e_lo, mu = _mu_window(H, b.S, W)
+    dp, NBo, EBo = pole_window_pass(H, b.S, b.Xr, W, own, cr.dV, mu, e_lo, 20, True)
+    d, NB, EB, ns, es, _ = _pass(b, cells, H, mu, e_lo, M=20, precision=precision)
+    tol = 1e-12 if precision == "fp64" else 2e-5
+    assert np.max(np.abs(d - dp)) < tol
+    assert np.max(np.abs(NB - NBo)) < tol and np.max(np.abs(EB - EBo)) < (tol if precision == "fp64" else 1e-4)
+    assert abs(ns - NBo.sum()) < (1e-11 if precision == "fp64" else 1e-5 * cr.n_elec)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
