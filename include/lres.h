@@ -195,6 +195,20 @@ int lres_lambda_min(lres_plan* plan, double tol, double* h_lambda_min);
 int lres_mu_solve(lres_plan* plan, double N, double e_lo, double lo, double hi, double f_lo, double f_hi,
                   double xtol, double ftol, double* d_dens, double* h_mu, double* h_info);
 
+/* The warm step's mu search on the device side: from the previous mu0 (count f0
+ * there, f = 2 sum N_B - N), secant steps with the last known slope (slope = 0:
+ * steps of 1e-3 towards the root), each at most 0.25 and with 20% overshoot once
+ * two points are on the same side, until |f| <= ftol or the root is bracketed,
+ * then Brent as lres_mu_solve (xtol, ftol); at most 60 secant steps.  No
+ * counterpart in the reference (its mu search runs on stored eigenvalues); the same
+ * steps as the host-side search it replaces, one call instead of one per pass.
+ * h_info[8]: passes run, f at the returned mu, 1 if the last pass was at the
+ * returned mu (its D_B blocks stay for lres_density_last), sum E_B there, the slope
+ * after the search, 1 if no bracket was found (the returned mu is then NaN),
+ * sum N_B there, 1 if the root came from Brent. */
+int lres_mu_solve_warm(lres_plan* plan, double N, double e_lo, double mu0, double f0, double slope, double ftol,
+                       double xtol, double* h_mu, double* h_info);
+
 /* One-shot form with lrft_windows' argument list (src/lrft_cpp.cpp:129-132):
  * host CSR H, S, X, window tables, nb, dV, mu, e_lo, M; outputs h_dens (ng, may be
  * NULL), h_NB, h_EB (nwin).  Builds and destroys a plan on the null stream. */

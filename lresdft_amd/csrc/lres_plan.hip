@@ -1250,6 +1250,69 @@ int lres_mu_solve(lres_plan* P, double N, double e_lo, double lo, double hi, dou
     });
 }
 
+int lres_mu_solve_warm(lres_plan* P, double N, double e_lo, double mu0, double f0, double slope, double ftol,
+                       double xtol, double* h_mu, double* h_info) {
+    return guarded([&] {
+        require(P != nullptr, "null plan");
+        require(h_mu && h_info, "null output");
+        int evals = 0;
+        double sums[2] = {0.0, 0.0};
+        double last_mu = NAN, last_f = NAN, last_e = NAN;
+        auto f = [&](double mu) {
+            run_pass(P, mu, e_lo, nullptr, nullptr, nullptr, sums);
+            ++evals;
+            last_mu = mu;
+            last_f = 2.0 * sums[0] - N;
+            last_e = sums[1];
+            return last_f;
+        };
+        // slope of the count between two trial points when it is positive and finite
+        auto note = [&](double m0, double g0, double m1, double g1) {
+            if (m1 != m0 && std::isfinite(g1) && (g1 - g0) / (m1 - m0) > 0) slope = (g1 - g0) / (m1 - m0);
+        };
+        double a = mu0, fa = f0;
+        double step = slope != 0.0 ? -fa / slope : (fa < 0 ? 1e-3 : -1e-3);
+        double mu = NAN;
+        bool found = false, by_brent = false;
+        for (int it = 0; it < 60 && !found; ++it) {
+            step = std::min(std::max(step, -0.25), 0.25);
+            if (std::fabs(step) < 1e-12) step = fa < 0 ? 1e-12 : -1e-12;
+            double b = a + step;
+            if (b <= e_lo + 1e-6) b = 0.5 * (a + e_lo + 1e-6);
+            const double fb = f(b);
+            note(a, fa, b, fb);
+            if (std::fabs(fb) <= ftol) {
+                mu = b;
+                found = true;
+                break;
+            }
+            if ((fa < 0) != (fb < 0)) {
+                const bool lo_a = a < b;
+                const double lo = lo_a ? a : b, hi = lo_a ? b : a, flo = lo_a ? fa : fb, fhi = lo_a ? fb : fa;
+                int bevals = 0;
+                mu = brent_root(f, lo, hi, flo, fhi, xtol, ftol, 200, &bevals);
+                found = by_brent = true;
+                break;
+            }
+            const double sl = (fb - fa) / (b - a);
+            // same side of the root: secant extrapolation with 20% overshoot to bracket it
+            step = sl > 0 ? -1.2 * fb / sl : 4.0 * step;
+            a = b;
+            fa = fb;
+        }
+        *h_mu = mu;
+        const bool last = found && mu == last_mu;
+        h_info[0] = evals;
+        h_info[1] = last ? last_f : NAN;
+        h_info[2] = last ? 1.0 : 0.0;
+        h_info[3] = last ? last_e : NAN;
+        h_info[4] = slope;
+        h_info[5] = found ? 0.0 : 1.0;
+        h_info[6] = last ? sums[0] : NAN;
+        h_info[7] = by_brent ? 1.0 : 0.0;
+    });
+}
+
 int lres_windows(const int64_t* Hp, const int64_t* Hi, const double* Hv, const int64_t* Sp, const int64_t* Si,
                  const double* Sv, const int64_t* Xp, const int64_t* Xi, const double* Xv, const int64_t* Wflat,
                  const int64_t* Woff, int64_t nwin, const int64_t* Oflat, const int64_t* Ooff, int64_t nb, int64_t ng,

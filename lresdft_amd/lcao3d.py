@@ -371,31 +371,27 @@ class LRFTLCAOCpp:
 
     def _solve_mu_warm(self, mu0, f0, e_lo, N):
         """Secant steps from the previous mu (count f0 there) with the last known
-        slope dN/dmu until the root is bracketed, then Brent."""
-        ftol = self._ftol(N)
-        a, fa = mu0, f0
-        slope = self._slope
-        step = -fa / slope if slope else (1e-3 if fa < 0 else -1e-3)
-        for _ in range(60):
-            step = float(np.clip(step, -0.25, 0.25))
-            if abs(step) < 1e-12:
-                step = 1e-12 if fa < 0 else -1e-12
-            b = a + step
-            if b <= e_lo + 1e-6:
-                b = 0.5 * (a + e_lo + 1e-6)
-            fb = self._count(b, e_lo, N)
-            self._note_slope(a, fa, b, fb)
-            if abs(fb) <= ftol:
-                return b
-            if (fa < 0) != (fb < 0):
-                lo, hi, flo, fhi = (a, b, fa, fb) if a < b else (b, a, fb, fa)
-                mu, _ = self._brent(N, e_lo, lo, hi, flo, fhi)
-                return mu
-            s = (fb - fa) / (b - a)
-            # same side of the root: secant extrapolation with 20% overshoot to bracket it
-            step = -1.2 * fb / s if s > 0 else 4.0 * step
-            a, fa = b, fb
-        raise StaleMu("no electron-count bracket around the previous mu")
+        slope dN/dmu until the root is bracketed, then Brent (lres_mu_solve_warm:
+        the whole search in one native call, each trial a pass without density)."""
+        mu = np.zeros(1)
+        info = np.zeros(8)
+        nat.check(nat.lib().lres_mu_solve_warm(self.plan.h, float(N), float(e_lo), float(mu0), float(f0),
+                                               float(self._slope or 0.0), float(self._ftol(N)),
+                                               float(self.mu_xtol), nat.pf(mu), nat.pf(info)))
+        self.plan.passes += int(info[0])
+        if info[4] > 0:
+            self._slope = float(info[4])
+        if info[5] == 1.0:
+            self.plan.last = None
+            raise StaleMu("no electron-count bracket around the previous mu")
+        mu = float(mu[0])
+        if info[2] == 1.0:      # the device D_B blocks are those of the returned mu
+            # a Brent root records its count as (f + N) / 2, a secant step its pass sum
+            nsum = 0.5 * (info[1] + N) if info[7] == 1.0 else float(info[6])
+            self.plan.last = (mu, float(e_lo), nsum, float(info[3]), False)
+        else:
+            self.plan.last = None
+        return mu
 
     def refine(self, N=None):
         """fp64 finish of a converged step (on the H of the last call): an fp64
