@@ -552,17 +552,16 @@ __global__ void __launch_bounds__(kDensBlkThreads) k_density_blk(const WinDesc* 
             double xa[6];
 #pragma unroll
             for (int i = 0; i < 6; ++i) xa[i] = xval[r0 + 6 * A + i];
-            // diagonal block from its upper triangle: x^T D x = 2 sum_i x_i (D_ii x_i / 2
-            // + sum_{j > i} D_ij x_j)
+            // diagonal block, every entry read from the upper triangle
             double t = 0.0;
 #pragma unroll
             for (int i = 0; i < 6; ++i) {
-                double y = 0.5 * Dr(sa + i, sa + i) * xa[i];
+                double y = 0.0;
 #pragma unroll
-                for (int j = i + 1; j < 6; ++j) y = fma(Dr(sa + i, sa + j), xa[j], y);
+                for (int j = 0; j < 6; ++j) y = fma(Dr(sa + (i < j ? i : j), sa + (i < j ? j : i)), xa[j], y);
                 t = fma(xa[i], y, t);
             }
-            acc = fma(2.0, t, acc);
+            acc += t;
 #pragma unroll 1
             for (int B = A + 1; B < nb; ++B) {
                 const int sb = xidx[r0 + 6 * B];
