@@ -176,7 +176,10 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
 
     // pivot rows of block p = 4 JPc + pq into buffer b: tile row JPc of the owned
     // columns J >= JPc, and left of it the tile column JPc above the diagonal,
-    // transposed and negated (those rows are pivoted, the pivot rows are not)
+    // transposed.  Left of JPc the row panel is the negated transpose (those rows are
+    // pivoted, the pivot rows are not): it is stored without the sign, which the
+    // consumers fold in (the A operands of the pivoted rows would negate it back; the
+    // B operands of columns J < JP take the opposite signs of their products)
     auto publish = [&](auto jpc, int pq, int b, int li, int lg) __attribute__((always_inline)) {
         constexpr int JPc = decltype(jpc)::value;
         constexpr int jc = JPc / NWC;
@@ -186,8 +189,8 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
             for (int I = 0; I < JPc; ++I)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    rpr[b][16 * I + MF::row(lg, r)][sc] = -are[I][jc][r];
-                    rpi[b][16 * I + MF::row(lg, r)][sc] = -aim[I][jc][r];
+                    rpr[b][16 * I + MF::row(lg, r)][sc] = are[I][jc][r];
+                    rpi[b][16 * I + MF::row(lg, r)][sc] = aim[I][jc][r];
                 }
         }
 #pragma unroll
@@ -279,14 +282,15 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
 #pragma unroll
                 for (int s = 0; s < 4; ++s) prow[s] = {__shfl(a.re, 4 * lg + s, 64), __shfl(a.im, 4 * lg + s, 64)};
                 // A operands: C[16 I + li][lg] = A[16 I + li][4p + lg] - d, read from the
-                // row panel with the sign flipped on the pivoted rows (16 I + li < 4p);
-                // the identity shift only in tile row JP
+                // row panel with the sign flipped on the pivoted rows (16 I + li < 4p; the
+                // tiles I < JP are stored without the panel's sign, so only the pivoted
+                // rows of tile row JP flip here); the identity shift only in tile row JP
                 R car[NT], cai[NT];
 #pragma unroll
                 for (int I = 0; I < NT; ++I) {
                     R xr = rpr[b][16 * I + li][lg];
                     R xi = rpi[b][16 * I + li][lg];
-                    if (I < JP || (I == JP && li < 4 * pp)) {
+                    if (I == JP && li < 4 * pp) {
                         xr = -xr;
                         xi = -xi;
                     }
@@ -319,9 +323,12 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
                         xr = fma(prow[s].re, tr[s], fma(-prow[s].im, ti[s], xr));
                         xi = fma(prow[s].re, ti[s], fma(prow[s].im, tr[s], xi));
                     }
-                    nre[j] = -xr;
-                    rim[j] = xi;
-                    nim[j] = -xi;
+                    // columns left of the pivot tile read the panel without its sign:
+                    // their products come out negated (exactly), so the signs swap
+                    const bool left = wv + NWC * j < JP;
+                    nre[j] = left ? xr : -xr;
+                    rim[j] = left ? -xi : xi;
+                    nim[j] = left ? xi : -xi;
                 }
                 // A <- A - C Rt:  Re -= Cr Rr - Ci Ri,  Im -= Cr Ri + Ci Rr
 #pragma unroll
