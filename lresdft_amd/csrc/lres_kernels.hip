@@ -532,7 +532,17 @@ __global__ void __launch_bounds__(kDensBlkThreads) k_density_blk(const WinDesc* 
     const int k = d.k;
     const double* Dw = D + d.kk_off;
     if constexpr (STAGE) {
-        for (int e = threadIdx.x; e < k * k; e += blockDim.x) Dl[e] = Dw[e];
+        // eight loads in flight per thread (clamped index, masked store): a plain
+        // copy loop waits for each load before the next
+        const int kk = k * k, nt = blockDim.x;
+        for (int e0 = threadIdx.x; e0 < kk; e0 += 8 * nt) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = Dw[min(e0 + u * nt, kk - 1)];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (e0 + u * nt < kk) Dl[e0 + u * nt] = v[u];
+        }
         __syncthreads();
     }
     auto Dr = [&](int a, int b) __attribute__((always_inline)) {
@@ -546,37 +556,60 @@ __global__ void __launch_bounds__(kDensBlkThreads) k_density_blk(const WinDesc* 
         const int64_t r0 = xptr[o];
         const int nb = (int)((xptr[o + 1] - r0) / 6);
         double acc = 0.0;
-#pragma unroll 1
-        for (int A = 0; A < nb; ++A) {
-            const int sa = xidx[r0 + 6 * A];
+        if (nb > 0) {
+            // each block's column index and values are loaded one step ahead of
+            // their use (the next block's index clamped to the last block), so a
+            // load is in flight while the previous block's terms are summed
+            const int32_t* xi = xidx + r0;
+            const double* xv = xval + r0;
+            int sa = xi[0];
             double xa[6];
 #pragma unroll
-            for (int i = 0; i < 6; ++i) xa[i] = xval[r0 + 6 * A + i];
-            // diagonal block, every entry read from the upper triangle
-            double t = 0.0;
-#pragma unroll
-            for (int i = 0; i < 6; ++i) {
-                double y = 0.0;
-#pragma unroll
-                for (int j = 0; j < 6; ++j) y = fma(Dr(sa + (i < j ? i : j), sa + (i < j ? j : i)), xa[j], y);
-                t = fma(xa[i], y, t);
-            }
-            acc += t;
+            for (int i = 0; i < 6; ++i) xa[i] = xv[i];
 #pragma unroll 1
-            for (int B = A + 1; B < nb; ++B) {
-                const int sb = xidx[r0 + 6 * B];
-                double xb[6];
+            for (int A = 0; A < nb; ++A) {
+                const int An = min(A + 1, nb - 1);
+                const int sn = xi[6 * An];
+                double xn[6];
 #pragma unroll
-                for (int j = 0; j < 6; ++j) xb[j] = xval[r0 + 6 * B + j];
-                double u = 0.0;
+                for (int i = 0; i < 6; ++i) xn[i] = xv[6 * An + i];
+                // diagonal block, every entry read from the upper triangle
+                double t = 0.0;
 #pragma unroll
                 for (int i = 0; i < 6; ++i) {
                     double y = 0.0;
 #pragma unroll
-                    for (int j = 0; j < 6; ++j) y = fma(Dr(sa + i, sb + j), xb[j], y);
-                    u = fma(xa[i], y, u);
+                    for (int j = 0; j < 6; ++j) y = fma(Dr(sa + (i < j ? i : j), sa + (i < j ? j : i)), xa[j], y);
+                    t = fma(xa[i], y, t);
                 }
-                acc = fma(2.0, u, acc);
+                acc += t;
+                int sb = sn;
+                double xb[6];
+#pragma unroll
+                for (int j = 0; j < 6; ++j) xb[j] = xn[j];
+#pragma unroll 1
+                for (int B = A + 1; B < nb; ++B) {
+                    const int Bn = min(B + 1, nb - 1);
+                    const int sbn = xi[6 * Bn];
+                    double xbn[6];
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) xbn[j] = xv[6 * Bn + j];
+                    double u = 0.0;
+#pragma unroll
+                    for (int i = 0; i < 6; ++i) {
+                        double y = 0.0;
+#pragma unroll
+                        for (int j = 0; j < 6; ++j) y = fma(Dr(sa + i, sb + j), xb[j], y);
+                        u = fma(xa[i], y, u);
+                    }
+                    acc = fma(2.0, u, acc);
+                    sb = sbn;
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) xb[j] = xbn[j];
+                }
+                sa = sn;
+#pragma unroll
+                for (int i = 0; i < 6; ++i) xa[i] = xn[i];
             }
         }
         dens[own_g[o]] = acc;
