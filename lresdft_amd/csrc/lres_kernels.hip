@@ -1128,16 +1128,32 @@ __global__ void __launch_bounds__(64) k_pair_v(const int32_t* __restrict__ pairA
     for (int i = 0; i < NF; ++i)
 #pragma unroll
         for (int j = 0; j < NF; ++j) acc[i][j] = 0.0;
-    for (int64_t e = pent_off[p] + threadIdx.x; e < pent_off[p + 1]; e += 64) {
-        const double vr = v[ent_r[e]];
-        const double* xa = XA + (int64_t)ent_a[e] * nfa;
-        const double* xb = XB + (int64_t)ent_b[e] * nfb;
-        double bv[NF];
+    // Every load is issued unconditionally (function index clamped to the atom's
+    // last, the value masked afterwards: a guarded load compiles to a branch and a
+    // wait of its own), and the next entry's indices are loaded one step ahead.
+    const int64_t e_end = pent_off[p + 1];
+    int64_t e = pent_off[p] + threadIdx.x;
+    const int64_t e0 = min(e, e_end - 1);
+    int r_n = e_end > 0 ? ent_r[e0] : 0, a_n = e_end > 0 ? ent_a[e0] : 0, b_n = e_end > 0 ? ent_b[e0] : 0;
+    for (; e < e_end; e += 64) {
+        const int r = r_n, ia = a_n, ib = b_n;
+        const int64_t en = min(e + 64, e_end - 1);
+        r_n = ent_r[en];
+        a_n = ent_a[en];
+        b_n = ent_b[en];
+        const double vr = v[r];
+        const double* xa = XA + (int64_t)ia * nfa;
+        const double* xb = XB + (int64_t)ib * nfb;
+        double bv[NF], xv[NF];
 #pragma unroll
-        for (int j = 0; j < NF; ++j) bv[j] = j < nfb ? xb[j] : 0.0;
+        for (int j = 0; j < NF; ++j) bv[j] = xb[min(j, nfb - 1)];   // a paired atom has nf >= 1
+#pragma unroll
+        for (int i = 0; i < NF; ++i) xv[i] = xa[min(i, nfa - 1)];
+#pragma unroll
+        for (int j = 0; j < NF; ++j) bv[j] = j < nfb ? bv[j] : 0.0;
 #pragma unroll
         for (int i = 0; i < NF; ++i) {
-            const double av = i < nfa ? xa[i] * vr : 0.0;
+            const double av = i < nfa ? xv[i] * vr : 0.0;
 #pragma unroll
             for (int j = 0; j < NF; ++j) acc[i][j] = fma(av, bv[j], acc[i][j]);
         }
