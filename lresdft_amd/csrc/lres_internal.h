@@ -57,10 +57,9 @@ hipError_t launch_pair_v(int nf_max, int npair, const int32_t* pairA, const int3
                          const int32_t* ent_r, const int32_t* ent_a, const int32_t* ent_b, const int64_t* xat_off,
                          const int32_t* nf_of, const double* xat, const int64_t* vp_off, const double* v, double dV,
                          double* vpair, hipStream_t st);
-hipError_t launch_window_hv(const lres::WinDesc* wins, int nwin, const int64_t* Woff, const int32_t* lat,
-                            const int32_t* lf, const int64_t* wat_off, const int32_t* wat_g, const int64_t* wpt_off,
-                            const int32_t* pairtab, const int32_t* nf_of, const int64_t* vp_off, const double* vpair,
-                            const double* TW, double* HW, hipStream_t st);
+// HW[q] = TW[q] + vpair[hmap[q]] (TW[q] where hmap[q] < 0) over total entries
+hipError_t launch_window_hv(int64_t total, const int32_t* hmap, const double* vpair, const double* TW, double* HW,
+                            hipStream_t st);
 // Windows above the register-resident limit (blocked sweep in a global workspace).
 size_t big_lds_bytes(int nmax, int b, int elem_bytes);
 hipError_t launch_pole_big(int fp32, int bmax, int nmax, const lres::WinDesc* wins, const int32_t* ids, int nids,

@@ -1169,44 +1169,18 @@ __global__ void __launch_bounds__(64) k_pair_v(const int32_t* __restrict__ pairA
         }
 }
 
-// H_W[pa][pb] = T_W[pa][pb] + V(atom(a), atom(b))[fa][fb] in the T-last order.
-// Per window: lat/lf give, per T-last position, the window-local atom and the
-// function's index within that atom; the window's atom table wat_g (global atom
-// ids) and its natw x natw pair table (global pair id or -1) are at wat_off /
-// wpt_off.  Pairs are stored once with A <= B (global ids); the other
-// orientation reads the transposed block.
-__global__ void __launch_bounds__(256) k_window_hv(const WinDesc* __restrict__ wins, const int64_t* __restrict__ Woff,
-                                                   const int32_t* __restrict__ lat_of_pos,
-                                                   const int32_t* __restrict__ lf_of_pos,
-                                                   const int64_t* __restrict__ wat_off,
-                                                   const int32_t* __restrict__ wat_g,
-                                                   const int64_t* __restrict__ wpt_off,
-                                                   const int32_t* __restrict__ pairtab,
-                                                   const int32_t* __restrict__ nf_of,
-                                                   const int64_t* __restrict__ vp_off,
+// H_W = T_W + V gathered into the window blocks (T-last order), over all windows'
+// entries at once: hmap[q] is the place in vpair of entry q's V value, or -1 where
+// the two functions' atoms share no grid point (built on the host with the pair
+// tables: pairs are stored once with A <= B, the other orientation reads the
+// transposed block).
+__global__ void __launch_bounds__(256) k_window_hv(int64_t total, const int32_t* __restrict__ hmap,
                                                    const double* __restrict__ vpair, const double* __restrict__ TW,
                                                    double* __restrict__ HW) {
-    const int w = blockIdx.x;
-    const WinDesc d = wins[w];
-    const int n = d.n;
-    const int32_t* lat = lat_of_pos + Woff[w];
-    const int32_t* lf = lf_of_pos + Woff[w];
-    const int32_t* gat = wat_g + wat_off[w];
-    const int natw = (int)(wat_off[w + 1] - wat_off[w]);
-    const int32_t* ptab = pairtab + wpt_off[w];
-    const double* T = TW + d.mat_off;
-    double* Hout = HW + d.mat_off;
-    for (int64_t q = threadIdx.x; q < (int64_t)n * n; q += blockDim.x) {
-        const int pa = (int)(q / n), pb = (int)(q - (int64_t)pa * n);
-        const int la = lat[pa], lb = lat[pb];
-        double val = T[q];
-        const int pid = ptab[la * natw + lb];
-        if (pid >= 0) {
-            const int ga = gat[la], gb = gat[lb];
-            const double* blk = vpair + vp_off[pid];
-            val += ga <= gb ? blk[lf[pa] * nf_of[gb] + lf[pb]] : blk[lf[pb] * nf_of[ga] + lf[pa]];
-        }
-        Hout[q] = val;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t m = hmap[q];
+        const double t = TW[q];
+        HW[q] = m >= 0 ? t + vpair[m] : t;
     }
 }
 
@@ -1226,12 +1200,11 @@ hipError_t launch_pair_v(int nf_max, int npair, const int32_t* pairA, const int3
     return hipGetLastError();
 }
 
-hipError_t launch_window_hv(const WinDesc* wins, int nwin, const int64_t* Woff, const int32_t* lat, const int32_t* lf,
-                            const int64_t* wat_off, const int32_t* wat_g, const int64_t* wpt_off,
-                            const int32_t* pairtab, const int32_t* nf_of, const int64_t* vp_off, const double* vpair,
-                            const double* TW, double* HW, hipStream_t st) {
-    hipLaunchKernelGGL(k_window_hv, dim3(nwin), dim3(256), 0, st, wins, Woff, lat, lf, wat_off, wat_g, wpt_off,
-                       pairtab, nf_of, vp_off, vpair, TW, HW);
+hipError_t launch_window_hv(int64_t total, const int32_t* hmap, const double* vpair, const double* TW, double* HW,
+                            hipStream_t st) {
+    if (total == 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((total + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_window_hv, dim3((unsigned)blocks), dim3(256), 0, st, total, hmap, vpair, TW, HW);
     return hipGetLastError();
 }
 

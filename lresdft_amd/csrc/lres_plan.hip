@@ -199,8 +199,8 @@ struct lres_plan {
     bool have_basis = false;
     int nf_max = 0;
     int64_t npair = 0;
-    DevBuf<int32_t> d_pairA, d_pairB, d_ent_r, d_ent_a, d_ent_b, d_nf_of, d_lat, d_lf, d_wat_g, d_pairtab;
-    DevBuf<int64_t> d_pent_off, d_xat_off, d_vp_off, d_wat_off, d_wpt_off;
+    DevBuf<int32_t> d_pairA, d_pairB, d_ent_r, d_ent_a, d_ent_b, d_nf_of, d_hmap;
+    DevBuf<int64_t> d_pent_off, d_xat_off, d_vp_off;
     DevBuf<double> d_xat, d_vpair, d_TW;
     bool have_H = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -908,10 +908,12 @@ int lres_plan_set_basis(lres_plan* P, const int64_t* aof, int64_t natom, const i
             ea.insert(ea.end(), L.a.begin(), L.a.end());
             eb.insert(eb.end(), L.b.begin(), L.b.end());
         }
-        // per-window atom tables, pair tables and dense T_W (T-last order)
-        std::vector<int32_t> lat(Woff[nwin]), lf(Woff[nwin]), wat_g, pairtab;
-        std::vector<int64_t> wat_off(nwin + 1, 0), wpt_off(nwin + 1, 0);
+        // per window: dense T_W and, per entry, the place of its V value in vpair
+        // (-1: the two functions' atoms share no grid point), both in the T-last order
+        require(vpoff[npair] < (int64_t(1) << 31), "pair blocks exceed 2^31 values");
+        std::vector<int32_t> lat(Woff[nwin]), lf(Woff[nwin]), pairtab;
         std::vector<double> TW(P->d_SW.n, 0.0);
+        std::vector<int32_t> hmap(P->d_SW.n, -1);
         std::vector<int32_t> pos_all(Woff[nwin]);
         hip_check(hipMemcpy(pos_all.data(), P->d_pos.p, pos_all.size() * sizeof(int32_t), hipMemcpyDeviceToHost),
                   "copy");
@@ -933,9 +935,7 @@ int lres_plan_set_basis(lres_plan* P, const int64_t* aof, int64_t natom, const i
                 loc[W[a]] = a;
             }
             const int natw = (int)atoms.size();
-            wat_off[w + 1] = wat_off[w] + natw;
-            wpt_off[w + 1] = wpt_off[w] + (int64_t)natw * natw;
-            wat_g.insert(wat_g.end(), atoms.begin(), atoms.end());
+            pairtab.clear();
             for (int i = 0; i < natw; ++i)
                 for (int j = 0; j < natw; ++j) {
                     const int32_t A = std::min(atoms[i], atoms[j]), B = std::max(atoms[i], atoms[j]);
@@ -943,6 +943,17 @@ int lres_plan_set_basis(lres_plan* P, const int64_t* aof, int64_t natom, const i
                     pairtab.push_back(it == pid.end() ? -1 : it->second);
                 }
             const int64_t m0 = P->wins[w].mat_off;
+            const int32_t* wl = lat.data() + Woff[w];
+            const int32_t* wf = lf.data() + Woff[w];
+            for (int pa = 0; pa < n; ++pa)
+                for (int pb = 0; pb < n; ++pb) {
+                    const int32_t q = pairtab[(size_t)wl[pa] * natw + wl[pb]];
+                    if (q < 0) continue;
+                    // pairs are stored once with A <= B: the other orientation reads the transposed block
+                    const int32_t ga = atoms[wl[pa]], gb = atoms[wl[pb]];
+                    hmap[m0 + (int64_t)pa * n + pb] =
+                        (int32_t)(vpoff[q] + (ga <= gb ? (int64_t)wf[pa] * nf[gb] + wf[pb] : (int64_t)wf[pb] * nf[ga] + wf[pa]));
+                }
             for (int a = 0; a < n; ++a)
                 for (int64_t p = Tp[W[a]]; p < Tp[W[a] + 1]; ++p) {
                     const int b = loc[Ti[p]];
@@ -951,8 +962,6 @@ int lres_plan_set_basis(lres_plan* P, const int64_t* aof, int64_t natom, const i
             for (int32_t A : atoms) loc_atom[A] = -1;
             for (int a = 0; a < n; ++a) loc[W[a]] = -1;
         }
-        std::vector<int64_t> woff_rel(nwin + 1);
-        for (int64_t w = 0; w <= nwin; ++w) woff_rel[w] = Woff[w] - Woff[0];
         hipStream_t st = P->st;
         P->d_pairA.upload(pA.data(), npair, st);
         P->d_pairB.upload(pB.data(), npair, st);
@@ -965,12 +974,7 @@ int lres_plan_set_basis(lres_plan* P, const int64_t* aof, int64_t natom, const i
         P->d_xat.upload(xat.data(), xat.size(), st);
         P->d_vp_off.upload(vpoff.data(), npair + 1, st);
         P->d_vpair.alloc(std::max<int64_t>(1, vpoff[npair]));
-        P->d_lat.upload(lat.data(), lat.size(), st);
-        P->d_lf.upload(lf.data(), lf.size(), st);
-        P->d_wat_off.upload(wat_off.data(), nwin + 1, st);
-        P->d_wat_g.upload(wat_g.data(), wat_g.size(), st);
-        P->d_wpt_off.upload(wpt_off.data(), nwin + 1, st);
-        P->d_pairtab.upload(pairtab.data(), pairtab.size(), st);
+        P->d_hmap.upload(hmap.data(), hmap.size(), st);
         P->d_TW.upload(TW.data(), TW.size(), st);
         hip_check(hipStreamSynchronize(st), "sync");
         P->nf_max = nf_max;
@@ -992,9 +996,7 @@ int lres_set_potential(lres_plan* P, const double* d_v) {
                                 P->d_ent_a.p, P->d_ent_b.p, P->d_xat_off.p, P->d_nf_of.p, P->d_xat.p, P->d_vp_off.p,
                                 d_v, P->dV, P->d_vpair.p, P->st),
                   "pair kernel");
-        hip_check(launch_window_hv(P->d_wins.p, (int)P->nwin, P->d_Woff.p, P->d_lat.p, P->d_lf.p, P->d_wat_off.p,
-                                   P->d_wat_g.p, P->d_wpt_off.p, P->d_pairtab.p, P->d_nf_of.p, P->d_vp_off.p,
-                                   P->d_vpair.p, P->d_TW.p, P->d_HW.p, P->st),
+        hip_check(launch_window_hv(P->d_TW.n, P->d_hmap.p, P->d_vpair.p, P->d_TW.p, P->d_HW.p, P->st),
                   "window H kernel");
         if (timed) hip_check(hipEventRecord(P->ev[1], P->st), "event");
         P->have_H = true;
