@@ -772,6 +772,7 @@ __global__ void k_sum2(const double* __restrict__ a, const double* __restrict__ 
     if (threadIdx.x == 0) {
         out[0] = ta;
         out[1] = tb;
+        __threadfence_system();   // out may be mapped host memory
     }
 }
 

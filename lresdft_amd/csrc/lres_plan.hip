@@ -186,7 +186,6 @@ struct lres_plan {
     DevBuf<double> d_xval;
     DevBuf<int64_t> d_Hp, d_Hi;
     DevBuf<double> d_Hv;
-    DevBuf<double> d_sigma;
     DevBuf<unsigned long long> d_itot;
     DevBuf<int32_t> d_cand, d_icnt;              // lambda_min: candidate windows, per-window counts
     DevBuf<double> d_rq;                         // lambda_min: per-window Rayleigh bound
@@ -208,11 +207,14 @@ struct lres_plan {
     // cumulative pole-kernel statistics per precision: passes and milliseconds (HIP events)
     int64_t stat_passes[2] = {0, 0};
     double stat_ms[2] = {0.0, 0.0};
-    // pinned host staging: a pass's sums come back in slots 0, 1 (an asynchronous
-    // copy, then one wait); the block from kPinSlots stages inertia shifts on the way
-    // in (every call that fills it waits for its stream before returning)
+    // pinned, mapped, coherent host staging (d_pin: its device address): a pass's
+    // sums are written there by the sum kernel itself (slots 0, 1), inertia totals
+    // come back in slots 2.. (an asynchronous copy), and the inertia kernels read
+    // their shifts from the block at kPinSlots; every call that uses it waits for its
+    // stream before returning
     static constexpr int kPinSlots = 2 + 1024;
     double* h_pin = nullptr;
+    double* d_pin = nullptr;
     ~lres_plan() {
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -356,9 +358,8 @@ void run_pass(lres_plan* P, double mu, double e_lo, double* d_dens, double* d_NB
         hip_check(launch_density(P->d_wins.p, (int)P->nwin, P->kmax, P->d_D.p, P->d_own_g.p, P->d_xptr.p, P->d_xidx.p,
                                  P->d_xval.p, d_dens, density_blocks(P), P->st),
                   "density kernel");
-    hip_check(launch_sum2(NB, EB, (int)P->nwin, P->d_sums.p, P->st), "sum kernel");
+    hip_check(launch_sum2(NB, EB, (int)P->nwin, h_sums ? P->d_pin : P->d_sums.p, P->st), "sum kernel");
     if (h_sums) {
-        hip_check(hipMemcpyAsync(P->h_pin, P->d_sums.p, 2 * sizeof(double), hipMemcpyDeviceToHost, P->st), "copy");
         hip_check(hipStreamSynchronize(P->st), "sync");
         h_sums[0] = P->h_pin[0];
         h_sums[1] = P->h_pin[1];
@@ -392,10 +393,8 @@ void inertia_launch(lres_plan* P, const double* sig, int ns, int32_t* d_counts,
     require(P->have_H, "no Hamiltonian loaded");
     require(ns > 0 && ns <= 1024, "nshift must be in 1..1024");
     require(ns <= lres_plan::kPinSlots, "too many shifts");
-    if ((int)P->d_sigma.n < ns) P->d_sigma.alloc(ns);
-    double* stage = P->h_pin + lres_plan::kPinSlots;
-    std::memcpy(stage, sig, ns * sizeof(double));
-    hip_check(hipMemcpyAsync(P->d_sigma.p, stage, ns * sizeof(double), hipMemcpyHostToDevice, P->st), "upload");
+    std::memcpy(P->h_pin + lres_plan::kPinSlots, sig, ns * sizeof(double));
+    const double* d_sig = P->d_pin + lres_plan::kPinSlots;
     if ((int)P->d_itot.n < ns) P->d_itot.alloc(ns);
     hip_check(hipMemsetAsync(P->d_itot.p, 0, ns * sizeof(unsigned long long), P->st), "memset");
     std::vector<int32_t> sub;        // candidate ids grouped by bucket
@@ -416,16 +415,16 @@ void inertia_launch(lres_plan* P, const double* sig, int ns, int32_t* d_counts,
         const int nids = cand ? (int)(sub_off[bi + 1] - sub_off[bi]) : (int)b.ids.size();
         if (nids == 0) continue;
         if (b.big)
-            hip_check(launch_inertia_big(b.bmax, b.nmax, P->d_wins.p, ids, nids, P->d_SW.p, P->d_HW.p, P->d_sigma.p,
+            hip_check(launch_inertia_big(b.bmax, b.nmax, P->d_wins.p, ids, nids, P->d_SW.p, P->d_HW.p, d_sig,
                                          ns, P->d_work.p, P->d_ws_off.p, P->ngroup, d_counts, P->d_itot.p, P->st),
                       "blocked inertia kernel");
         else if (inertia_on_mfma(b.nmax))
             hip_check(launch_inertia_mfma((b.nmax + 15) / 16, P->d_wins.p, ids, nids, P->d_SW.p, P->d_HW.p,
-                                          P->d_sigma.p, ns, d_counts, P->d_itot.p, P->st),
+                                          d_sig, ns, d_counts, P->d_itot.p, P->st),
                       "matrix-core inertia kernel");
         else
             hip_check(launch_inertia(b.ishape.T, b.ishape.G, P->d_wins.p, ids, nids, P->d_SW.p, P->d_HW.p,
-                                     P->d_sigma.p, ns, d_counts, P->d_itot.p, P->st),
+                                     d_sig, ns, d_counts, P->d_itot.p, P->st),
                       "inertia kernel");
     }
 }
@@ -433,8 +432,9 @@ void inertia_launch(lres_plan* P, const double* sig, int ns, int32_t* d_counts,
 void inertia(lres_plan* P, const double* sig, int ns, int64_t* h_total, int32_t* d_counts,
              const std::vector<int32_t>* cand = nullptr, int32_t* h_counts = nullptr) {
     inertia_launch(P, sig, ns, d_counts, cand);
-    std::vector<unsigned long long> tot(ns);
-    hip_check(hipMemcpyAsync(tot.data(), P->d_itot.p, ns * sizeof(unsigned long long), hipMemcpyDeviceToHost, P->st),
+    static_assert(sizeof(unsigned long long) == sizeof(double), "totals share the staging slots");
+    const unsigned long long* tot = reinterpret_cast<const unsigned long long*>(P->h_pin + 2);
+    hip_check(hipMemcpyAsync(P->h_pin + 2, P->d_itot.p, ns * sizeof(unsigned long long), hipMemcpyDeviceToHost, P->st),
               "copy");
     // per-window counts (counts[w * ns + s]) with the totals, under one synchronisation
     if (h_counts && d_counts)
@@ -648,8 +648,10 @@ int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, 
         P->precision = precision;
         P->st = (hipStream_t)stream;
         for (auto& e : P->ev) hip_check(hipEventCreate(&e), "hipEventCreate");
-        hip_check(hipHostMalloc((void**)&P->h_pin, 2 * lres_plan::kPinSlots * sizeof(double), hipHostMallocDefault),
+        hip_check(hipHostMalloc((void**)&P->h_pin, 2 * lres_plan::kPinSlots * sizeof(double),
+                                hipHostMallocMapped | hipHostMallocCoherent),
                   "hipHostMalloc");
+        hip_check(hipHostGetDevicePointer((void**)&P->d_pin, P->h_pin, 0), "hipHostGetDevicePointer");
         P->wins.resize(nwin);
         P->n_of.resize(nwin);
         P->k_of.resize(nwin);
