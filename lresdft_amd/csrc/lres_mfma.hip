@@ -151,9 +151,12 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
     constexpr bool kEven = NT % NWC == 0;
     // row panel of the current block, planar (real and imaginary parts apart) so
     // that accumulator registers go to LDS and come back as MFMA operands without
-    // repacking: rp[b][j][s] = A[4p + s][j]
-    __shared__ __attribute__((aligned(16))) R rpr[2][NR][4];
-    __shared__ __attribute__((aligned(16))) R rpi[2][NR][4];
+    // repacking: rp[b][j][s] = A[4p + s][j].  fp64 rows are padded to 40 bytes: the
+    // B operands read one column per lane, and at a 32-byte stride lanes j and j + 8
+    // would share LDS banks
+    constexpr int RS = sizeof(R) == 8 ? 5 : 4;
+    __shared__ __attribute__((aligned(16))) R rpr[2][NR][RS];
+    __shared__ __attribute__((aligned(16))) R rpi[2][NR][RS];
     const int w = ids[blockIdx.x];
     const int gq = blockIdx.y;
     const WinDesc d = wins[w];
