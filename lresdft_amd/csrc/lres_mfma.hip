@@ -423,8 +423,10 @@ k_inertia_mfma(const WinDesc* __restrict__ wins, const int32_t* __restrict__ ids
     using MF = Mfma<double>;
     using V = typename MF::V;
     constexpr int NR = 16 * NT;
-    // row panel of the current block: rp[b][j][s] = A[4p + s][j]
-    __shared__ __attribute__((aligned(16))) double rp[2][NR][4];
+    // row panel of the current block: rp[b][j][s] = A[4p + s][j], rows padded to 40
+    // bytes (a 32-byte stride puts the column reads of lanes j and j + 8 on the same
+    // LDS banks)
+    __shared__ __attribute__((aligned(16))) double rp[2][NR][5];
     const int w = ids[blockIdx.x];
     const int s_id = blockIdx.y;
     const WinDesc d = wins[w];
