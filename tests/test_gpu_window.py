@@ -226,6 +226,30 @@ def test_device_hamiltonian_equals_csr_route(sys3):
     assert np.max(np.abs(d1 - d2)) < 1e-12
 
 
+def test_device_hamiltonian_equals_csr_route_uneven_windows():
+    """The same on 27 H2 at random centres (windows of 36 to 96 functions in several
+    size buckets): the device gathers V into windows of every size through its
+    per-entry index map."""
+    import torch
+    from lresdft_amd import system
+    from lresdft_amd.lcao3d import LRFTLCAOCpp, csr64
+    cr, centres = system.random_h2_box(27, 6.0 * 3, dmin=3.0, seed=7, dx=0.6, rc=0.6)
+    b = LCAOBasis(cr, **BASIS)
+    v = guess_v(cr)
+    print("[gpu] device H(v) vs CSR gather, 27 H2 at random centres, R_w=6 M=32", flush=True)
+    C = LRFTLCAOCpp(b, centres, R_w=6.0, M=32, device_h=True)
+    C.plan.set_h(csr64(b.H(v)))
+    lmin = C.plan.lambda_min()
+    e_lo, mu = lmin - 0.3, lmin + 0.35
+    n1, e1 = C.plan.run(mu, e_lo)
+    d1 = C.plan.dens.cpu().numpy().copy()
+    C.plan.set_potential(torch.from_numpy(np.ascontiguousarray(v.reshape(-1))).cuda())
+    n2, e2 = C.plan.run(mu, e_lo)
+    d2 = C.plan.dens.cpu().numpy()
+    assert abs(n1 - n2) < 1e-11 and abs(e1 - e2) < 1e-11
+    assert np.max(np.abs(d1 - d2)) < 1e-12
+
+
 def test_gpu_scf_driver_matches_oracle_scf():
     """Device SCF (torch Hartree/LDA/DIIS + device H(v) + pole passes) vs the
     oracle's CPU SCF with the eigen route, n=2 R_w 8 M 32, tol 1e-8: energy to 1e-10 Ha/e."""
