@@ -58,6 +58,38 @@ class GridOps:
         vG = -4 * math.pi * Sg.reshape(cr.shape) * torch.exp(-self.G2 * cr.rc**2 / 4) * self.inv_G2 / vol
         return torch.fft.ifftn(vG).real * cr.ng
 
+    def ion_forces(self, n, chunk=256):
+        """The ion-potential part of the force, -dV sum_r n(r) dv_ion(r)/dR_A, for every
+        atom (natoms x 3 tensor).  With v_ion = Re IFFT(vG) as ion_potential forms it
+        (three_d.py:102-119), vG = -4 pi sum_A Z_A e^{-iG.R_A} e^{-G^2 rc^2/4} / (G^2 vol),
+        the sum is Re sum_G (-i G) vG_A(G) conj(FFT(n)(G)); the per-atom phase e^{-iG.R_A}
+        factorises per axis, so it is two batched products per chunk of atoms (the
+        second for the G_z weight)."""
+        torch, cr = self.torch, self.cr
+        n1, n2, n3 = cr.shape
+        vol = float(np.prod(cr.cell))
+        nG = torch.fft.fftn(n.reshape(cr.shape))
+        P = (-4 * math.pi / vol) * torch.exp(-self.G2 * cr.rc**2 / 4) * self.inv_G2 * torch.conj(nG)
+        P2 = P.reshape(n1 * n2, n3)
+        k1, k2, k3 = self.k1
+        K1 = k1.repeat_interleave(n2)          # G_x of the flattened (i, j) index
+        K2 = k2.repeat(n1)                     # G_y
+        pos = torch.tensor(cr.pos, dtype=torch.float64, device=self.device)
+        Z = torch.tensor(cr.Z, dtype=torch.float64, device=self.device)
+        out = torch.empty((len(cr.pos), 3), dtype=torch.float64, device=self.device)
+        for a0 in range(0, len(cr.pos), chunk):
+            p = pos[a0:a0 + chunk]
+            ex = torch.exp(-1j * p[:, 0:1] * k1[None, :])
+            ey = torch.exp(-1j * p[:, 1:2] * k2[None, :])
+            ez = torch.exp(-1j * p[:, 2:3] * k3[None, :])
+            Q = (P2 @ ez.transpose(0, 1)).transpose(0, 1)                        # (atoms, n1 n2)
+            Qz = (P2 @ (k3[:, None] * ez.transpose(0, 1))).transpose(0, 1)
+            E = (ex[:, :, None] * ey[:, None, :]).reshape(len(p), n1 * n2)
+            t = torch.stack([(E * Q * K1[None, :]).sum(1), (E * Q * K2[None, :]).sum(1), (E * Qz).sum(1)], 1)
+            # sum_r n dv/dR = Re(-i t); the force is -dV times it
+            out[a0:a0 + chunk] = -cr.dV * Z[a0:a0 + chunk, None] * torch.imag(t)
+        return out
+
     def hartree(self, n):
         """v_H = IFFT(4 pi FFT(n) / G^2), G = 0 dropped (three_d.py:122-129); the
         density is real, so real-to-complex transforms over the half spectrum."""

@@ -296,6 +296,68 @@ def build_windows(basis, cells, R_w):
     return windows, own
 
 
+def _d2_ref(g, c, L):
+    """Squared minimum-image distance in the operation order of build_windows'
+    einsum ((x^2 + y^2) + z^2), elementwise over broadcast g, c."""
+    dd = min_image(g - c, L)
+    return (dd[..., 0] * dd[..., 0] + dd[..., 1] * dd[..., 1]) + dd[..., 2] * dd[..., 2]
+
+
+def voronoi_owner_fast(grid, cells, L, k=16, chunk=65536):
+    """The owner array of build_windows (nearest centre by the same squared
+    distance, lowest index on ties) without the N_g x N_cells scan: a periodic
+    k-d tree proposes the k nearest centres of every point, their distances are
+    recomputed with build_windows' formula, and the smallest value with the
+    lowest index wins.  A point is resolved this way only if its k-th proposal is
+    farther than its best by a margin far above rounding (so no centre at the
+    minimal distance can be missing); the rest take the full scan.  Same result
+    as build_windows (checked against it on tie-heavy lattices and random boxes)."""
+    from scipy.spatial import cKDTree
+    cells = np.asarray(cells, float)
+    ncell = len(cells)
+    k = min(k, ncell)
+    tree = cKDTree(np.mod(cells, L), boxsize=L)
+    owner = np.empty(len(grid), np.int64)
+    for s in range(0, len(grid), chunk):
+        g = grid[s:s + chunk]
+        dk, ik = tree.query(g, k=k)
+        if k == 1:
+            dk, ik = dk[:, None], ik[:, None]
+        d2 = _d2_ref(g[:, None, :], cells[ik], L)
+        best = d2.min(axis=1)
+        # lowest cell index among the candidates at the minimal value
+        cand = np.where(d2 == best[:, None], ik, ncell)
+        own_c = cand.min(axis=1)
+        safe = (dk[:, -1] ** 2 > best * (1 + 1e-9) + 1e-12) if k < ncell else np.ones(len(g), bool)
+        bad = np.flatnonzero(~safe)
+        for q in bad:
+            full = _d2_ref(g[q][None, :], cells, L)
+            own_c[q] = int(np.flatnonzero(full == full.min())[0])
+        owner[s:s + chunk] = own_c
+    return owner
+
+
+def build_windows_fast(basis, cells, R_w):
+    """build_windows with voronoi_owner_fast for the grid ownership (the same
+    tables; for the full-size configurations, where the N_g x N_cells scan of the
+    restatement takes about a minute)."""
+    cr = basis.cr
+    cells = np.asarray(cells, float)
+    da = min_image(cr.pos[:, None, :] - cells[None, :, :], cr.cell)
+    cell_of_atom = np.argmin(np.linalg.norm(da, axis=2), axis=1)
+    cell_of_fn = cell_of_atom[basis.atom_of_fn]
+    dc = np.linalg.norm(min_image(cells[:, None, :] - cells[None, :, :], cr.cell), axis=2)
+    grid = np.stack(np.meshgrid(*[np.arange(m) for m in cr.shape], indexing="ij"), -1).reshape(-1, 3) * cr.dx
+    owner = voronoi_owner_fast(grid, cells, np.asarray(cr.cell, float))
+    order = np.argsort(owner, kind="stable")
+    bounds = np.searchsorted(owner[order], np.arange(len(cells) + 1))
+    windows, own = [], []
+    for B in range(len(cells)):
+        windows.append(np.flatnonzero(np.isin(cell_of_fn, np.flatnonzero(dc[B] <= R_w))).astype(np.int64))
+        own.append(order[bounds[B]:bounds[B + 1]].astype(np.int64))
+    return windows, own
+
+
 # ---------------------------------------------------------------------------
 # filter and pole rule
 # ---------------------------------------------------------------------------
@@ -372,6 +434,32 @@ def eig_window_pass(H, S, Xr, windows, own, dV, mu, e_lo, M, upper=None, want_de
     lamoff = np.concatenate([[0], np.cumsum([len(l) for l in lams])]).astype(np.int64)
     out = (dens, np.concatenate(lams), np.concatenate(cs), lamoff)
     return out + (y2,) if keep_y2 else out + (None,)
+
+
+def window_spectra(H, S, Xr, windows, own):
+    """Full generalised spectra of every window and the squared owned-point
+    amplitudes: [(lam, Y^2)] with Y = X[own, W] U (src/lrft_cpp.cpp:159-185 with
+    upper = inf, Y^2 kept in float64).  One decomposition serves any number of
+    (mu, e_lo, M) evaluations by spectra_pass."""
+    out = []
+    for W, o in zip(windows, own):
+        lam, Y = _eig_one(H, S, Xr, W, o, None, None)
+        out.append((lam, Y * Y))
+    return out
+
+
+def spectra_pass(spectra, own, ng, dV, mu, e_lo, M):
+    """The eigen-route pass from window_spectra: (dens (N_g,), N_B, E_B) with
+    N_B = sum_k c_k r_M(lam_k), E_B = sum_k c_k lam_k r_M(lam_k), c_k = sum_own Y^2 dV,
+    dens[own] = Y^2 r_M (no spin factor; lrft_cpp.cpp:179-190)."""
+    dens = np.zeros(ng)
+    NB, EB = np.zeros(len(spectra)), np.zeros(len(spectra))
+    for B, (lam, Y2) in enumerate(spectra):
+        occ = r_M(lam, mu, e_lo, M)
+        c = np.sum(Y2, axis=0) * dV
+        NB[B], EB[B] = np.dot(c, occ), np.dot(c, lam * occ)
+        dens[own[B]] = Y2 @ occ
+    return dens, NB, EB
 
 
 def pole_window_block(HW, SW, T, zs, ws):

@@ -1,0 +1,233 @@
+"""GPU parity at the full sizes of the benchmark configurations.
+
+Every configuration's window step (HIP, through the C-ABI) against the oracle at
+a fixed circle (mu, e_lo) on the same H(v):
+  * 216 H2 (432 e-), 6x6x6 lattice, R_w 8 (n_W 84), M 32;
+  * 512 H2 (1024 e-), 8x8x8 lattice, R_w 8 and 10 (n_W 84 / 228), M 32 and 64;
+  * the same at a = 5.1 (0.85x compressed, 8 grid points per cell), R_w 8 and 10
+    (n_W 228 / 324);
+  * 343 H2 at random centres (d_min 3.0 bohr, seed 7) in a 42 bohr box, R_w 8
+    (n_W 36..216, several size buckets);
+  * 1000 H2 (2000 e-), 10x10x10 lattice, R_w 10 (n_W 228), M 64.
+The oracle side is the eigen route with full window spectra (one generalised
+eigendecomposition per window, then r_M at each M; the pole route equals it by
+Lemma 1, lrft/lcao3d.py:17-22, and results/window_resolvent.json shows the two
+routes agreeing to <= 2.6e-14).  Bars: owned-point density max-abs 1e-12, N_B and
+E_B per window 1e-12, sum N_B and sum E_B 1e-13 per electron; the complex64 pass
+at the fp32 bar (relative 1e-5 on the sums, density 5e-5; 1e-4 / 2e-3 for the
+random-centre box, whose close pairs condition S_W worse).  The window and
+ownership tables of the native builder equal the oracle's at every size.
+
+SCF level: the fp64 GPU SCF against the oracle's SCF at 432 e- (tol 1e-8,
+1e-10 Ha per electron), and the mixed-precision SCF against the fp64 GPU SCF at
+every 1024 e- variant and at 2000 e- (|dE| <= 2e-7 Ha per electron, the target of
+the mixed path).
+
+Potentials: the guess potential v_ion + v_H(n0) + v_xc(n0) of the uniform start
+density n0 (benchmarks/three_d.py:40-42), formed on the device (its parity with
+the oracle's potentials is tested at small sizes in test_gpu_scf_grid.py); both
+sides receive the same v."""
+import time
+
+import numpy as np
+import pytest
+
+from oracle.lres_oracle import (BASIS, LCAOBasis, LResEigen, amorphous_h2, build_windows_fast, cell_centres,
+                                h2_crystal, scf_lcao, solve_mu, spectra_pass, window_spectra)
+
+pytestmark = pytest.mark.gpu
+
+_SYS = {}
+
+
+def _say(msg):
+    print(f"[gpu-configs {time.strftime('%H:%M:%S')}] {msg}", flush=True)
+
+
+def system(kind):
+    """(crystal, oracle basis, cell centres, label) of a configuration, cached per module."""
+    if kind in _SYS:
+        return _SYS[kind]
+    if kind == "rand343":
+        cr, cells = amorphous_h2(343, 42.0, dmin=3.0, seed=7)
+        label = "343 H2 random centres, L 42"
+    else:
+        n, a = kind
+        cr = h2_crystal((n, n, n), a=a, pts_per_cell=int(round(a / 0.6)), rc=0.6)
+        cells = cell_centres((n, n, n), a)
+        label = f"{n}^3 H2 lattice a={a}"
+    b = LCAOBasis(cr, **BASIS)
+    _SYS.clear()            # keep one system alive at a time (2000 e- spectra take ~2 GB)
+    _SYS[kind] = (cr, b, cells, label)
+    return _SYS[kind]
+
+
+def guess_potential(cr):
+    """Guess potential on the device (host copy returned)."""
+    import torch
+
+    from lresdft_amd.scf import GridOps
+    g = GridOps(cr)
+    vion = g.ion_potential()
+    d0 = torch.full(cr.shape, cr.n_elec / (cr.ng * cr.dV), dtype=torch.float64, device="cuda")
+    return g.potential(d0, vion).cpu().numpy().reshape(cr.shape)
+
+
+CASES = [
+    # (system, R_w, M values, expected largest n_W)
+    ((6, 6.0), 8.0, (32,), 84),
+    ((8, 6.0), 8.0, (32, 64), 84),
+    ((8, 6.0), 10.0, (32, 64), 228),
+    ((8, 5.1), 8.0, (32,), 228),
+    ((8, 5.1), 10.0, (32,), 324),
+    ("rand343", 8.0, (32,), None),
+    ((10, 6.0), 10.0, (64,), 228),
+]
+
+
+def _case_id(c):
+    s = c[0] if isinstance(c[0], str) else f"n{c[0][0]}_a{c[0][1]}"
+    return f"{s}_rw{c[1]:g}_M{'-'.join(map(str, c[2]))}"
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("case", CASES, ids=[_case_id(c) for c in CASES])
+def test_fixed_circle_pass_full_size(case):
+    from lresdft_amd.lcao3d import LRFTLCAOCpp, csr64
+    kind, Rw, Ms, nmax = case
+    cr, b, cells, label = system(kind)
+    N = cr.n_elec
+    _say(f"{label}, R_w {Rw}: potential, windows, oracle spectra")
+    v = guess_potential(cr)
+    H = b.H(v)
+    W, own = build_windows_fast(b, cells, Rw)
+    spectra = window_spectra(H, b.S, b.Xr, W, own)
+    lam = np.concatenate([s[0] for s in spectra])
+    c = np.concatenate([np.sum(s[1], axis=0) * cr.dV for s in spectra])
+    e_lo = lam.min() - 0.3
+    for M in Ms:
+        mu = solve_mu(lam, c, e_lo, N, M)
+        _say(f"{label}, R_w {Rw}, M {M}: mu {mu:.6f}, GPU passes")
+        C = LRFTLCAOCpp(b, cells, R_w=Rw, M=M)
+        if nmax is not None:
+            assert int(C.plan.n.max()) == nmax
+        # the native window/ownership tables are the oracle's
+        assert len(C.windows) == len(W)
+        assert all(np.array_equal(x, y) for x, y in zip(C.windows, W))
+        assert all(np.array_equal(x, y) for x, y in zip(C.own, own))
+        dref, NBref, EBref = spectra_pass(spectra, own, cr.ng, cr.dV, mu, e_lo, M)
+        C.plan.set_h(csr64(H))
+        nsum, esum = C.plan.run(mu, e_lo, want_density=True)
+        dens = C.plan.dens.cpu().numpy()
+        NB, EB = C.plan.NB.cpu().numpy(), C.plan.EB.cpu().numpy()
+        errs = dict(dens=float(np.max(np.abs(dens - dref))), NB=float(np.max(np.abs(NB - NBref))),
+                    EB=float(np.max(np.abs(EB - EBref))), sumN=abs(nsum - NBref.sum()) / N,
+                    sumE=abs(esum - EBref.sum()) / N)
+        _say(f"  fp64 vs oracle: {errs}")
+        assert errs["dens"] < 1e-12 and errs["NB"] < 1e-12 and errs["EB"] < 1e-12
+        assert errs["sumN"] < 1e-13 and errs["sumE"] < 1e-13
+        # the device-assembled H(v) gives the same pass (the route every SCF takes)
+        import torch
+        C.plan.set_potential(torch.from_numpy(np.ascontiguousarray(v.reshape(-1))).cuda())
+        n2, e2 = C.plan.run(mu, e_lo, want_density=True)
+        d2 = C.plan.dens.cpu().numpy()
+        assert np.max(np.abs(d2 - dref)) < 1e-12
+        assert abs(n2 - nsum) / N < 1e-13 and abs(e2 - esum) / N < 1e-13
+        # complex64 pass at the fp32 bar
+        C.set_precision("mixed")
+        n32, e32 = C.plan.run(mu, e_lo, want_density=True)
+        d32 = C.plan.dens.cpu().numpy()
+        e32s = dict(sumN=abs(n32 - nsum) / nsum, sumE=abs(e32 - esum) / abs(esum),
+                    dens=float(np.max(np.abs(d32 - dref))))
+        _say(f"  complex64 vs oracle: {e32s}")
+        # random centres 3 bohr apart make nearly dependent diffuse pairs (worse
+        # conditioned S_W): measured 1.7e-5 / 4.3e-4 there against <= 6e-7 / 8e-7 on
+        # the lattices
+        rel, dtol = (1e-4, 2e-3) if kind == "rand343" else (1e-5, 5e-5)
+        assert e32s["sumN"] < rel and e32s["sumE"] < rel and e32s["dens"] < dtol
+        del C
+    del spectra
+
+
+@pytest.mark.timeout(900)
+def test_fp64_scf_matches_oracle_scf_432e():
+    """216 H2 (432 e-), R_w 8, M 32, SCF to 1e-8: the device SCF driver with the fp64
+    window step vs the oracle's SCF (eigen route, full spectra, second pass instead
+    of the float32 re-density).
+
+    With mu solved to the strict count tolerance every iteration (count_rel tiny),
+    the GPU run follows the oracle's trajectory: same iterations and branch history,
+    e_lo within 1e-9, energy within 1e-10 Ha per electron.  The production setting
+    (mu to 1e-2 of the density residual while it is large) changes the early
+    densities, hence lambda_min where the keep-edge test of lcao3d.py:420 fails and
+    e_lo is reset; at M 32 the first virtual states sit where r_M is ~1e-5 and
+    steep, so the converged energy moves with e_lo (measured 6e-10 Ha per electron
+    here); that run is held to 5e-9 per electron and to the same e_lo within 1e-3."""
+    from lresdft_amd.lcao3d import LRFTLCAOCpp
+    from lresdft_amd.scf import GridOps, scf_lcao as scf_gpu
+    cr, b, cells, label = system((6, 6.0))
+    _say(f"{label}: oracle SCF (tol 1e-8)")
+    ref_solver = LResEigen(b, cells, R_w=8.0, M=32, full_spectrum=True, reuse_vectors=False)
+    ref = scf_lcao(cr, ref_solver, tol=1e-8)
+    _say(f"  oracle: E {ref['energy']:.12f}, {ref['iterations']} iterations, e_lo {ref_solver.e_lo:.12f}, "
+         f"history {ref_solver.history}")
+    g = GridOps(cr)
+    vion = g.ion_potential()
+    C = LRFTLCAOCpp(b, cells, R_w=8.0, M=32)
+    strict = scf_gpu(cr, C, tol=1e-8, grid=g, vion=vion, count_rel=1e-12)
+    dE = abs(strict["energy"] - ref["energy"]) / cr.n_elec
+    hist = [h.replace("warm-resolve", "warm-second-pass").replace("cold-edge", "cold") for h in C.history]
+    _say(f"  GPU strict mu: E {strict['energy']:.12f}, {strict['iterations']} iterations, e_lo {C.e_lo:.12f}, "
+         f"|dE|/e {dE:.2e}, history {C.history}")
+    assert ref["converged"] and strict["converged"]
+    assert abs(C.e_lo - ref_solver.e_lo) < 1e-9
+    assert strict["iterations"] == ref["iterations"] and hist == ref_solver.history
+    assert dE < 1e-10
+    C.mu = None
+    C.history = []
+    prod = scf_gpu(cr, C, tol=1e-8, grid=g, vion=vion)
+    dEp = abs(prod["energy"] - ref["energy"]) / cr.n_elec
+    _say(f"  GPU production: E {prod['energy']:.12f}, {prod['iterations']} iterations, e_lo {C.e_lo:.12f}, "
+         f"|dE|/e {dEp:.2e}")
+    assert prod["converged"] and dEp < 5e-9 and abs(C.e_lo - ref_solver.e_lo) < 1e-3
+
+
+MIXED_CASES = [
+    ((8, 6.0), 8.0, 32),
+    ((8, 6.0), 8.0, 64),
+    ((8, 6.0), 10.0, 32),
+    ((8, 6.0), 10.0, 64),
+    ((8, 5.1), 8.0, 32),
+    ((8, 5.1), 10.0, 32),
+    ((10, 6.0), 10.0, 64),
+]
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("case", MIXED_CASES, ids=[f"n{c[0][0]}_a{c[0][1]}_rw{c[1]:g}_M{c[2]}" for c in MIXED_CASES])
+def test_mixed_scf_meets_energy_target_full_size(case):
+    """complex64 passes + fp64 finish vs the fp64 window step, same SCF (tol 1e-6):
+    |dE| <= 2e-7 Ha per electron, and the fp64 finish leaves |count| <= 1e-8 N."""
+    from lresdft_amd import system as psys
+    from lresdft_amd.lcao3d import LRFTLCAOCpp
+    from lresdft_amd.scf import GridOps, scf_lcao as scf_gpu
+    (n, a), Rw, M = case
+    cr = psys.h2_crystal((n, n, n), a=a, pts_per_cell=int(round(a / 0.6)), rc=0.6)
+    b = psys.LCAOBasis(cr, **psys.DEFAULT_BASIS)
+    cells = psys.cell_centres((n, n, n), a)
+    g = GridOps(cr)
+    vion = g.ion_potential()
+    C = LRFTLCAOCpp(b, cells, R_w=Rw, M=M)
+    _say(f"{cr.n_elec} e- a={a} R_w {Rw} M {M}: fp64 SCF")
+    r64 = scf_gpu(cr, C, tol=1e-6, grid=g, vion=vion)
+    C.set_precision("mixed")
+    C.mu = None
+    _say(f"  fp64 {r64['time']:.2f} s, {r64['iterations']} it; mixed SCF")
+    r32 = scf_gpu(cr, C, tol=1e-6, grid=g, vion=vion)
+    dE = abs(r32["energy"] - r64["energy"]) / cr.n_elec
+    _say(f"  mixed {r32['time']:.2f} s, {r32['iterations']} it, |dE|/e {dE:.2e}")
+    assert r64["converged"] and r32["converged"]
+    assert dE < 2e-7
+    C.set_precision("fp64")
+    n64, _ = C.plan.run(C.mu, C.e_lo, want_density=False)
+    assert abs(2 * n64 - cr.n_elec) <= 1e-8 * cr.n_elec

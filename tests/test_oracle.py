@@ -216,3 +216,54 @@ def test_stored_vectors_reproduce_second_pass(sys3):
         out.append(C.density(0.999 * v))
     assert abs(out[0][1] - out[1][1]) < 1e-9
     assert np.max(np.abs(out[0][0] - out[1][0])) < 1e-6
+
+
+@pytest.mark.parametrize("n,gap", [(2, 0.40642405101199747), (3, 0.4048221304541437)])
+def test_published_ks_gaps_n2_n3(n, gap):
+    """results/three_d.json "cost" n=2, 3: KS gap at the guess potential (uniform start density)."""
+    cr = crystal(n)
+    b = LCAOBasis(cr, **BASIS)
+    _, _, w = ks_lcao(b, guess_v(cr))
+    N = cr.n_elec
+    assert abs((w[N // 2] - w[N // 2 - 1]) - gap) < 1e-12
+
+
+@pytest.mark.parametrize("case", ["lattice", "compressed", "random"])
+def test_fast_window_builder_equals_restatement(case):
+    """build_windows_fast (k-d tree candidates, the restatement's distance formula) gives
+    the same W(B) and own(B) as build_windows, Voronoi ties included: the a = 6 lattice
+    puts ~27% of grid points on cell boundaries, the a = 5.1 lattice (8 points per cell)
+    and a random-centre box test the general case."""
+    from oracle.lres_oracle import build_windows_fast
+    if case == "lattice":
+        cr = crystal(3)
+        cells = cell_centres((3, 3, 3), 6.0)
+    elif case == "compressed":
+        cr = h2_crystal((3, 3, 3), a=5.1, pts_per_cell=8, rc=0.6)
+        cells = cell_centres((3, 3, 3), 5.1)
+    else:
+        cr, cells = amorphous_h2(27, 18.0, dmin=3.0)
+    b = LCAOBasis(cr, **BASIS)
+    W0, own0 = build_windows(b, cells, 8.0)
+    W1, own1 = build_windows_fast(b, cells, 8.0)
+    assert len(W0) == len(W1)
+    assert all(np.array_equal(x, y) for x, y in zip(W0, W1))
+    assert all(np.array_equal(x, y) for x, y in zip(own0, own1))
+
+
+def test_spectra_pass_equals_eigen_pass(sys3):
+    """window_spectra + spectra_pass (one decomposition, several filters) equal
+    eig_window_pass with full spectra at M 32 and 64."""
+    from oracle.lres_oracle import spectra_pass, window_spectra
+    cr, b, v = sys3
+    H = b.H(v)
+    W, own = build_windows(b, cell_centres((3, 3, 3), 6.0), 8.0)
+    sp = window_spectra(H, b.S, b.Xr, W, own)
+    lmin = min(s[0].min() for s in sp)
+    for M in (32, 64):
+        e_lo, mu = lmin - 0.3, lmin + 0.35
+        de, lam, c, lamoff, _ = eig_window_pass(H, b.S, b.Xr, W, own, cr.dV, mu, e_lo, M, None, True)
+        ds, NB, EB = spectra_pass(sp, own, cr.ng, cr.dV, mu, e_lo, M)
+        assert np.max(np.abs(ds - de)) < 1e-14
+        occ = r_M(lam, mu, e_lo, M)
+        assert abs(NB.sum() - np.dot(c, occ)) < 1e-12 and abs(EB.sum() - np.dot(c, lam * occ)) < 1e-12
