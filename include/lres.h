@@ -149,6 +149,19 @@ int lres_lda_potential(const double* d_n, const double* d_vion, const double* d_
 int lres_scf_residual(const double* d_nout, double scale, const double* d_nin, int64_t ng, int nhist, int slot,
                       double* d_Nh, double* d_Rh, double* d_part, double* d_stats, void* stream);
 
+/* lres_scf_residual without the d_part workspace argument: the same kernels, same
+ * results bit for bit, with the partial sums in a static device array of
+ * (LRES_MAX_HIST + 1) x LRES_MIX_PARTS doubles (72 KiB per GPU, held until the
+ * process ends; its device address is looked up on the first call on each
+ * device).  Allocates nothing (no hipMalloc / hipFree, no allocating library
+ * call), for callers that cannot allocate: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid it.  nhist above LRES_MAX_HIST is
+ * rejected (the workspace's row limit).  Calls that overlap on one GPU (on
+ * different streams, or from several threads) share the workspace and race;
+ * order them on one stream.  No counterpart in the reference. */
+int lres_scf_residual_noalloc(const double* d_nout, double scale, const double* d_nin, int64_t ng, int nhist,
+                              int slot, double* d_Nh, double* d_Rh, double* d_stats, void* stream);
+
 /* The integrals of the Harris-Foulkes energy at density d_n with Hartree potential
  * d_vh (lcao3d.py:276-279): d_out[0] = sum n v_H, d_out[1] = sum n e_xc(n),
  * d_out[2] = sum n (v_H + v_xc(n)) (grid sums, no dV).  d_part: 3 x LRES_MIX_PARTS

@@ -47,6 +47,8 @@ SIGNATURES = {
                                 _vp, _vp, _vp]),
     "lres_lda_potential": (C.c_int, [_vp, _vp, _vp, C.c_int64, _vp, _vp]),
     "lres_scf_residual": (C.c_int, [_vp, C.c_double, _vp, C.c_int64, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp]),
+    "lres_scf_residual_noalloc": (C.c_int, [_vp, C.c_double, _vp, C.c_int64, C.c_int, C.c_int, _vp, _vp, _vp,
+                                            _vp]),
     "lres_harris_terms": (C.c_int, [_vp, _vp, C.c_int64, _vp, _vp, _vp]),
     "lres_window_pass": (C.c_int, [_vp, C.c_double, C.c_double, _vp, _vp, _vp, _f64p]),
     "lres_density_last": (C.c_int, [_vp, _vp]),

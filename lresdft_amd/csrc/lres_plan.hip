@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -1044,6 +1045,51 @@ int lres_scf_residual(const double* d_nout, double scale, const double* d_nin, i
         require(slot >= 0 && slot < nhist, "history slot out of range");
         require(d_nout && d_nin && d_Nh && d_Rh && d_part && d_stats, "null buffer");
         hip_check(launch_scf_residual(d_nout, scale, d_nin, ng, nhist, slot, d_Nh, d_Rh, d_part, d_stats,
+                                      (hipStream_t)stream),
+                  "residual kernels");
+    });
+}
+
+// Workspace of lres_scf_residual_noalloc: the per-block partial sums of the
+// residual's fixed-order reductions, (LRES_MAX_HIST + 1) rows of LRES_MIX_PARTS
+// doubles (72 KiB).  Every GPU the library runs on holds its own copy until the
+// process ends; calls on one GPU share it.
+static __device__ __align__(256) double scratch[(LRES_MAX_HIST + 1) * LRES_MIX_PARTS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this GPU's copy of `scratch`, looked up once per device (the
+// array's name is a host-side stand-in, never a device address)
+double* scratch_on_current_device() {
+    int dev = -1;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    require(dev >= 0 && dev < kMaxDevices, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        require(p != nullptr, "no device address for the residual workspace");
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+int lres_scf_residual_noalloc(const double* d_nout, double scale, const double* d_nin, int64_t ng, int nhist,
+                              int slot, double* d_Nh, double* d_Rh, double* d_stats, void* stream) {
+    return guarded([&] {
+        require(ng >= 1, "empty grid");
+        require(nhist >= 1 && nhist <= LRES_MAX_HIST,
+                "history rows must be 1..LRES_MAX_HIST (the size of the static workspace)");
+        require(slot >= 0 && slot < nhist, "history slot out of range");
+        require(d_nout && d_nin && d_Nh && d_Rh && d_stats, "null buffer");
+        double* part = scratch_on_current_device();
+        // k_scf_residual writes every partial that k_rows_total reads (rows 0..nhist,
+        // all LRES_MIX_PARTS blocks), so nothing left by an earlier call is read
+        hip_check(launch_scf_residual(d_nout, scale, d_nin, ng, nhist, slot, d_Nh, d_Rh, part, d_stats,
                                       (hipStream_t)stream),
                   "residual kernels");
     });
