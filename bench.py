@@ -113,50 +113,71 @@ def roofline(stats, flops, peak, kernel, traffic):
                              "(profiles/pole_kernels_traffic_1024e_rw8_m32.txt)") if traffic else None}
 
 
-def cpu_baseline(cr_p, v_host, n, rw, M, iters, budget_s=40.0):
-    """Oracle (eigen route, scipy LAPACK) timed on the host on a bounded sample."""
-    from threadpoolctl import threadpool_limits
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(n, a, rw, M, tol, vion_host, sample_windows=64):
+    """The oracle's LRes SCF timed on the host: the reference's eigen-route algorithm
+    (lrft_windows, dsygvd / dsygvx per window, src/lrft_cpp.cpp:129-233) in the C++/OpenMP
+    restatement oracle/lres_cpu.cpp with scipy's LAPACK, the reference's cold/warm/mu
+    orchestration (LResEigen, lcao3d.py:400-456: truncation at c + 1.5 R, float32 Y^2
+    re-density) and SCF driver (scf_lcao, lcao3d.py:237-284: scipy H(v), numpy FFT
+    Hartree, LDA, DIIS).  Two full SCFs from the uniform density are timed, on one
+    thread and on every core of the process's CPU set (at most 16, the host share of
+    one GPU); the ion potential is formed outside the timed region, as in the GPU
+    timing.  The pole route (zgesv per pole and window, window_resolvent_vs_eig.py:58-65)
+    is timed per pass at the guess potential: all cores on every window, one thread on
+    `sample_windows` windows (scaled to all)."""
+    from oracle import cpu_native
     from oracle import lres_oracle as orc
-    cr = orc.h2_crystal((n, n, n), a=6.0, pts_per_cell=10, rc=0.6)
+    pts = int(round(a / 0.6))
+    cr = orc.h2_crystal((n, n, n), a=a, pts_per_cell=pts, rc=0.6)
     b = orc.LCAOBasis(cr, **orc.BASIS)
-    cells = orc.cell_centres((n, n, n), 6.0)
-    # the host share of one GPU is 16 cores, whatever the machine reports
+    cells = orc.cell_centres((n, n, n), a)
     cores = max(1, min(16, len(os.sched_getaffinity(0))))
     out = {}
     for threads in (1, cores):
-        with threadpool_limits(1):
-            t = time.perf_counter()
-            H = b.H(v_host)
-            t_h = time.perf_counter() - t
-            t = time.perf_counter()
-            dens = np.full(cr.shape, cr.n_elec / (cr.ng * cr.dV))
-            orc.hartree(cr, dens)
-            orc.lda_xc(dens)
-            t_pot = time.perf_counter() - t
-            L = orc.LResEigen(b, cells, R_w=rw, M=M, nthreads=threads)
-            t = time.perf_counter()
-            L.density(v_host)
-            t_cold = time.perf_counter() - t
-            t = time.perf_counter()
-            L.density(v_host)
-            t_warm = time.perf_counter() - t
-            sub = cr.pos[:64]
-            crs = orc.Crystal(sub, np.ones(len(sub)), cr.cell, cr.shape, cr.rc)
-            t = time.perf_counter()
-            orc.ion_potential(crs)
-            t_ion = (time.perf_counter() - t) * len(cr.pos) / len(sub)
-        # the GPU timing starts from a precomputed ion potential, so the estimate leaves it out
-        est = t_cold + (iters - 1) * t_warm + iters * (t_h + t_pot)
-        out[threads] = dict(est=est, t_cold=t_cold, t_warm=t_warm, t_H=t_h, t_pot=t_pot, t_ion=t_ion)
+        L = orc.LResEigen(b, cells, R_w=rw, M=M, native_threads=threads, fast_windows=True)
+        t = time.perf_counter()
+        r = orc.scf_lcao(cr, L, tol=tol, vion=vion_host)
+        wall = time.perf_counter() - t
+        out[threads] = dict(scf_s=wall, iterations=r["iterations"], passes=L.passes, energy=r["energy"],
+                            converged=bool(r["converged"]))
+    # pole route, one pass at the guess potential
+    d0 = np.full(cr.shape, cr.n_elec / (cr.ng * cr.dV))
+    v = vion_host + orc.hartree(cr, d0) + orc.lda_xc(d0)[1]
+    H = b.H(v)
+    L = orc.LResEigen(b, cells, R_w=rw, M=M, native_threads=cores, fast_windows=True)
+    _, lam, _, _, _ = cpu_native.eig_pass(L._tab, H, 0.0, -1.0, M, None, False, cores)
+    e_lo = lam.min() - 0.3
+    mu = e_lo + 0.65
+    t = time.perf_counter()
+    cpu_native.pole_pass(L._tab, H, mu, e_lo, M, False, cores)
+    t_pole_all = time.perf_counter() - t
+    k = min(sample_windows, L._tab.nwin)
+    sub = cpu_native.Tables(b, L.windows[:k], L.own[:k])
+    t = time.perf_counter()
+    cpu_native.pole_pass(sub, H, mu, e_lo, M, False, 1)
+    t_pole_1 = (time.perf_counter() - t) * L._tab.nwin / k
     one, allc = out[1], out[cores]
-    return {"value": allc["est"], "unit": "s", "cores": cores, "kind": "port",
-            "value_1thread": one["est"],
-            "sample": (f"oracle eigen-route LRes (scipy eigh per window, R_w {rw}, M {M}) on the same "
-                       f"{n}^3 crystal: one cold + one warm density build, one H(v), Hartree+LDA; SCF "
-                       f"estimate = cold + (it-1) warm + it (H + pot) with it = {iters} (the GPU run's "
-                       f"iterations); the ion potential, precomputed outside the GPU timing too, is "
-                       f"reported apart (t_ion, timed on 64 atoms and scaled)"),
-            "parts_1thread": one, "parts_all_cores": allc}
+    return {"value": allc["scf_s"], "unit": "s", "cores": cores, "kind": "port",
+            "value_1thread": one["scf_s"], "cpu_model": cpu_model(),
+            "sample": (f"full LRes SCF (tol {tol}) of the same {n}^3 crystal on the host, eigen route "
+                       f"(oracle/lres_cpu.cpp, dsygvd/dsygvx per window from scipy's LAPACK, OpenMP over windows) "
+                       f"with the reference's orchestration and SCF driver; value = {cores} threads, value_1thread "
+                       f"= 1 thread; ion potential precomputed as in the GPU timing"),
+            "scf_1thread": one, "scf_all_cores": allc,
+            "pole_route_pass_s": {"all_cores": t_pole_all,
+                                  "one_thread_scaled": t_pole_1,
+                                  "note": f"one pole-route pass (M {M}, zgesv per pole and window); one thread on "
+                                          f"{k} of {L._tab.nwin} windows, scaled"}}
 
 
 def forces_fd(C, v, F, cells, args, K, seed=11):
@@ -320,8 +341,8 @@ def main():
     if not args.no_forces:
         line["adjoint"] = adjoint_timing(C, grid, vion, mixed[-1], cells, args)
     if not args.no_cpu_baseline and not args.random:
-        v_host = grid.potential(fp64[-1]["dens_in"], vion).cpu().numpy().reshape(cr.shape)
-        line["cpu_baseline"] = cpu_baseline(cr, v_host, n, args.rw, args.M, fp64[-1]["iterations"])
+        line["cpu_baseline"] = cpu_baseline(n, args.a, args.rw, args.M, args.tol,
+                                            vion.cpu().numpy().reshape(cr.shape))
     print(json.dumps(line), flush=True)
 
 

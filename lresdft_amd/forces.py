@@ -103,3 +103,22 @@ def band_forces(solver, v, mu=None, e_lo=None, N=None, hmu=1e-5):
         for i in range(3):
             forces[a, i] = -float(np.sum(Ex * dX[i][:, cols][rows].toarray()))
     return forces, {"beta": beta, "e_band": 2.0 * esum, "count": 2.0 * nsum - N}
+
+
+def total_forces(solver, grid, vion, dens_in, mu=None, e_lo=None, N=None):
+    """Forces -dE/dR_A (Ha/bohr, natoms x 3) of the Harris-Foulkes SCF energy
+    E = e_band - int (v_H + v_xc) n_in + E_H + E_xc (lcao3d.py:275-281, no ion-ion
+    term, as in the reference) at a converged SCF with last input density
+    ``dens_in`` (device tensor): the fixed-N band-energy forces with the converged
+    potential v = v_ion + v_H(n_in) + v_xc(n_in) frozen (band_forces: basis
+    derivatives through H, S and the Q_B), plus the ion-potential response
+    -int n_in dv_ion/dR_A (GridOps.ion_forces).  The derivative of E through n_in
+    vanishes to the extent that the band energy's density response equals n_in
+    (Harris-Foulkes stationarity); for the windowed LRes energy the two differ at
+    the level of the window error, which the FD checks measure.  Also returns the
+    parts {"band", "ion", "beta", "e_band", "count"}."""
+    v = grid.potential(dens_in, vion)
+    F_band, parts = band_forces(solver, v.cpu().numpy(), mu, e_lo, N)
+    F_ion = grid.ion_forces(dens_in).cpu().numpy()
+    parts = dict(parts, band=F_band, ion=F_ion)
+    return F_band + F_ion, parts

@@ -476,7 +476,12 @@ class LRFTLCAOCpp:
         return (2.0 * out[0],) + out[1:] if spin_scaled else out
 
     def _density_loaded(self, N):
-        tol = max(1e-8 * N, self._ftol(N))   # the reference's keep-mu rule, floored at the pass precision
+        # the reference's keep-mu rule |count| <= 1e-8 N (lcao3d.py:412, 428), floored at
+        # the pass precision; under an SCF driver that sets count_tol (relative to its
+        # density residual) the kept count error also stays below count_tol, else an SCF
+        # asked for a residual below ~1e-8 would stall on the kept count error
+        keep = 1e-8 if self.count_tol is None else min(1e-8, self.count_tol)
+        tol = max(keep * N, self._ftol(N))
         m = self.margin
         if self.mu is not None:
             mu, e_lo = self.mu, self.e_lo

@@ -295,9 +295,10 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
         if adaptive:
             # mu to a count accuracy matched to the current density residual: a count
             # error dN moves the residual by about |dN| / N, so |dN| <= 1e-2 err N
-            # costs the iteration nothing; the converged step is polished in fp64 to
-            # the strict tolerance below
-            solver.count_tol = 1e-4 if err is None else min(1e-4, max(count_rel * err, 1e-11))
+            # costs the iteration nothing (the first step, before any residual is
+            # known, takes count_rel x 1e-2); the converged step is polished in fp64
+            # to the strict tolerance below
+            solver.count_tol = min(1e-4, max(count_rel * (1e-2 if err is None else err), 1e-11))
         ts = time.perf_counter()
         # the new history slot (the oldest one once the history is full)
         free = [q for q in range(ndiis) if q not in slots]

@@ -555,11 +555,18 @@ class LResEigen:
     pole route at M = 32, where the truncation drops r_M up to 1.5^-32)."""
 
     def __init__(self, basis, cells, R_w=12.1, M=64, margin=0.3, nthreads=1, full_spectrum=False,
-                 reuse_vectors=True):
+                 reuse_vectors=True, keep_tol=1e-8, native_threads=None, fast_windows=False):
         self.b, self.M, self.margin, self.nthreads = basis, M, margin, nthreads
-        self.windows, self.own = build_windows(basis, cells, R_w)
+        self.windows, self.own = (build_windows_fast if fast_windows else build_windows)(basis, cells, R_w)
+        # native_threads: run the window passes in the C++/OpenMP restatement
+        # (oracle/lres_cpu.cpp) on that many threads instead of scipy per window
+        self.native_threads = native_threads
+        if native_threads:
+            from . import cpu_native
+            self._tab = cpu_native.Tables(basis, self.windows, self.own)
         self.nW = np.array([len(w) for w in self.windows])
         self.full_spectrum, self.reuse_vectors = full_spectrum, reuse_vectors
+        self.keep_tol = keep_tol   # keep mu when |count| <= keep_tol N (the reference: 1e-8, lcao3d.py:412)
         self.mu = self.e_lo = None
         self.passes = 0
         self.history = []
@@ -573,9 +580,14 @@ class LResEigen:
     def _pass(self, H, mu, e_lo, upper, want_density):
         self.passes += 1
         b = self.b
-        dens, lam, c, lamoff, y2 = eig_window_pass(H, b.S, b.Xr, self.windows, self.own, b.cr.dV, mu, e_lo,
-                                                   self.M, upper, want_density, self.nthreads,
-                                                   keep_y2=want_density)
+        if self.native_threads:
+            from . import cpu_native
+            dens, lam, c, lamoff, y2 = cpu_native.eig_pass(self._tab, H, mu, e_lo, self.M, upper, want_density,
+                                                           self.native_threads, keep_y2=want_density)
+        else:
+            dens, lam, c, lamoff, y2 = eig_window_pass(H, b.S, b.Xr, self.windows, self.own, b.cr.dV, mu, e_lo,
+                                                       self.M, upper, want_density, self.nthreads,
+                                                       keep_y2=want_density)
         self._lamoff, self._y2 = lamoff, y2
         return dens, lam, c
 
@@ -600,7 +612,7 @@ class LResEigen:
         cr = b.cr
         N = cr.n_elec if N is None else N
         H = b.H(v)
-        tol = 1e-8 * N
+        tol = self.keep_tol * N
         M = self.M
         if self.mu is not None:
             mu, e_lo = self.mu, self.e_lo
@@ -643,9 +655,11 @@ class KSDense:
 # ---------------------------------------------------------------------------
 
 
-def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, dens0=None):
-    """Pulay-mixed SCF with Harris-Foulkes energy at the last input density."""
-    vion = ion_potential(cr)
+def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, dens0=None, vion=None):
+    """Pulay-mixed SCF with Harris-Foulkes energy at the last input density.
+    vion: the ion potential if already formed (otherwise ion_potential(cr), as
+    lcao3d.py:239 does)."""
+    vion = ion_potential(cr) if vion is None else vion
     dens = np.full(cr.shape, cr.n_elec / (cr.ng * cr.dV)) if dens0 is None else dens0.copy()
     hist_n, hist_r = [], []
     converged = False
