@@ -52,6 +52,10 @@ def parse():
     ap.add_argument("--forces-fd", type=int, default=0, metavar="K",
                     help="also check the forces of K seeded random atoms against central differences of the "
                          "fixed-N band energy (steps 1e-4 / 5e-5 bohr, Richardson)")
+    ap.add_argument("--profile-scf", action="store_true",
+                    help="profiling mode: one warm-up SCF per precision, then one mixed and one fp64 SCF, each "
+                         "preceded and followed by 0.2 s of GPU idle time (the SCFs' borders in a kernel trace); "
+                         "prints the host wall of each and no JSON line")
     ap.add_argument("--profile-pass", type=int, default=0, metavar="K",
                     help="profiling mode: K warm window passes per precision at the workload's guess potential "
                          "and a 256 MiB device copy (the byte-counter calibration), no SCF, no JSON line")
@@ -295,6 +299,18 @@ def main():
 
     if args.profile_pass:
         profile_passes(C, cr, grid, vion, args.profile_pass)
+        return
+    if args.profile_scf:
+        for prec in ("fp64", "mixed"):
+            run_scf(C, cr, grid, vion, prec, args.tol, not args.no_graph)
+        for prec in ("mixed", "fp64"):
+            torch.cuda.synchronize()
+            time.sleep(0.2)
+            r = run_scf(C, cr, grid, vion, prec, args.tol, not args.no_graph)
+            time.sleep(0.2)
+            print(f"profile SCF {prec}: wall {1e3 * r['wall']:.3f} ms, {r['iterations']} iterations, "
+                  f"{r['passes']} passes, passes per iteration {r['pass_log']}, history {r['history']}, "
+                  f"residuals {['%.1e' % e for e in r['residuals']]}", flush=True)
         return
 
     for _ in range(args.warmup):

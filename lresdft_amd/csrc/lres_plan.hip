@@ -1308,12 +1308,18 @@ int lres_mu_solve_warm(lres_plan* P, double N, double e_lo, double mu0, double f
         int evals = 0;
         double sums[2] = {0.0, 0.0};
         double last_mu = NAN, last_f = NAN, last_e = NAN;
+        // LRES_TRACE_MU=1: one line per trial pass on stderr (mu, count, tolerance)
+        static const bool trace = std::getenv("LRES_TRACE_MU") != nullptr;
+        if (trace)
+            fprintf(stderr, "[lres mu] warm search from mu %.12f f %.3e slope %.3e ftol %.3e (%s)\n", mu0, f0, slope,
+                    ftol, P->precision == LRES_MIXED ? "c64" : "f64");
         auto f = [&](double mu) {
             run_pass(P, mu, e_lo, nullptr, nullptr, nullptr, sums);
             ++evals;
             last_mu = mu;
             last_f = 2.0 * sums[0] - N;
             last_e = sums[1];
+            if (trace) fprintf(stderr, "[lres mu]   pass mu %.12f f %.3e\n", mu, last_f);
             return last_f;
         };
         // slope of the count between two trial points when it is positive and finite

@@ -28,10 +28,15 @@ orchestration of lcao3d.py:407-456:
 from __future__ import annotations
 
 import math
+import os
+import sys
 
 import numpy as np
 
 from . import _native as nat
+
+
+_TRACE_MU = bool(os.environ.get("LRES_TRACE_MU"))   # one stderr line per mu decision
 
 
 class StaleMu(RuntimeError):
@@ -314,6 +319,29 @@ class LRFTLCAOCpp:
     def passes(self):
         return self.plan.passes
 
+    def _window(self, H, S, B, upper=None):
+        """Window spectrum (lam, Y) of window B, the diagnostic surface of the
+        reference's Python back end (LRFTLCAO._window, lcao3d.py:163-173, called by
+        benchmarks/referee_checks.py:63, 90, 483): the generalised eigenpairs of
+        (H_W, S_W) (those below `upper` if given) and Y = X[own(B), W] U, the
+        eigenvectors on the owned points.  The pole route never forms eigenpairs;
+        this evaluates them on the GPU (Cholesky of S_W and a symmetric
+        eigensolver, rocSOLVER through torch) for callers that ask for them."""
+        torch = self.plan.torch
+        W = self.windows[B]
+        HW = torch.from_numpy(H[W][:, W].toarray()).cuda()
+        SW = torch.from_numpy(S[W][:, W].toarray()).cuda()
+        L = torch.linalg.cholesky(SW)
+        Li = torch.linalg.solve_triangular(L, torch.eye(len(W), dtype=torch.float64, device="cuda"), upper=False)
+        A = Li @ HW @ Li.transpose(0, 1)
+        lam, V = torch.linalg.eigh(0.5 * (A + A.transpose(0, 1)))
+        U = Li.transpose(0, 1) @ V
+        if upper is not None:
+            keep = lam <= upper
+            lam, U = lam[keep], U[:, keep]
+        Xo = torch.from_numpy(self.b.Xr[self.own[B]][:, W].toarray()).cuda()
+        return lam.cpu().numpy(), (Xo @ U).cpu().numpy()
+
     def set_precision(self, precision):
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
@@ -489,6 +517,9 @@ class LRFTLCAOCpp:
             if c_lo == 0 and c_hi > 0:            # keep the filter edge (lcao3d.py:420)
                 nsum, _ = self.plan.run(mu, e_lo, want_density=False)
                 cnt = 2.0 * nsum - N
+                if _TRACE_MU:
+                    print(f"[lres mu] keep test at mu {mu:.12f}: count {cnt:.3e}, tolerance {tol:.3e} "
+                          f"({self.precision})", file=sys.stderr, flush=True)
                 if abs(cnt) <= tol or (self.lag_mu and abs(cnt) < 0.25):
                     # keep mu (lcao3d.py:423-427, 447-448): the density of this pass
                     dens, e_band, _ = self._finish(mu, e_lo)
