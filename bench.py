@@ -246,8 +246,9 @@ def adjoint_timing(C, grid, vion, res, cells, args):
 
 
 def profile_passes(C, cr, grid, vion, K):
-    """Warm passes at the guess potential (uniform density), both precisions, then a
-    256 MiB device-to-device copy whose FETCH_SIZE / WRITE_SIZE calibrate the counters."""
+    """Warm passes at the guess potential (uniform density), both precisions, each with
+    its H(v) assembly and its density, then a 256 MiB device-to-device copy whose
+    FETCH_SIZE / WRITE_SIZE calibrate the byte counters."""
     import torch
     d0 = torch.full(cr.shape, cr.n_elec / (cr.ng * cr.dV), dtype=torch.float64, device="cuda")
     v = grid.potential(d0, vion)
@@ -257,7 +258,9 @@ def profile_passes(C, cr, grid, vion, K):
     for prec in ("mixed", "fp64"):
         C.set_precision(prec)
         for _ in range(K + 1):          # the first pass also runs the autotuner
+            C.plan.set_potential(v)
             C.plan.run(mu, e_lo, want_density=False)
+            C.plan.density_last()
     src = torch.ones(32 * 2 ** 20, dtype=torch.float64, device="cuda")
     dst = torch.empty_like(src)
     for _ in range(3):

@@ -212,7 +212,9 @@ int lres_mu_solve(lres_plan* plan, double N, double e_lo, double lo, double hi, 
  * there, f = 2 sum N_B - N), secant steps with the last known slope (slope = 0:
  * steps of 1e-3 towards the root), each at most 0.25 and with 20% overshoot once
  * two points are on the same side, until |f| <= ftol or the root is bracketed,
- * then Brent as lres_mu_solve (xtol, ftol); at most 60 secant steps.  No
+ * then Brent (xtol, ftol) on f, or, when f(hi) > 10 |f(lo)|, on log(1 + f / A),
+ * A = 1.25 |f(lo)| (the count grows exponentially towards the conduction band; the
+ * transform keeps the root and straightens such a bracket); at most 60 secant steps.  No
  * counterpart in the reference (its mu search runs on stored eigenvalues); the same
  * steps as the host-side search it replaces, one call instead of one per pass.
  * h_info[8]: passes run, f at the returned mu, 1 if the last pass was at the
@@ -279,6 +281,34 @@ int lres_pass_blocks(const lres_plan* plan, double* d_D, double* d_F);
  * (its 3D path has no forces, README.md:35).
  * Allocates workspaces of up to 2 x 4 GiB (windows are processed in chunks). */
 int lres_adjoint(lres_plan* plan, double mu, double e_lo, double alpha, double beta, double* d_gH, double* d_gS);
+
+/* Per-geometry set-up of the device force contraction: the basis functions of every
+ * atom (the Gaussians s_exps[0..ns) then x, y, z per p exponent, times
+ * (1 - (r/rcut)^2)^2, normalised by the grid quadrature: LCAOBasis,
+ * lrft/lcao3d.py:49-100) and their derivatives with respect to the atom's position,
+ * tabulated on a dense box of (2 ceil(rcut/dx) + 1)^3 grid points around round(R/dx)
+ * (h_pos natom x 3, h_box the cell lengths, h_shape the grid); the atom pairs whose
+ * boxes meet, with their entries in the plan's window blocks; the owning window of
+ * every grid point.  Functions must be atom-major with ns + 3 np (<= 8) per atom.
+ * Device memory: 8 (5 nf) (2 ceil(rcut/dx) + 1)^3 bytes per atom.  No counterpart in
+ * the reference (its 3D path has no forces, README.md:35). */
+int lres_force_setup(lres_plan* plan, const double* h_pos, int64_t natom, const int64_t* h_atom_of_fn, int64_t nb,
+                     double rcut, const double* h_exps, int ns, int np, const double* h_box, const int64_t* h_shape,
+                     double dx);
+
+/* Band-energy forces F_A = -dE/dR_A (d_forces, natom x 3 device doubles) from the
+ * window step's gradients: d_gH, d_gS the lres_adjoint output (layout of
+ * lres_plan_layout) for alpha = 1 and the given beta, d_D, d_F the lres_pass_blocks
+ * output of the pass at the same (mu, e_lo), d_v the grid potential of H (ng device
+ * doubles).  With the spin factor 2, G^H = 2 sum_W d_gH, G^S = 2 sum_W d_gS and
+ * G^Q_B = 2 (F_B + beta D_B):
+ *   dE/dR_A = 2 dV sum_{c on A} sum_g dx_c/dR_A [ sum_b G^H_cb ((T + v) x_b)
+ *                                                 + G^S_cb x_b + sum_u G^Q_{B(g),cu} x_u ](g)
+ * (H = X^T (T + diag v) X dV, S = X^T X dV, Q_B = sum_{g in own B} x_g x_g^T dV, T the
+ * 7-point kinetic operator of three_d.py:151-164).  Fixed-order sums: bitwise
+ * reproducible.  Kernel launches only, on the plan's stream. */
+int lres_band_forces(lres_plan* plan, const double* d_gH, const double* d_gS, const double* d_D, const double* d_F,
+                     double beta, const double* d_v, double* d_forces);
 
 #ifdef __cplusplus
 }

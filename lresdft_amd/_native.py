@@ -68,6 +68,9 @@ SIGNATURES = {
     "lres_get_window_blocks": (C.c_int, [_vp, _vp, _vp]),
     "lres_adjoint": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_double, _vp, _vp]),
     "lres_pass_blocks": (C.c_int, [_vp, _vp, _vp]),
+    "lres_force_setup": (C.c_int, [_vp, _f64p, C.c_int64, _i64p, C.c_int64, C.c_double, _f64p, C.c_int, C.c_int,
+                                   _f64p, _i64p, C.c_double]),
+    "lres_band_forces": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_double, _vp, _vp]),
 }
 
 _lib = None

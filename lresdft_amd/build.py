@@ -12,7 +12,7 @@ import tempfile
 from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SOURCES = ["csrc/lres_kernels.hip", "csrc/lres_plan.hip"]
+SOURCES = ["csrc/lres_kernels.hip", "csrc/lres_plan.hip", "csrc/lres_forces.hip"]
 MFMA_SOURCE = "csrc/lres_mfma.hip"
 MFMA_PARTS = [("float", nt) for nt in range(2, 17)] + [("double", nt) for nt in range(2, 9)]
 LIB = os.path.join(HERE, "liblres_hip.so")

@@ -104,3 +104,20 @@ int mfma_scratch_bytes(int fp32, int nt, int nwc);
 hipError_t launch_adjoint(int bmax, int nmax, const lres::WinDesc* wins, const int32_t* ids, const int64_t* woff,
                           int nids, const double* SW, const double* HW, const double* Q, int M, double c, double Rr,
                           double alpha, double beta, void* Gw, void* Zw, double* gH, double* gS, hipStream_t st);
+// Forces on the device (lres_forces.hip): the basis on per-atom boxes, the gradient
+// blocks of the force pairs, the pair and Q parts and the per-atom totals.
+hipError_t launch_atom_boxes(int natom, const double* pos, const int64_t* origin, int nbox, double dx,
+                             const double* L, double rcut, const double* exps, int ns, int np, double dV, double* X,
+                             double* DX, double* TX, hipStream_t st);
+hipError_t launch_force_gather(int npair, int nf, const int64_t* ent_ptr, const int64_t* ent_off,
+                               const int32_t* ent_n, const double* gH, const double* gS, double* GH, double* GS,
+                               hipStream_t st);
+hipError_t launch_force_pairs(int npair, int nf, const int32_t* pA, const int32_t* pB, const int64_t* origin, int nbox,
+                              const int64_t* shape, const double* X, const double* DX, const double* TX,
+                              const double* v, const double* GH, const double* GS, double* out, hipStream_t st);
+hipError_t launch_force_q(int natom, int nf, const int64_t* origin, int nbox, const int64_t* shape, const double* X,
+                          const double* DX, const int32_t* owner, const int32_t* row_of, const int32_t* tpos,
+                          const lres::WinDesc* wins, const int64_t* xptr, const int32_t* xidx, const double* xval,
+                          const double* D, const double* F, double beta, double* out, hipStream_t st);
+hipError_t launch_force_total(int natom, const int64_t* aptr, const int64_t* aent, const double* fpair,
+                              const double* fq, double dV, double* F, hipStream_t st);

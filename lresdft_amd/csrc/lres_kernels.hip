@@ -1318,8 +1318,16 @@ __device__ int big_block_step(E* __restrict__ A, int n, int p0, int b, E* Cl, E*
 // pivot block), the pivot columns are the pivot rows with the sign flipped on the
 // pivoted rows, and the update touches i <= j only (half the workspace traffic).
 // On return the upper triangle holds that of the Gauss-Jordan state.
+//
+// Only rows and columns >= lo are read and written.  The caller needs the block
+// [m, n) x [m, n) of the inverse (the T-last window order puts T_B there): while the
+// pivots lie below m, lo = p0 makes the step a right-looking elimination of the
+// trailing block (after the pivots of D = [0, m) it holds the Schur complement
+// A_TT - A_TD A_DD^-1 A_DT, and rows / columns of D are never read again); from m on,
+// lo = m runs Gauss-Jordan on that k x k block, whose in-place result is its inverse,
+// (A^-1)[T,T].  With lo = 0 this is the whole-matrix sweep.
 template <typename E>
-__device__ void big_block_step_sym(E* __restrict__ A, int n, int p0, int b, E* Cl, E* Rl, E* Pl) {
+__device__ void big_block_step_sym(E* __restrict__ A, int n, int p0, int b, E* Cl, E* Rl, E* Pl, int lo) {
     using Fo = Fld<E>;
     const int tid = threadIdx.x, nt = blockDim.x;
     // pivot rows: right of the diagonal from the rows themselves (coalesced along j),
@@ -1330,7 +1338,7 @@ __device__ void big_block_step_sym(E* __restrict__ A, int n, int p0, int b, E* C
     }
     for (int idx = tid; idx < (p0 + b) * b; idx += nt) {
         const int j = idx / b, q = idx - (idx / b) * b;
-        if (j < p0 + q) {
+        if (j >= lo && j < p0 + q) {
             const E v = A[(int64_t)j * n + p0 + q];
             Rl[q * n + j] = j < p0 ? Fo::mul(v, Fo::add_one(Fo::zero(), -1)) : v;
         }
@@ -1339,6 +1347,7 @@ __device__ void big_block_step_sym(E* __restrict__ A, int n, int p0, int b, E* C
     // pivot columns: the pivot rows with the sign flipped on the pivoted rows
     for (int idx = tid; idx < n * b; idx += nt) {
         const int i = idx / b, q = idx - (idx / b) * b;
+        if (i < lo) continue;
         const E v = Rl[q * n + i];
         Cl[idx] = i < p0 ? Fo::mul(v, Fo::add_one(Fo::zero(), -1)) : v;
     }
@@ -1364,7 +1373,7 @@ __device__ void big_block_step_sym(E* __restrict__ A, int n, int p0, int b, E* C
     }
     __syncthreads();
     // R~ = Pi (R + [I on P]) in place, one column per thread; C~ = C - [I on P]
-    for (int j = tid; j < n; j += nt) {
+    for (int j = lo + tid; j < n; j += nt) {
         E v[16];
         for (int q = 0; q < b; ++q) v[q] = (j - p0 == q) ? Fo::add_one(Rl[q * n + j], 1) : Rl[q * n + j];
         for (int q = 0; q < b; ++q) {
@@ -1375,10 +1384,11 @@ __device__ void big_block_step_sym(E* __restrict__ A, int n, int p0, int b, E* C
     }
     for (int q = tid; q < b; q += nt) Cl[(p0 + q) * b + q] = Fo::add_one(Cl[(p0 + q) * b + q], -1);
     __syncthreads();
-    // A -= C~ R~ on i <= j: each thread four rows of one column per round
-    const int rows4 = (n + 3) / 4;
-    for (int idx = tid; idx < rows4 * n; idx += nt) {
-        const int i0 = (idx / n) * 4, j = idx - (idx / n) * n;
+    // A -= C~ R~ on lo <= i <= j: each thread four rows of one column per round
+    const int nl = n - lo;
+    const int rows4 = (nl + 3) / 4;
+    for (int idx = tid; idx < rows4 * nl; idx += nt) {
+        const int i0 = lo + (idx / nl) * 4, j = lo + idx - (idx / nl) * nl;
         if (i0 > j) continue;
         E a[4];
 #pragma unroll
@@ -1427,7 +1437,16 @@ __global__ void __launch_bounds__(kBigThreads) k_pole_big(const WinDesc* __restr
             A[q] = {(R)(z.re * S[q] - H[q]), (R)(z.im * S[q])};
         }
         __syncthreads();
-        for (int p0 = 0; p0 < n; p0 += bmax) big_block_step_sym<E>(A, n, p0, min(bmax, n - p0), Cl, Rl, Pl);
+        // pivots below m: elimination of the trailing block; from m on: Gauss-Jordan on T x T
+        for (int p0 = 0; p0 < n; p0 += bmax) {
+            const int bb = min(bmax, n - p0);
+            if (p0 < m && p0 + bb > m) {   // a block straddling m: split it there
+                big_block_step_sym<E>(A, n, p0, m - p0, Cl, Rl, Pl, p0);
+                big_block_step_sym<E>(A, n, m, p0 + bb - m, Cl, Rl, Pl, m);
+            } else {
+                big_block_step_sym<E>(A, n, p0, bb, Cl, Rl, Pl, p0 < m ? p0 : m);
+            }
+        }
         for (int q = threadIdx.x; q < k * k; q += blockDim.x) {
             const int r = q / k, col = q - (q / k) * k;
             if (r > col) continue;   // upper triangle (the partial sums are symmetric)

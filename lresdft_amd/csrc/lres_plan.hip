@@ -203,6 +203,15 @@ struct lres_plan {
     DevBuf<int64_t> d_pent_off, d_xat_off, d_vp_off;
     DevBuf<double> d_xat, d_vpair, d_TW;
     bool have_H = false;
+    // forces (lres_force_setup / lres_band_forces): the basis on per-atom boxes, the
+    // atom pairs whose boxes meet with their window entries, grid-point owners
+    bool have_forces = false;
+    int f_nf = 0, f_nbox = 0;
+    int64_t f_natom = 0, f_npair = 0;
+    int64_t f_shape[3] = {0, 0, 0};
+    DevBuf<int64_t> d_forigin, d_fent_ptr, d_fent_off, d_faptr, d_faent;
+    DevBuf<int32_t> d_fpA, d_fpB, d_fent_n, d_fowner, d_frow, d_ftpos;
+    DevBuf<double> d_fpos, d_fexps, d_fL, d_fX, d_fDX, d_fTX, d_fGH, d_fGS, d_fpair, d_fq;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double timing[3] = {0, 0, 0};
     // cumulative pole-kernel statistics per precision: passes and milliseconds (HIP events)
@@ -1345,8 +1354,21 @@ int lres_mu_solve_warm(lres_plan* P, double N, double e_lo, double mu0, double f
             if ((fa < 0) != (fb < 0)) {
                 const bool lo_a = a < b;
                 const double lo = lo_a ? a : b, hi = lo_a ? b : a, flo = lo_a ? fa : fb, fhi = lo_a ? fb : fa;
+                // where the bracket's high end lies far up the count's exponential rise
+                // towards the conduction band (f(hi) > 10 |f(lo)|, e.g. f in [-0.06, +400]),
+                // interpolation on f creeps along the low end: Brent then runs on
+                // g = log(1 + f / A), A = 1.25 |f(lo)|, which has the root and monotonicity
+                // of f (f >= f(lo) on the bracket keeps 1 + f / A > 0) and is nearly linear
+                // there; on a nearly linear bracket Brent runs on f itself
                 int bevals = 0;
-                mu = brent_root(f, lo, hi, flo, fhi, xtol, ftol, 200, &bevals);
+                if (fhi > 10.0 * std::fabs(flo)) {
+                    const double Ab = 1.25 * std::fabs(flo);
+                    auto g = [&](double m) { return std::log1p(f(m) / Ab); };
+                    const double gtol = std::log1p(ftol / Ab);    // |g| <= gtol implies |f| <= ftol
+                    mu = brent_root(g, lo, hi, std::log1p(flo / Ab), std::log1p(fhi / Ab), xtol, gtol, 200, &bevals);
+                } else {
+                    mu = brent_root(f, lo, hi, flo, fhi, xtol, ftol, 200, &bevals);
+                }
                 found = by_brent = true;
                 break;
             }
@@ -1487,6 +1509,171 @@ int lres_adjoint(lres_plan* P, double mu, double e_lo, double alpha, double beta
             hip_check(hipStreamSynchronize(P->st), "sync");   // the id / offset lists are reused by the next chunk
             w0 = w;
         }
+    });
+}
+
+int lres_force_setup(lres_plan* P, const double* h_pos, int64_t natom, const int64_t* h_aof, int64_t nb, double rcut,
+                     const double* h_exps, int ns, int np, const double* h_box, const int64_t* h_shape, double dx) {
+    return guarded([&] {
+        require(P != nullptr, "null plan");
+        require(natom > 0 && nb == P->nb, "basis size differs from the plan's");
+        const int nf = ns + 3 * np;
+        require(nf >= 1 && nf <= 8, "1..8 functions per atom");
+        require(nb == natom * nf, "every atom must carry the same functions");
+        for (int64_t f = 0; f < nb; ++f) require(h_aof[f] == f / nf, "functions must be atom-major (atom_of_fn)");
+        const int rad = (int)std::ceil(rcut / dx);
+        const int nbox = 2 * rad + 1;
+        for (int a = 0; a < 3; ++a) require(h_shape[a] > nbox, "grid shorter than an atom's box along an axis");
+        require((int64_t)h_shape[0] * h_shape[1] * h_shape[2] == P->ng, "grid size differs from the plan's");
+        P->f_nf = nf;
+        P->f_nbox = nbox;
+        P->f_natom = natom;
+        for (int a = 0; a < 3; ++a) P->f_shape[a] = h_shape[a];
+        // box origins: round(R / dx) - rad, as the basis places its functions
+        std::vector<int64_t> org(3 * natom);
+        for (int64_t A = 0; A < natom; ++A)
+            for (int a = 0; a < 3; ++a) org[3 * A + a] = (int64_t)std::llrint(h_pos[3 * A + a] / dx) - rad;
+        auto overlap = [&](int64_t A, int64_t B) {
+            for (int a = 0; a < 3; ++a) {
+                const int64_t n = h_shape[a];
+                int64_t d = (org[3 * B + a] - org[3 * A + a]) % n;
+                if (d < 0) d += n;
+                if (!(d < nbox || n - d < nbox)) return false;
+            }
+            return true;
+        };
+        // force pairs (A <= B, boxes meeting, both in some window) and their window entries
+        std::unordered_map<int64_t, int32_t> pid;
+        std::vector<int32_t> pA, pB;
+        std::vector<std::vector<std::pair<int64_t, int32_t>>> ents;
+        int64_t wo = 0;
+        for (int64_t w = 0; w < P->nwin; ++w) {
+            const int n = P->n_of[w];
+            std::vector<std::pair<int64_t, int>> at;   // (atom, first position)
+            for (int p = 0; p < n; p += nf) {
+                const int64_t fn = P->fn_at[wo + p];
+                require(fn % nf == 0 && P->fn_at[wo + p + nf - 1] == fn + nf - 1,
+                        "window layout does not keep an atom's functions together");
+                at.push_back({fn / nf, p});
+            }
+            for (const auto& x : at)
+                for (const auto& y : at) {
+                    if (x.first > y.first || !overlap(x.first, y.first)) continue;
+                    const int64_t key = x.first * natom + y.first;
+                    auto it = pid.find(key);
+                    int32_t id;
+                    if (it == pid.end()) {
+                        id = (int32_t)pA.size();
+                        pid.emplace(key, id);
+                        pA.push_back((int32_t)x.first);
+                        pB.push_back((int32_t)y.first);
+                        ents.emplace_back();
+                    } else {
+                        id = it->second;
+                    }
+                    ents[id].push_back({P->wins[w].mat_off + (int64_t)x.second * n + y.second, n});
+                }
+            wo += n;
+        }
+        const int64_t npair = (int64_t)pA.size();
+        std::vector<int64_t> eptr(npair + 1, 0), eoff;
+        std::vector<int32_t> en;
+        for (int64_t q = 0; q < npair; ++q) {
+            for (const auto& e : ents[q]) {
+                eoff.push_back(e.first);
+                en.push_back(e.second);
+            }
+            eptr[q + 1] = (int64_t)eoff.size();
+        }
+        // per atom: its pair parts in pair order (offset 6 pair + 3 side into the pair output)
+        std::vector<std::vector<int64_t>> per(natom);
+        for (int64_t q = 0; q < npair; ++q) {
+            per[pA[q]].push_back(6 * q);
+            if (pB[q] != pA[q]) per[pB[q]].push_back(6 * q + 3);
+        }
+        std::vector<int64_t> aptr(natom + 1, 0), aent;
+        for (int64_t A = 0; A < natom; ++A) {
+            aent.insert(aent.end(), per[A].begin(), per[A].end());
+            aptr[A + 1] = (int64_t)aent.size();
+        }
+        // grid owners and owned-CSR rows; T-local position of each atom per window
+        std::vector<int64_t> own_g(P->ng);
+        hip_check(hipMemcpy(own_g.data(), P->d_own_g.p, P->ng * sizeof(int64_t), hipMemcpyDeviceToHost), "copy");
+        std::vector<int32_t> owner(P->ng, -1), row(P->ng, -1);
+        for (int64_t w = 0; w < P->nwin; ++w)
+            for (int32_t r = 0; r < P->wins[w].nown; ++r) {
+                const int64_t g = own_g[P->wins[w].own_off + r];
+                owner[g] = (int32_t)w;
+                row[g] = (int32_t)(P->wins[w].own_off + r);
+            }
+        for (int64_t g = 0; g < P->ng; ++g) require(owner[g] >= 0, "grid point owned by no window");
+        std::vector<int32_t> tpos((size_t)P->nwin * natom, -1);
+        wo = 0;
+        for (int64_t w = 0; w < P->nwin; ++w) {
+            const int n = P->n_of[w], k = P->k_of[w];
+            for (int p = n - k; p < n; p += nf) tpos[(size_t)w * natom + P->fn_at[wo + p] / nf] = p - (n - k);
+            wo += n;
+        }
+        auto up64 = [&](DevBuf<int64_t>& b, const std::vector<int64_t>& h) {
+            b.upload(h.empty() ? nullptr : h.data(), h.size(), P->st);
+            if (h.empty()) b.alloc(1);
+        };
+        auto up32 = [&](DevBuf<int32_t>& b, const std::vector<int32_t>& h) {
+            b.upload(h.empty() ? nullptr : h.data(), h.size(), P->st);
+            if (h.empty()) b.alloc(1);
+        };
+        up64(P->d_forigin, org);
+        up64(P->d_fent_ptr, eptr);
+        up64(P->d_fent_off, eoff);
+        up64(P->d_faptr, aptr);
+        up64(P->d_faent, aent);
+        up32(P->d_fpA, pA);
+        up32(P->d_fpB, pB);
+        up32(P->d_fent_n, en);
+        up32(P->d_fowner, owner);
+        up32(P->d_frow, row);
+        up32(P->d_ftpos, tpos);
+        P->d_fpos.upload(h_pos, 3 * natom, P->st);
+        P->d_fexps.upload(h_exps, ns + np, P->st);
+        P->d_fL.upload(h_box, 3, P->st);
+        const size_t nb3 = (size_t)nbox * nbox * nbox;
+        P->d_fX.alloc((size_t)natom * nf * nb3);
+        P->d_fDX.alloc((size_t)natom * 3 * nf * nb3);
+        P->d_fTX.alloc((size_t)natom * nf * nb3);
+        P->d_fGH.alloc(std::max<size_t>(1, (size_t)npair * nf * nf));
+        P->d_fGS.alloc(std::max<size_t>(1, (size_t)npair * nf * nf));
+        P->d_fpair.alloc(std::max<size_t>(1, (size_t)npair * 6));
+        P->d_fq.alloc((size_t)natom * 3);
+        P->f_npair = npair;
+        hip_check(launch_atom_boxes((int)natom, P->d_fpos.p, P->d_forigin.p, nbox, dx, h_box, rcut, P->d_fexps.p, ns,
+                                   np, P->dV, P->d_fX.p, P->d_fDX.p, P->d_fTX.p, P->st),
+                  "basis box kernel");
+        hip_check(hipStreamSynchronize(P->st), "sync");   // host tables above are released on return
+        P->have_forces = true;
+    });
+}
+
+int lres_band_forces(lres_plan* P, const double* d_gH, const double* d_gS, const double* d_D, const double* d_F,
+                     double beta, const double* d_v, double* d_forces) {
+    return guarded([&] {
+        require(P != nullptr, "null plan");
+        require(P->have_forces, "no force set-up: call lres_force_setup first");
+        require(d_gH && d_gS && d_D && d_F && d_v && d_forces, "null buffer");
+        const int nf = P->f_nf;
+        hip_check(launch_force_gather((int)P->f_npair, nf, P->d_fent_ptr.p, P->d_fent_off.p, P->d_fent_n.p, d_gH,
+                                      d_gS, P->d_fGH.p, P->d_fGS.p, P->st),
+                  "force gather kernel");
+        hip_check(launch_force_pairs((int)P->f_npair, nf, P->d_fpA.p, P->d_fpB.p, P->d_forigin.p, P->f_nbox,
+                                     P->f_shape, P->d_fX.p, P->d_fDX.p, P->d_fTX.p, d_v, P->d_fGH.p, P->d_fGS.p,
+                                     P->d_fpair.p, P->st),
+                  "force pair kernel");
+        hip_check(launch_force_q((int)P->f_natom, nf, P->d_forigin.p, P->f_nbox, P->f_shape, P->d_fX.p, P->d_fDX.p,
+                                 P->d_fowner.p, P->d_frow.p, P->d_ftpos.p, P->d_wins.p, P->d_xptr.p, P->d_xidx.p,
+                                 P->d_xval.p, d_D, d_F, beta, P->d_fq.p, P->st),
+                  "force Q kernel");
+        hip_check(launch_force_total((int)P->f_natom, P->d_faptr.p, P->d_faent.p, P->d_fpair.p, P->d_fq.p, P->dV,
+                                     d_forces, P->st),
+                  "force total kernel");
     });
 }
 
