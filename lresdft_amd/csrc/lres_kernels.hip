@@ -1246,7 +1246,7 @@ template <typename R> struct Fld<cx<R>> {
 // LDS: Cl (n x b), Rl (b x n), Pl (b x b).  Returns (in thread 0) the number of
 // positive scalar pivots met inside the pivot block (used by the inertia count).
 template <typename E>
-__device__ int big_block_step(E* __restrict__ A, int n, int p0, int b, E* Cl, E* Rl, E* Pl) {
+__device__ int big_block_step(E* __restrict__ A, int n, int p0, int b, E* Cl, E* Rl, E* Pl, int lo = 0) {
     using Fo = Fld<E>;
     const int tid = threadIdx.x, nt = blockDim.x;
     for (int idx = tid; idx < n * b; idx += nt) {
@@ -1278,7 +1278,7 @@ __device__ int big_block_step(E* __restrict__ A, int n, int p0, int b, E* Cl, E*
     }
     __syncthreads();
     // R~ = Pi (R + [I on P]) in place, one column per thread; C~ = C - [I on P]
-    for (int j = tid; j < n; j += nt) {
+    for (int j = lo + tid; j < n; j += nt) {
         E v[32];
         for (int q = 0; q < b; ++q) v[q] = (j - p0 == q) ? Fo::add_one(Rl[q * n + j], 1) : Rl[q * n + j];
         for (int q = 0; q < b; ++q) {
@@ -1289,10 +1289,14 @@ __device__ int big_block_step(E* __restrict__ A, int n, int p0, int b, E* Cl, E*
     }
     for (int q = tid; q < b; q += nt) Cl[(p0 + q) * b + q] = Fo::add_one(Cl[(p0 + q) * b + q], -1);
     __syncthreads();
-    // A -= C~ R~ : each thread four rows of one column per round (row-major, coalesced in j)
-    const int rows4 = (n + 3) / 4;
-    for (int idx = tid; idx < rows4 * n; idx += nt) {
-        const int i0 = (idx / n) * 4, j = idx - (idx / n) * n;
+    // A -= C~ R~ on rows and columns >= lo: each thread four rows of one column per round
+    // (row-major, coalesced in j).  lo = 0: the Gauss-Jordan step of the whole matrix;
+    // lo = p0: a right-looking elimination step (only the trailing Schur complement is
+    // kept current, which is all that the inertia count reads)
+    const int nl = n - lo;
+    const int rows4 = (nl + 3) / 4;
+    for (int idx = tid; idx < rows4 * nl; idx += nt) {
+        const int i0 = lo + (idx / nl) * 4, j = lo + idx - (idx / nl) * nl;
         E a[4];
 #pragma unroll
         for (int s = 0; s < 4; ++s) a[s] = (i0 + s < n) ? A[(int64_t)(i0 + s) * n + j] : Fo::zero();
@@ -1308,6 +1312,105 @@ __device__ int big_block_step(E* __restrict__ A, int n, int p0, int b, E* Cl, E*
     }
     __syncthreads();
     return npos;
+}
+
+// A -= C~ R~ on lo <= i <= j (C~ stored transposed, Cl[q n + i]; R~ in Rl[q n + j]).
+// complex64: the vector ALU, each thread four rows of one column per round.
+__device__ void big_update_sym(cx<float>* __restrict__ A, int n, int lo, int b, const cx<float>* Cl,
+                               const cx<float>* Rl) {
+    using Fo = Fld<cx<float>>;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int nl = n - lo;
+    const int rows4 = (nl + 3) / 4;
+    for (int idx = tid; idx < rows4 * nl; idx += nt) {
+        const int i0 = lo + (idx / nl) * 4, j = lo + idx - (idx / nl) * nl;
+        if (i0 > j) continue;
+        cx<float> a[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) a[s] = (i0 + s <= j && i0 + s < n) ? A[(int64_t)(i0 + s) * n + j] : Fo::zero();
+        for (int q = 0; q < b; ++q) {
+            const cx<float> r = Rl[q * n + j];
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+                if (i0 + s <= j && i0 + s < n) a[s] = Fo::fms(a[s], Cl[q * n + i0 + s], r);
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+            if (i0 + s <= j && i0 + s < n) A[(int64_t)(i0 + s) * n + j] = a[s];
+    }
+}
+
+// complex128: on the matrix cores, one 16 x 16 tile (I <= J) of the upper triangle per
+// wave at a time, four v_mfma_f64_16x16x4f64 per complex product and k-step of four
+// panel columns.  Layouts (gfx950): accumulator lane l, register r -> row l / 16 + 4 r,
+// column l % 16; A operand lane l -> A[l % 16][l / 16]; B operand lane l -> B[l / 16][l % 16].
+// Operand rows / columns beyond n read as zero; rows or columns below lo (not kept in
+// the panels) only reach output entries below lo, which are not stored.
+__device__ void big_update_sym(cx<double>* __restrict__ A, int n, int lo, int b, const cx<double>* Cl,
+                               const cx<double>* Rl) {
+    typedef double V4 __attribute__((ext_vector_type(4)));
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int li = lane & 15, lg = lane >> 4;
+    const int t0 = lo >> 4, ntile = ((n + 15) >> 4) - t0;
+    const int tiles = ntile * (ntile + 1) / 2;
+    const int nk = (b + 3) >> 2;
+    // tile t -> (I, J), I <= J, row-major over the upper triangle of the tile grid
+    auto tile_of = [&](int t, int& r0, int& c0) {
+        int I = 0, rem = t;
+        while (rem >= ntile - I) {
+            rem -= ntile - I;
+            ++I;
+        }
+        r0 = 16 * (t0 + I);
+        c0 = 16 * (t0 + I + rem);
+    };
+    auto load = [&](int r0, int c0, V4& re, V4& im) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = r0 + lg + 4 * r, col = c0 + li;
+            const bool ok = row >= lo && row <= col && col < n;
+            const cx<double> a = ok ? A[(int64_t)row * n + col] : cx<double>{0.0, 0.0};
+            re[r] = a.re;
+            im[r] = a.im;
+        }
+    };
+    int t = wv;
+    if (t >= tiles) return;
+    int r0, c0;
+    tile_of(t, r0, c0);
+    V4 are, aim;
+    load(r0, c0, are, aim);
+    for (; t < tiles; t += nw) {
+        // the next tile's entries are loaded while this tile's products run
+        int r1 = 0, c1 = 0;
+        V4 bre = {0.0, 0.0, 0.0, 0.0}, bim = {0.0, 0.0, 0.0, 0.0};
+        const bool more = t + nw < tiles;
+        if (more) {
+            tile_of(t + nw, r1, c1);
+            load(r1, c1, bre, bim);
+        }
+        for (int kk = 0; kk < nk; ++kk) {
+            const int q = 4 * kk + lg;
+            const int ra = r0 + li, cb = c0 + li;
+            const cx<double> ca = (q < b && ra < n) ? Cl[q * n + ra] : cx<double>{0.0, 0.0};
+            const cx<double> rb = (q < b && cb < n) ? Rl[q * n + cb] : cx<double>{0.0, 0.0};
+            // A -= C R:  re += (-Cr) Rr + Ci Ri,  im += (-Cr) Ri + (-Ci) Rr
+            are = __builtin_amdgcn_mfma_f64_16x16x4f64(-ca.re, rb.re, are, 0, 0, 0);
+            are = __builtin_amdgcn_mfma_f64_16x16x4f64(ca.im, rb.im, are, 0, 0, 0);
+            aim = __builtin_amdgcn_mfma_f64_16x16x4f64(-ca.re, rb.im, aim, 0, 0, 0);
+            aim = __builtin_amdgcn_mfma_f64_16x16x4f64(-ca.im, rb.re, aim, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = r0 + lg + 4 * r, col = c0 + li;
+            if (row >= lo && row <= col && col < n) A[(int64_t)row * n + col] = cx<double>{are[r], aim[r]};
+        }
+        if (!more) break;
+        r0 = r1;
+        c0 = c1;
+        are = bre;
+        aim = bim;
+    }
 }
 
 // big_block_step on the upper triangle only, for complex-symmetric A (the pole
@@ -1344,9 +1447,11 @@ __device__ void big_block_step_sym(E* __restrict__ A, int n, int p0, int b, E* C
         }
     }
     __syncthreads();
-    // pivot columns: the pivot rows with the sign flipped on the pivoted rows
+    // pivot columns (stored transposed, Cl[q n + i]: consecutive rows at consecutive
+    // addresses for the matrix-core operands): the pivot rows with the sign flipped on
+    // the pivoted rows
     for (int idx = tid; idx < n * b; idx += nt) {
-        const int i = idx / b, q = idx - (idx / b) * b;
+        const int q = idx / n, i = idx - (idx / n) * n;
         if (i < lo) continue;
         const E v = Rl[q * n + i];
         Cl[idx] = i < p0 ? Fo::mul(v, Fo::add_one(Fo::zero(), -1)) : v;
@@ -1382,27 +1487,9 @@ __device__ void big_block_step_sym(E* __restrict__ A, int n, int p0, int b, E* C
             Rl[q * n + j] = acc;
         }
     }
-    for (int q = tid; q < b; q += nt) Cl[(p0 + q) * b + q] = Fo::add_one(Cl[(p0 + q) * b + q], -1);
+    for (int q = tid; q < b; q += nt) Cl[q * n + p0 + q] = Fo::add_one(Cl[q * n + p0 + q], -1);
     __syncthreads();
-    // A -= C~ R~ on lo <= i <= j: each thread four rows of one column per round
-    const int nl = n - lo;
-    const int rows4 = (nl + 3) / 4;
-    for (int idx = tid; idx < rows4 * nl; idx += nt) {
-        const int i0 = lo + (idx / nl) * 4, j = lo + idx - (idx / nl) * nl;
-        if (i0 > j) continue;
-        E a[4];
-#pragma unroll
-        for (int s = 0; s < 4; ++s) a[s] = (i0 + s <= j && i0 + s < n) ? A[(int64_t)(i0 + s) * n + j] : Fo::zero();
-        for (int q = 0; q < b; ++q) {
-            const E r = Rl[q * n + j];
-#pragma unroll
-            for (int s = 0; s < 4; ++s)
-                if (i0 + s <= j && i0 + s < n) a[s] = Fo::fms(a[s], Cl[(i0 + s) * b + q], r);
-        }
-#pragma unroll
-        for (int s = 0; s < 4; ++s)
-            if (i0 + s <= j && i0 + s < n) A[(int64_t)(i0 + s) * n + j] = a[s];
-    }
+    big_update_sym(A, n, lo, b, Cl, Rl);
     __syncthreads();
 }
 
@@ -1488,7 +1575,10 @@ __global__ void __launch_bounds__(kBigThreads) k_inertia_big(const WinDesc* __re
         for (int64_t q = threadIdx.x; q < (int64_t)n * n; q += blockDim.x) A[q] = sg * S[q] - H[q];
         __syncthreads();
         int npos = 0;
-        for (int p0 = 0; p0 < n; p0 += bmax) npos += big_block_step<double>(A, n, p0, min(bmax, n - p0), Cl, Rl, Pl);
+        // right-looking elimination: the pivots of each block are those of the unpivoted
+        // LDL^T, and only the trailing block is updated
+        for (int p0 = 0; p0 < n; p0 += bmax)
+            npos += big_block_step<double>(A, n, p0, min(bmax, n - p0), Cl, Rl, Pl, p0);
         if (threadIdx.x == 0) {
             if (counts) counts[(int64_t)w * nshift + s_id] = npos;
             atomicAdd(&total[s_id], (unsigned long long)npos);
