@@ -101,6 +101,8 @@ def kernel_name(bucket, fp32):
     T, G, kind = bucket[5:8] if fp32 else bucket[2:5]
     if kind == 2:
         return f"k_pole_mfma<{'float' if fp32 else 'double'}, tiles {T}, waves {G}>"
+    if T == 0:
+        return f"k_pole_big<{'float' if fp32 else 'double'}, 1024 threads, panel 16>"
     return f"k_pole_gj<T {T}, {'float' if fp32 else 'double'}, grid {G}x{G}>"
 
 

@@ -1314,6 +1314,118 @@ __device__ int big_block_step(E* __restrict__ A, int n, int p0, int b, E* Cl, E*
     return npos;
 }
 
+// Pi = A_PP^-1 of the b x b pivot block (rows p0.. of the row panel Rl, columns p0..)
+// into Pl (b x b, row-major), by the scalar shifted-pivot sweep
+// a_ij -= (a_ip - d_ip)(a_pj + d_pj) / a_pp, which leaves the in-place Gauss-Jordan
+// result.  complex64: wave 0 through LDS.
+__device__ void big_pivot_inverse(const cx<float>* Rl, cx<float>* Pl, int n, int p0, int b) {
+    using Fo = Fld<cx<float>>;
+    const int tid = threadIdx.x;
+    if (tid >= 64) return;
+    for (int e = tid; e < b * b; e += 64) Pl[e] = Rl[(e / b) * n + p0 + (e - (e / b) * b)];
+    __builtin_amdgcn_wave_barrier();
+    for (int p = 0; p < b; ++p) {
+        const cx<float> piv = Pl[p * b + p];
+        const cx<float> inv = Fo::recip(piv);
+        cx<float> colv[4], rowv[4];
+        int cnt = 0;
+        for (int e = tid; e < b * b; e += 64, ++cnt) {
+            const int i = e / b, j = e - (e / b) * b;
+            colv[cnt] = (i == p) ? Fo::add_one(piv, -1) : Pl[i * b + p];
+            rowv[cnt] = Fo::mul((j == p) ? Fo::add_one(piv, 1) : Pl[p * b + j], inv);
+        }
+        __builtin_amdgcn_wave_barrier();
+        cnt = 0;
+        for (int e = tid; e < b * b; e += 64, ++cnt) Pl[e] = Fo::fms(Pl[e], colv[cnt], rowv[cnt]);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// complex128: the block (padded to 16 x 16 with the identity) sits in wave 0's registers,
+// lane l holding column l % 16 of rows 4 (l / 16) + r, r < 4; each pivot's row, column
+// and entry come from lane shuffles, so the 16 pivots run without LDS round trips.
+__device__ void big_pivot_inverse(const cx<double>* Rl, cx<double>* Pl, int n, int p0, int b) {
+    using Fo = Fld<cx<double>>;
+    if (threadIdx.x >= 64) return;
+    const int lane = threadIdx.x;
+    const int j = lane & 15, g = lane >> 4;
+    cx<double> a[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = 4 * g + r;
+        a[r] = (i < b && j < b) ? Rl[i * n + p0 + j] : cx<double>{i == j ? 1.0 : 0.0, 0.0};
+    }
+#pragma unroll
+    for (int p = 0; p < 16; ++p) {
+        const int src_row = (p >> 2) * 16 + j;        // lane holding A[p][j] (register p & 3)
+        const int src_piv = (p >> 2) * 16 + p;        // lane holding A[p][p]
+        const int src_col = g * 16 + p;               // lane holding A[4 g + r][p] (register r)
+        const cx<double> rowp = {__shfl(a[p & 3].re, src_row, 64), __shfl(a[p & 3].im, src_row, 64)};
+        const cx<double> piv = {__shfl(a[p & 3].re, src_piv, 64), __shfl(a[p & 3].im, src_piv, 64)};
+        cx<double> colp[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) colp[r] = {__shfl(a[r].re, src_col, 64), __shfl(a[r].im, src_col, 64)};
+        const cx<double> rv = Fo::mul((j == p) ? Fo::add_one(rowp, 1) : rowp, Fo::recip(piv));
+#pragma unroll
+        for (int r = 0; r < 4; ++r) a[r] = Fo::fms(a[r], (4 * g + r == p) ? Fo::add_one(colp[r], -1) : colp[r], rv);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = 4 * g + r;
+        if (i < b && j < b) Pl[i * b + j] = a[r];
+    }
+}
+
+// R~ = Pi (R + [I on P]) in place on the row panel's columns >= lo.  complex64: one
+// column per thread.
+__device__ void big_rows_tilde(cx<float>* Rl, const cx<float>* Pl, int n, int lo, int p0, int b) {
+    using Fo = Fld<cx<float>>;
+    for (int jc = lo + threadIdx.x; jc < n; jc += blockDim.x) {
+        cx<float> v[16];
+        for (int q = 0; q < b; ++q) v[q] = (jc - p0 == q) ? Fo::add_one(Rl[q * n + jc], 1) : Rl[q * n + jc];
+        for (int q = 0; q < b; ++q) {
+            cx<float> acc = Fo::zero();
+            for (int r = 0; r < b; ++r) acc = Fo::fma_(acc, Pl[q * b + r], v[r]);
+            Rl[q * n + jc] = acc;
+        }
+    }
+}
+
+// complex128: one 16-column tile per wave on the matrix cores (a 16 x 16 by 16 x 16
+// product, four v_mfma_f64_16x16x4f64 per k-step of four), operands padded with zeros
+// beyond b; each wave reads all of its columns before writing them back.
+__device__ void big_rows_tilde(cx<double>* Rl, const cx<double>* Pl, int n, int lo, int p0, int b) {
+    typedef double V4 __attribute__((ext_vector_type(4)));
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int li = lane & 15, lg = lane >> 4;
+    for (int T = (lo >> 4) + wv; T < (n + 15) >> 4; T += nw) {
+        const int col = 16 * T + li;
+        const bool colok = col >= lo && col < n;
+        cx<double> bv[4], av[4];
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const int q = 4 * kk + lg;
+            cx<double> v = (q < b && colok) ? Rl[q * n + col] : cx<double>{0.0, 0.0};
+            if (q < b && col == p0 + q) v.re += 1.0;
+            bv[kk] = v;
+            av[kk] = (li < b && q < b) ? Pl[li * b + q] : cx<double>{0.0, 0.0};
+        }
+        V4 re = {0.0, 0.0, 0.0, 0.0}, im = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            re = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kk].re, bv[kk].re, re, 0, 0, 0);
+            re = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[kk].im, bv[kk].im, re, 0, 0, 0);
+            im = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kk].re, bv[kk].im, im, 0, 0, 0);
+            im = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kk].im, bv[kk].re, im, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int q = lg + 4 * r;
+            if (q < b && colok) Rl[q * n + col] = cx<double>{re[r], im[r]};
+        }
+    }
+}
+
 // A -= C~ R~ on lo <= i <= j (C~ stored transposed, Cl[q n + i]; R~ in Rl[q n + j]).
 // complex64: the vector ALU, each thread four rows of one column per round.
 __device__ void big_update_sym(cx<float>* __restrict__ A, int n, int lo, int b, const cx<float>* Cl,
@@ -1457,36 +1569,9 @@ __device__ void big_block_step_sym(E* __restrict__ A, int n, int p0, int b, E* C
         Cl[idx] = i < p0 ? Fo::mul(v, Fo::add_one(Fo::zero(), -1)) : v;
     }
     __syncthreads();
-    if (tid < 64) {   // wave 0: Pi = A_PP^-1 by the scalar shifted-pivot sweep
-        for (int e = tid; e < b * b; e += 64) Pl[e] = Rl[(e / b) * n + p0 + (e - (e / b) * b)];
-        __builtin_amdgcn_wave_barrier();
-        for (int p = 0; p < b; ++p) {
-            const E piv = Pl[p * b + p];
-            const E inv = Fo::recip(piv);
-            E colv[4], rowv[4];
-            int cnt = 0;
-            for (int e = tid; e < b * b; e += 64, ++cnt) {
-                const int i = e / b, j = e - (e / b) * b;
-                colv[cnt] = (i == p) ? Fo::add_one(piv, -1) : Pl[i * b + p];
-                rowv[cnt] = Fo::mul((j == p) ? Fo::add_one(piv, 1) : Pl[p * b + j], inv);
-            }
-            __builtin_amdgcn_wave_barrier();
-            cnt = 0;
-            for (int e = tid; e < b * b; e += 64, ++cnt) Pl[e] = Fo::fms(Pl[e], colv[cnt], rowv[cnt]);
-            __builtin_amdgcn_wave_barrier();
-        }
-    }
+    big_pivot_inverse(Rl, Pl, n, p0, b);   // Pi = A_PP^-1 into Pl
     __syncthreads();
-    // R~ = Pi (R + [I on P]) in place, one column per thread; C~ = C - [I on P]
-    for (int j = lo + tid; j < n; j += nt) {
-        E v[16];
-        for (int q = 0; q < b; ++q) v[q] = (j - p0 == q) ? Fo::add_one(Rl[q * n + j], 1) : Rl[q * n + j];
-        for (int q = 0; q < b; ++q) {
-            E acc = Fo::zero();
-            for (int r = 0; r < b; ++r) acc = Fo::fma_(acc, Pl[q * b + r], v[r]);
-            Rl[q * n + j] = acc;
-        }
-    }
+    big_rows_tilde(Rl, Pl, n, lo, p0, b);  // R~ = Pi (R + [I on P]) in place on columns >= lo
     for (int q = tid; q < b; q += nt) Cl[q * n + p0 + q] = Fo::add_one(Cl[q * n + p0 + q], -1);
     __syncthreads();
     big_update_sym(A, n, lo, b, Cl, Rl);
