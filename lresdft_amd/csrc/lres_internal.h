@@ -65,10 +65,12 @@ size_t big_lds_bytes(int nmax, int b, int elem_bytes);
 hipError_t launch_pole_big(int fp32, int bmax, int nmax, const lres::WinDesc* wins, const int32_t* ids, int nids,
                            const double* SW, const double* HW, int M, int ngroup, double c, double Rr, void* work,
                            const int64_t* ws_off, double* Dpart, double* Fpart, hipStream_t st);
+// Inertia counts of windows above the register-resident limit: work holds work_len
+// doubles, cut into slots of nmax^2; up to work_len / nmax^2 (window, shift) pairs are
+// factored at once (at least nids slots, else hipErrorInvalidValue).
 hipError_t launch_inertia_big(int bmax, int nmax, const lres::WinDesc* wins, const int32_t* ids, int nids,
-                              const double* SW, const double* HW, const double* sigma, int nshift, void* work,
-                              const int64_t* ws_off, int ngroup, int32_t* counts, unsigned long long* total,
-                              hipStream_t st);
+                              const double* SW, const double* HW, const double* sigma, int nshift, double* work,
+                              int64_t work_len, int32_t* counts, unsigned long long* total, hipStream_t st);
 // Pole solves on the matrix cores for windows of up to 128 functions: nt = 16 x 16
 // tiles per dimension (2..8), nwc = waves per (window, pole group) splitting the
 // tile columns (1..nt).  One object per (precision, nt),

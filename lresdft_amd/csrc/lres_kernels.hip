@@ -1242,130 +1242,41 @@ template <typename R> struct Fld<cx<R>> {
     __device__ static bool positive(cx<R> a) { return a.re > R(0); }
 };
 
-// One blocked Gauss-Jordan step on the n x n matrix A (global, row-major).
-// LDS: Cl (n x b), Rl (b x n), Pl (b x b).  Returns (in thread 0) the number of
-// positive scalar pivots met inside the pivot block (used by the inertia count).
+__device__ inline double shfl_e(double v, int src) { return __shfl(v, src, 64); }
+template <typename R> __device__ inline cx<R> shfl_e(cx<R> v, int src) {
+    return {__shfl(v.re, src, 64), __shfl(v.im, src, 64)};
+}
+
+// Pi = A_PP^-1 of the b x b pivot block (b <= 16; rows p0.. of the row panel Rl,
+// columns p0..) into Pl (b x b, row-major) by the scalar shifted-pivot sweep
+// a_ij -= (a_ip - d_ip)(a_pj + d_pj) / a_pp, whose in-place result is the inverse.
+// The block (padded to 16 x 16 with the identity, on which the sweep is a no-op) sits
+// in wave 0's registers, lane l holding column l % 16 of rows 4 (l / 16) + r, r < 4;
+// each pivot's row, column and entry come from lane shuffles, so the pivots run
+// without LDS round trips.  Returns (in wave 0) the number of positive scalar pivots,
+// which for the unpivoted block are the LDL^T pivots counted by the inertia.
 template <typename E>
-__device__ int big_block_step(E* __restrict__ A, int n, int p0, int b, E* Cl, E* Rl, E* Pl, int lo = 0) {
+__device__ int big_pivot_inverse(const E* Rl, E* Pl, int n, int p0, int b) {
     using Fo = Fld<E>;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    for (int idx = tid; idx < n * b; idx += nt) {
-        const int i = idx / b, q = idx - (idx / b) * b;
-        Cl[idx] = A[(int64_t)i * n + p0 + q];
-    }
-    for (int idx = tid; idx < b * n; idx += nt) Rl[idx] = A[(int64_t)p0 * n + idx];
-    __syncthreads();
-    int npos = 0;
-    if (tid < 64) {   // wave 0: Pi = A_PP^-1 by the scalar shifted-pivot sweep
-        for (int e = tid; e < b * b; e += 64) Pl[e] = Rl[(e / b) * n + p0 + (e - (e / b) * b)];
-        __builtin_amdgcn_wave_barrier();
-        for (int p = 0; p < b; ++p) {
-            const E piv = Pl[p * b + p];
-            npos += Fo::positive(piv) ? 1 : 0;
-            const E inv = Fo::recip(piv);
-            E colv[4], rowv[4];
-            int cnt = 0;
-            for (int e = tid; e < b * b; e += 64, ++cnt) {
-                const int i = e / b, j = e - (e / b) * b;
-                colv[cnt] = (i == p) ? Fo::add_one(piv, -1) : Pl[i * b + p];
-                rowv[cnt] = Fo::mul((j == p) ? Fo::add_one(piv, 1) : Pl[p * b + j], inv);
-            }
-            __builtin_amdgcn_wave_barrier();
-            cnt = 0;
-            for (int e = tid; e < b * b; e += 64, ++cnt) Pl[e] = Fo::fms(Pl[e], colv[cnt], rowv[cnt]);
-            __builtin_amdgcn_wave_barrier();
-        }
-    }
-    __syncthreads();
-    // R~ = Pi (R + [I on P]) in place, one column per thread; C~ = C - [I on P]
-    for (int j = lo + tid; j < n; j += nt) {
-        E v[32];
-        for (int q = 0; q < b; ++q) v[q] = (j - p0 == q) ? Fo::add_one(Rl[q * n + j], 1) : Rl[q * n + j];
-        for (int q = 0; q < b; ++q) {
-            E acc = Fo::zero();
-            for (int r = 0; r < b; ++r) acc = Fo::fma_(acc, Pl[q * b + r], v[r]);
-            Rl[q * n + j] = acc;
-        }
-    }
-    for (int q = tid; q < b; q += nt) Cl[(p0 + q) * b + q] = Fo::add_one(Cl[(p0 + q) * b + q], -1);
-    __syncthreads();
-    // A -= C~ R~ on rows and columns >= lo: each thread four rows of one column per round
-    // (row-major, coalesced in j).  lo = 0: the Gauss-Jordan step of the whole matrix;
-    // lo = p0: a right-looking elimination step (only the trailing Schur complement is
-    // kept current, which is all that the inertia count reads)
-    const int nl = n - lo;
-    const int rows4 = (nl + 3) / 4;
-    for (int idx = tid; idx < rows4 * nl; idx += nt) {
-        const int i0 = lo + (idx / nl) * 4, j = lo + idx - (idx / nl) * nl;
-        E a[4];
-#pragma unroll
-        for (int s = 0; s < 4; ++s) a[s] = (i0 + s < n) ? A[(int64_t)(i0 + s) * n + j] : Fo::zero();
-        for (int q = 0; q < b; ++q) {
-            const E r = Rl[q * n + j];
-#pragma unroll
-            for (int s = 0; s < 4; ++s)
-                if (i0 + s < n) a[s] = Fo::fms(a[s], Cl[(i0 + s) * b + q], r);
-        }
-#pragma unroll
-        for (int s = 0; s < 4; ++s)
-            if (i0 + s < n) A[(int64_t)(i0 + s) * n + j] = a[s];
-    }
-    __syncthreads();
-    return npos;
-}
-
-// Pi = A_PP^-1 of the b x b pivot block (rows p0.. of the row panel Rl, columns p0..)
-// into Pl (b x b, row-major), by the scalar shifted-pivot sweep
-// a_ij -= (a_ip - d_ip)(a_pj + d_pj) / a_pp, which leaves the in-place Gauss-Jordan
-// result.  complex64: wave 0 through LDS.
-__device__ void big_pivot_inverse(const cx<float>* Rl, cx<float>* Pl, int n, int p0, int b) {
-    using Fo = Fld<cx<float>>;
-    const int tid = threadIdx.x;
-    if (tid >= 64) return;
-    for (int e = tid; e < b * b; e += 64) Pl[e] = Rl[(e / b) * n + p0 + (e - (e / b) * b)];
-    __builtin_amdgcn_wave_barrier();
-    for (int p = 0; p < b; ++p) {
-        const cx<float> piv = Pl[p * b + p];
-        const cx<float> inv = Fo::recip(piv);
-        cx<float> colv[4], rowv[4];
-        int cnt = 0;
-        for (int e = tid; e < b * b; e += 64, ++cnt) {
-            const int i = e / b, j = e - (e / b) * b;
-            colv[cnt] = (i == p) ? Fo::add_one(piv, -1) : Pl[i * b + p];
-            rowv[cnt] = Fo::mul((j == p) ? Fo::add_one(piv, 1) : Pl[p * b + j], inv);
-        }
-        __builtin_amdgcn_wave_barrier();
-        cnt = 0;
-        for (int e = tid; e < b * b; e += 64, ++cnt) Pl[e] = Fo::fms(Pl[e], colv[cnt], rowv[cnt]);
-        __builtin_amdgcn_wave_barrier();
-    }
-}
-
-// complex128: the block (padded to 16 x 16 with the identity) sits in wave 0's registers,
-// lane l holding column l % 16 of rows 4 (l / 16) + r, r < 4; each pivot's row, column
-// and entry come from lane shuffles, so the 16 pivots run without LDS round trips.
-__device__ void big_pivot_inverse(const cx<double>* Rl, cx<double>* Pl, int n, int p0, int b) {
-    using Fo = Fld<cx<double>>;
-    if (threadIdx.x >= 64) return;
+    if (threadIdx.x >= 64) return 0;
     const int lane = threadIdx.x;
     const int j = lane & 15, g = lane >> 4;
-    cx<double> a[4];
+    E a[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int i = 4 * g + r;
-        a[r] = (i < b && j < b) ? Rl[i * n + p0 + j] : cx<double>{i == j ? 1.0 : 0.0, 0.0};
+        a[r] = (i < b && j < b) ? Rl[i * n + p0 + j] : Fo::add_one(Fo::zero(), i == j ? 1 : 0);
     }
+    int npos = 0;
 #pragma unroll
     for (int p = 0; p < 16; ++p) {
-        const int src_row = (p >> 2) * 16 + j;        // lane holding A[p][j] (register p & 3)
-        const int src_piv = (p >> 2) * 16 + p;        // lane holding A[p][p]
-        const int src_col = g * 16 + p;               // lane holding A[4 g + r][p] (register r)
-        const cx<double> rowp = {__shfl(a[p & 3].re, src_row, 64), __shfl(a[p & 3].im, src_row, 64)};
-        const cx<double> piv = {__shfl(a[p & 3].re, src_piv, 64), __shfl(a[p & 3].im, src_piv, 64)};
-        cx<double> colp[4];
+        const E rowp = shfl_e(a[p & 3], (p >> 2) * 16 + j);   // A[p][j]
+        const E piv = shfl_e(a[p & 3], (p >> 2) * 16 + p);    // A[p][p]
+        E colp[4];                                             // A[4 g + r][p]
 #pragma unroll
-        for (int r = 0; r < 4; ++r) colp[r] = {__shfl(a[r].re, src_col, 64), __shfl(a[r].im, src_col, 64)};
-        const cx<double> rv = Fo::mul((j == p) ? Fo::add_one(rowp, 1) : rowp, Fo::recip(piv));
+        for (int r = 0; r < 4; ++r) colp[r] = shfl_e(a[r], g * 16 + p);
+        if (p < b && Fo::positive(piv)) ++npos;
+        const E rv = Fo::mul((j == p) ? Fo::add_one(rowp, 1) : rowp, Fo::recip(piv));
 #pragma unroll
         for (int r = 0; r < 4; ++r) a[r] = Fo::fms(a[r], (4 * g + r == p) ? Fo::add_one(colp[r], -1) : colp[r], rv);
     }
@@ -1374,6 +1285,7 @@ __device__ void big_pivot_inverse(const cx<double>* Rl, cx<double>* Pl, int n, i
         const int i = 4 * g + r;
         if (i < b && j < b) Pl[i * b + j] = a[r];
     }
+    return npos;
 }
 
 // R~ = Pi (R + [I on P]) in place on the row panel's columns >= lo.  complex64: one
@@ -1422,6 +1334,31 @@ __device__ void big_rows_tilde(cx<double>* Rl, const cx<double>* Pl, int n, int 
         for (int r = 0; r < 4; ++r) {
             const int q = lg + 4 * r;
             if (q < b && colok) Rl[q * n + col] = cx<double>{re[r], im[r]};
+        }
+    }
+}
+
+// real fp64: one v_mfma_f64_16x16x4f64 per k-step of four, tiles as above
+__device__ void big_rows_tilde(double* Rl, const double* Pl, int n, int lo, int p0, int b) {
+    typedef double V4 __attribute__((ext_vector_type(4)));
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int li = lane & 15, lg = lane >> 4;
+    for (int T = (lo >> 4) + wv; T < (n + 15) >> 4; T += nw) {
+        const int col = 16 * T + li;
+        const bool colok = col >= lo && col < n;
+        V4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const int q = 4 * kk + lg;
+            double v = (q < b && colok) ? Rl[q * n + col] : 0.0;
+            if (q < b && col == p0 + q) v += 1.0;
+            const double pa = (li < b && q < b) ? Pl[li * b + q] : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(pa, v, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int q = lg + 4 * r;
+            if (q < b && colok) Rl[q * n + col] = acc[r];
         }
     }
 }
@@ -1525,6 +1462,105 @@ __device__ void big_update_sym(cx<double>* __restrict__ A, int n, int lo, int b,
     }
 }
 
+// real fp64: the same tiles with one v_mfma_f64_16x16x4f64 per k-step (the inertia
+// counts' LDL^T of the real symmetric sigma S_W - H_W)
+__device__ void big_update_sym(double* __restrict__ A, int n, int lo, int b, const double* Cl, const double* Rl) {
+    typedef double V4 __attribute__((ext_vector_type(4)));
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int li = lane & 15, lg = lane >> 4;
+    const int t0 = lo >> 4, ntile = ((n + 15) >> 4) - t0;
+    const int tiles = ntile * (ntile + 1) / 2;
+    const int nk = (b + 3) >> 2;
+    auto tile_of = [&](int t, int& r0, int& c0) {
+        int I = 0, rem = t;
+        while (rem >= ntile - I) {
+            rem -= ntile - I;
+            ++I;
+        }
+        r0 = 16 * (t0 + I);
+        c0 = 16 * (t0 + I + rem);
+    };
+    auto load = [&](int r0, int c0, V4& acc) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = r0 + lg + 4 * r, col = c0 + li;
+            acc[r] = (row >= lo && row <= col && col < n) ? A[(int64_t)row * n + col] : 0.0;
+        }
+    };
+    int t = wv;
+    if (t >= tiles) return;
+    int r0, c0;
+    tile_of(t, r0, c0);
+    V4 acc;
+    load(r0, c0, acc);
+    for (; t < tiles; t += nw) {
+        int r1 = 0, c1 = 0;
+        V4 nxt = {0.0, 0.0, 0.0, 0.0};
+        const bool more = t + nw < tiles;
+        if (more) {
+            tile_of(t + nw, r1, c1);
+            load(r1, c1, nxt);
+        }
+        for (int kk = 0; kk < nk; ++kk) {
+            const int q = 4 * kk + lg;
+            const int ra = r0 + li, cb = c0 + li;
+            const double ca = (q < b && ra < n) ? Cl[q * n + ra] : 0.0;
+            const double rb = (q < b && cb < n) ? Rl[q * n + cb] : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-ca, rb, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = r0 + lg + 4 * r, col = c0 + li;
+            if (row >= lo && row <= col && col < n) A[(int64_t)row * n + col] = acc[r];
+        }
+        if (!more) break;
+        r0 = r1;
+        c0 = c1;
+        acc = nxt;
+    }
+}
+
+// One blocked Gauss-Jordan step on the n x n matrix A (global, row-major).
+// LDS: Cl (n x b), Rl (b x n), Pl (b x b), b <= 16.  Returns (in wave 0) the number of
+// positive scalar pivots met inside the pivot block.
+template <typename E>
+__device__ int big_block_step(E* __restrict__ A, int n, int p0, int b, E* Cl, E* Rl, E* Pl, int lo = 0) {
+    using Fo = Fld<E>;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int idx = tid; idx < n * b; idx += nt) {
+        const int i = idx / b, q = idx - (idx / b) * b;
+        Cl[idx] = A[(int64_t)i * n + p0 + q];
+    }
+    for (int idx = tid; idx < b * n; idx += nt) Rl[idx] = A[(int64_t)p0 * n + idx];
+    __syncthreads();
+    const int npos = big_pivot_inverse(Rl, Pl, n, p0, b);   // Pi = A_PP^-1 into Pl
+    __syncthreads();
+    big_rows_tilde(Rl, Pl, n, lo, p0, b);   // R~ = Pi (R + [I on P]) in place; C~ = C - [I on P]
+    for (int q = tid; q < b; q += nt) Cl[(p0 + q) * b + q] = Fo::add_one(Cl[(p0 + q) * b + q], -1);
+    __syncthreads();
+    // A -= C~ R~ on rows and columns >= lo: each thread four rows of one column per round
+    // (row-major, coalesced in j).  lo = 0: the Gauss-Jordan step of the whole matrix
+    const int nl = n - lo;
+    const int rows4 = (nl + 3) / 4;
+    for (int idx = tid; idx < rows4 * nl; idx += nt) {
+        const int i0 = lo + (idx / nl) * 4, j = lo + idx - (idx / nl) * nl;
+        E a[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) a[s] = (i0 + s < n) ? A[(int64_t)(i0 + s) * n + j] : Fo::zero();
+        for (int q = 0; q < b; ++q) {
+            const E r = Rl[q * n + j];
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+                if (i0 + s < n) a[s] = Fo::fms(a[s], Cl[(i0 + s) * b + q], r);
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+            if (i0 + s < n) A[(int64_t)(i0 + s) * n + j] = a[s];
+    }
+    __syncthreads();
+    return npos;
+}
+
 // big_block_step on the upper triangle only, for complex-symmetric A (the pole
 // solves).  After the pivots of a set D the in-place state is symmetric on D x D
 // and on the rest U x U and antisymmetric between them (W[u][d] = -W[d][u]), so
@@ -1540,9 +1576,11 @@ __device__ void big_update_sym(cx<double>* __restrict__ A, int n, int lo, int b,
 // trailing block (after the pivots of D = [0, m) it holds the Schur complement
 // A_TT - A_TD A_DD^-1 A_DT, and rows / columns of D are never read again); from m on,
 // lo = m runs Gauss-Jordan on that k x k block, whose in-place result is its inverse,
-// (A^-1)[T,T].  With lo = 0 this is the whole-matrix sweep.
+// (A^-1)[T,T].  With lo = 0 this is the whole-matrix sweep.  The same steps with lo = p0
+// throughout are the unpivoted LDL^T of a real symmetric A; the return value (in wave 0)
+// counts the positive pivots of the block.
 template <typename E>
-__device__ void big_block_step_sym(E* __restrict__ A, int n, int p0, int b, E* Cl, E* Rl, E* Pl, int lo) {
+__device__ int big_block_step_sym(E* __restrict__ A, int n, int p0, int b, E* Cl, E* Rl, E* Pl, int lo) {
     using Fo = Fld<E>;
     const int tid = threadIdx.x, nt = blockDim.x;
     // pivot rows: right of the diagonal from the rows themselves (coalesced along j),
@@ -1569,13 +1607,14 @@ __device__ void big_block_step_sym(E* __restrict__ A, int n, int p0, int b, E* C
         Cl[idx] = i < p0 ? Fo::mul(v, Fo::add_one(Fo::zero(), -1)) : v;
     }
     __syncthreads();
-    big_pivot_inverse(Rl, Pl, n, p0, b);   // Pi = A_PP^-1 into Pl
+    const int npos = big_pivot_inverse(Rl, Pl, n, p0, b);   // Pi = A_PP^-1 into Pl
     __syncthreads();
     big_rows_tilde(Rl, Pl, n, lo, p0, b);  // R~ = Pi (R + [I on P]) in place on columns >= lo
     for (int q = tid; q < b; q += nt) Cl[q * n + p0 + q] = Fo::add_one(Cl[q * n + p0 + q], -1);
     __syncthreads();
     big_update_sym(A, n, lo, b, Cl, Rl);
     __syncthreads();
+    return npos;
 }
 
 constexpr int kBigThreads = 1024;
@@ -1638,32 +1677,37 @@ __global__ void __launch_bounds__(kBigThreads) k_pole_big(const WinDesc* __restr
     }
 }
 
+// Inertia counts of sigma_s S_W - H_W by the unpivoted LDL^T (right-looking blocked
+// elimination on the upper triangle, real fp64 on the matrix cores).  Block (x, y)
+// factors shifts y, y + gridDim.y, ... of window ids[x] in its own slot of the
+// workspace (stride entries apart), so every (window, shift) of the launch can run at
+// once when the workspace holds that many slots.
 __global__ void __launch_bounds__(kBigThreads) k_inertia_big(const WinDesc* __restrict__ wins, const int32_t* __restrict__ ids,
                                                      const double* __restrict__ SW, const double* __restrict__ HW,
                                                      const double* __restrict__ sigma, int nshift, int bmax,
-                                                     cx<double>* __restrict__ work, const int64_t* __restrict__ ws_off,
-                                                     int ngroup, int32_t* __restrict__ counts,
+                                                     double* __restrict__ work, int64_t stride,
+                                                     int32_t* __restrict__ counts,
                                                      unsigned long long* __restrict__ total) {
     extern __shared__ double lds_raw[];
     const int w = ids[blockIdx.x];
-    const int slot = blockIdx.y % ngroup;   // workspace slots are shared with the pole kernel
     const WinDesc d = wins[w];
     const int n = d.n;
-    double* A = reinterpret_cast<double*>(work + ws_off[w] + (int64_t)slot * n * n);
+    double* A = work + ((int64_t)blockIdx.x * gridDim.y + blockIdx.y) * stride;
     double* Cl = lds_raw;
     double* Rl = Cl + (size_t)n * bmax;
     double* Pl = Rl + (size_t)n * bmax;
     const double* S = SW + d.mat_off;
     const double* H = HW + d.mat_off;
-    for (int s_id = blockIdx.y; s_id < nshift; s_id += ngroup) {
+    for (int s_id = blockIdx.y; s_id < nshift; s_id += gridDim.y) {
         const double sg = sigma[s_id];
-        for (int64_t q = threadIdx.x; q < (int64_t)n * n; q += blockDim.x) A[q] = sg * S[q] - H[q];
+        for (int64_t q = threadIdx.x; q < (int64_t)n * n; q += blockDim.x) {
+            if (q / n > q % n) continue;   // upper triangle only
+            A[q] = sg * S[q] - H[q];
+        }
         __syncthreads();
         int npos = 0;
-        // right-looking elimination: the pivots of each block are those of the unpivoted
-        // LDL^T, and only the trailing block is updated
         for (int p0 = 0; p0 < n; p0 += bmax)
-            npos += big_block_step<double>(A, n, p0, min(bmax, n - p0), Cl, Rl, Pl, p0);
+            npos += big_block_step_sym<double>(A, n, p0, min(bmax, n - p0), Cl, Rl, Pl, p0);
         if (threadIdx.x == 0) {
             if (counts) counts[(int64_t)w * nshift + s_id] = npos;
             atomicAdd(&total[s_id], (unsigned long long)npos);
@@ -1698,14 +1742,18 @@ hipError_t launch_pole_big(int fp32, int bmax, int nmax, const WinDesc* wins, co
 }
 
 hipError_t launch_inertia_big(int bmax, int nmax, const WinDesc* wins, const int32_t* ids, int nids, const double* SW,
-                              const double* HW, const double* sigma, int nshift, void* work, const int64_t* ws_off,
-                              int ngroup, int32_t* counts, unsigned long long* total, hipStream_t st) {
+                              const double* HW, const double* sigma, int nshift, double* work, int64_t work_len,
+                              int32_t* counts, unsigned long long* total, hipStream_t st) {
     const size_t lds = big_lds_bytes(nmax, bmax, 8);
+    const int64_t stride = (int64_t)nmax * nmax;
+    const int64_t slots = work_len / stride;
+    if (nids <= 0 || slots < nids) return hipErrorInvalidValue;
     hipError_t e = hipFuncSetAttribute((const void*)k_inertia_big, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    dim3 grid(nids, std::min(ngroup, nshift)), blk(kBigThreads);
-    hipLaunchKernelGGL(k_inertia_big, grid, blk, lds, st, wins, ids, SW, HW, sigma, nshift, bmax, (cx<double>*)work,
-                       ws_off, std::min(ngroup, nshift), counts, total);
+    const int ny = (int)std::min<int64_t>(nshift, slots / nids);
+    dim3 grid(nids, ny), blk(kBigThreads);
+    hipLaunchKernelGGL(k_inertia_big, grid, blk, lds, st, wins, ids, SW, HW, sigma, nshift, bmax, work, stride, counts,
+                       total);
     return hipGetLastError();
 }
 

@@ -426,7 +426,7 @@ void inertia_launch(lres_plan* P, const double* sig, int ns, int32_t* d_counts,
         if (nids == 0) continue;
         if (b.big)
             hip_check(launch_inertia_big(b.bmax, b.nmax, P->d_wins.p, ids, nids, P->d_SW.p, P->d_HW.p, d_sig,
-                                         ns, P->d_work.p, P->d_ws_off.p, P->ngroup, d_counts, P->d_itot.p, P->st),
+                                         ns, P->d_work.p, (int64_t)P->d_work.n, d_counts, P->d_itot.p, P->st),
                       "blocked inertia kernel");
         else if (inertia_on_mfma(b.nmax))
             hip_check(launch_inertia_mfma((b.nmax + 15) / 16, P->d_wins.p, ids, nids, P->d_SW.p, P->d_HW.p,
@@ -783,7 +783,12 @@ int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, 
             }
         }
         P->d_ws_off.upload(ws_off.data(), nwin, P->st);
-        if (ws_total) P->d_work.alloc((size_t)ws_total * 2);
+        // the blocked inertia kernel cuts the same workspace into real nmax^2 slots and
+        // needs one per window of a bucket at least
+        int64_t ws_len = ws_total * 2;
+        for (const auto& b : P->buckets)
+            if (b.big) ws_len = std::max<int64_t>(ws_len, (int64_t)b.ids.size() * b.nmax * b.nmax);
+        if (ws_len) P->d_work.alloc((size_t)ws_len);
         const char* at = std::getenv("LRES_AUTOTUNE");
         P->autotune = !(at && at[0] == '0');
         const char* tl = std::getenv("LRES_TUNE_LOG");
@@ -1232,8 +1237,11 @@ int lres_lambda_min(lres_plan* P, double tol, double* out) {
         // most kMaxK shifts: log2(K + 1) bits of the bracket per round) on the
         // one-wave-per-shift kernels.
         constexpr int kWaves = 8192, kMaxK = 255;
-        // the blocked inertia kernel of windows above 128 functions factors its shifts
-        // one after another within a few blocks per window: 15 shifts per launch there
+        // the blocked inertia kernel of windows above 128 functions factors each
+        // (window, shift) in a workgroup of its own: about kBlocks of them per launch
+        // (a pass of the whole chip), so bisection while every window takes part and
+        // up to kMaxK shifts once a few are left
+        constexpr int kBlocks = 512;
         const bool blocked = std::any_of(P->buckets.begin(), P->buckets.end(),
                                          [](const lres_plan::Bucket& b) { return b.big; });
         std::vector<double> sg(kMaxK);
@@ -1244,7 +1252,8 @@ int lres_lambda_min(lres_plan* P, double tol, double* out) {
         bool all = true;
         while (hi - lo > tol) {
             const int ncand = all ? nwin : (int)cand.size();
-            const int K = blocked ? 15 : std::max(15, std::min(kMaxK, kWaves / std::max(1, ncand)));
+            const int K = blocked ? std::max(1, std::min(kMaxK, kBlocks / std::max(1, ncand)))
+                                  : std::max(15, std::min(kMaxK, kWaves / std::max(1, ncand)));
             for (int s = 0; s < K; ++s) sg[s] = lo + (hi - lo) * (s + 1) / (K + 1);
             // the per-window counts come back while the set is large enough for
             // shrinking it to pay (small sets run at up to kMaxK shifts anyway), and

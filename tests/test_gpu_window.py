@@ -298,6 +298,36 @@ def test_mixed_precision_scf_meets_energy_target():
     assert abs(2 * n64 - cr.n_elec) <= 1e-8 * cr.n_elec
 
 
+def test_blocked_inertia_counts_match_library_eigenvalues(sys3):
+    """Inertia counts of the blocked LDL^T (windows of 228 functions, n=3, R_w 8.5) equal the
+    dense generalized eigenvalue counts at 40 shifts across the spectrum -- more shifts than
+    the workspace holds slots per window, so blocks also take several shifts in turn -- and
+    lambda_min agrees with the lowest dense eigenvalue to 1e-10."""
+    cr, b, v = sys3
+    print("[gpu] blocked inertia n_W=228, 40 shifts", flush=True)
+    from lresdft_amd.lcao3d import csr64
+    cells = cell_centres((3, 3, 3), 6.0)
+    C = _plan(b, cells, 8.5, 32)
+    assert int(C.plan.n.min()) > 128
+    H = b.H(v)
+    W, _ = build_windows(b, cells, 8.5)
+    C.plan.set_h(csr64(H))
+    allv = np.sort(np.concatenate([linalg.eigh(H[w][:, w].toarray(), b.S[w][:, w].toarray(), eigvals_only=True)
+                                   for w in W]))
+    rng = np.random.default_rng(3)
+    sig = []
+    while len(sig) < 40:   # shifts at least 1e-6 away from every eigenvalue
+        s = rng.uniform(allv[0] - 0.2, allv[-1] + 0.2)
+        if np.min(np.abs(allv - s)) > 1e-6:
+            sig.append(s)
+    sig = np.array(sig)
+    tot = C.plan.inertia(sig)
+    want = np.searchsorted(allv, sig)
+    assert np.array_equal(np.asarray(tot), want)
+    lmin = C.plan.lambda_min(1e-11)
+    assert abs(lmin - allv[0]) < 1e-10
+
+
 @pytest.mark.parametrize("M", [32, 64])
 def test_big_window_pass_matches_oracle(sys3, M):
     """Windows above the register-resident limit of the vector kernels (n=3, R_w 8.5: 19 cells,
