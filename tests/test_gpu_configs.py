@@ -7,7 +7,8 @@ a fixed circle (mu, e_lo) on the same H(v):
   * the same at a = 5.1 (0.85x compressed, 8 grid points per cell), R_w 8 and 10
     (n_W 228 / 324);
   * 343 H2 at random centres (d_min 3.0 bohr, seed 7) in a 42 bohr box, R_w 8
-    (n_W 36..216, several size buckets);
+    (n_W 36..216, several size buckets) and R_w 10 (windows up to ~360 functions:
+    the blocked kernels in both precisions, panel width 8);
   * 1000 H2 (2000 e-), 10x10x10 lattice, R_w 10 (n_W 228), M 64.
 The oracle side is the eigen route with full window spectra (one generalised
 eigendecomposition per window, then r_M at each M; the pole route equals it by
@@ -81,6 +82,7 @@ CASES = [
     ((8, 5.1), 8.0, (32,), 228),
     ((8, 5.1), 10.0, (32,), 324),
     ("rand343", 8.0, (32,), None),
+    ("rand343", 10.0, (32,), None),
     ((10, 6.0), 10.0, (64,), 228),
 ]
 
