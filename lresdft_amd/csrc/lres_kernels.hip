@@ -1288,52 +1288,57 @@ __device__ int big_pivot_inverse(const E* Rl, E* Pl, int n, int p0, int b) {
     return npos;
 }
 
-// R~ = Pi (R + [I on P]) in place on the row panel's columns >= lo.  complex64: one
-// column per thread.
-__device__ void big_rows_tilde(cx<float>* Rl, const cx<float>* Pl, int n, int lo, int p0, int b) {
-    using Fo = Fld<cx<float>>;
-    for (int jc = lo + threadIdx.x; jc < n; jc += blockDim.x) {
-        cx<float> v[16];
-        for (int q = 0; q < b; ++q) v[q] = (jc - p0 == q) ? Fo::add_one(Rl[q * n + jc], 1) : Rl[q * n + jc];
-        for (int q = 0; q < b; ++q) {
-            cx<float> acc = Fo::zero();
-            for (int r = 0; r < b; ++r) acc = Fo::fma_(acc, Pl[q * b + r], v[r]);
-            Rl[q * n + jc] = acc;
-        }
-    }
-}
-
-// complex128: one 16-column tile per wave on the matrix cores (a 16 x 16 by 16 x 16
-// product, four v_mfma_f64_16x16x4f64 per k-step of four), operands padded with zeros
-// beyond b; each wave reads all of its columns before writing them back.
-__device__ void big_rows_tilde(cx<double>* Rl, const cx<double>* Pl, int n, int lo, int p0, int b) {
+// 16 x 16 x 4 matrix-core products per real type.  Layouts (gfx950, checked with one-hot
+// operands): A operand lane l -> A[l % 16][l / 16], B operand lane l -> B[l / 16][l % 16];
+// accumulator lane l, register r -> column l % 16 and row l / 16 + 4 r (f64) or
+// 4 (l / 16) + r (f32).
+template <typename R> struct Mfma16;
+template <> struct Mfma16<double> {
     typedef double V4 __attribute__((ext_vector_type(4)));
+    __device__ static V4 op(double a, double b, V4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    __device__ static int row(int lg, int r) { return lg + 4 * r; }
+};
+template <> struct Mfma16<float> {
+    typedef float V4 __attribute__((ext_vector_type(4)));
+    __device__ static V4 op(float a, float b, V4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    __device__ static int row(int lg, int r) { return 4 * lg + r; }
+};
+
+// R~ = Pi (R + [I on P]) in place on the row panel's columns >= lo: one 16-column tile per
+// wave on the matrix cores (a 16 x 16 by 16 x 16 complex product, four real MFMAs per
+// k-step of four), operands padded with zeros beyond b; each wave reads all of its
+// columns before writing them back.
+template <typename R>
+__device__ void big_rows_tilde(cx<R>* Rl, const cx<R>* Pl, int n, int lo, int p0, int b) {
+    using Mf = Mfma16<R>;
+    typedef typename Mf::V4 V4;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int li = lane & 15, lg = lane >> 4;
     for (int T = (lo >> 4) + wv; T < (n + 15) >> 4; T += nw) {
         const int col = 16 * T + li;
         const bool colok = col >= lo && col < n;
-        cx<double> bv[4], av[4];
+        const cx<R> zero = {R(0), R(0)};
+        cx<R> bv[4], av[4];
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
             const int q = 4 * kk + lg;
-            cx<double> v = (q < b && colok) ? Rl[q * n + col] : cx<double>{0.0, 0.0};
-            if (q < b && col == p0 + q) v.re += 1.0;
+            cx<R> v = (q < b && colok) ? Rl[q * n + col] : zero;
+            if (q < b && col == p0 + q) v.re += R(1);
             bv[kk] = v;
-            av[kk] = (li < b && q < b) ? Pl[li * b + q] : cx<double>{0.0, 0.0};
+            av[kk] = (li < b && q < b) ? Pl[li * b + q] : zero;
         }
-        V4 re = {0.0, 0.0, 0.0, 0.0}, im = {0.0, 0.0, 0.0, 0.0};
+        V4 re = {R(0), R(0), R(0), R(0)}, im = {R(0), R(0), R(0), R(0)};
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
-            re = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kk].re, bv[kk].re, re, 0, 0, 0);
-            re = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[kk].im, bv[kk].im, re, 0, 0, 0);
-            im = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kk].re, bv[kk].im, im, 0, 0, 0);
-            im = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kk].im, bv[kk].re, im, 0, 0, 0);
+            re = Mf::op(av[kk].re, bv[kk].re, re);
+            re = Mf::op(-av[kk].im, bv[kk].im, re);
+            im = Mf::op(av[kk].re, bv[kk].im, im);
+            im = Mf::op(av[kk].im, bv[kk].re, im);
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int q = lg + 4 * r;
-            if (q < b && colok) Rl[q * n + col] = cx<double>{re[r], im[r]};
+            const int q = Mf::row(lg, r);
+            if (q < b && colok) Rl[q * n + col] = cx<R>{re[r], im[r]};
         }
     }
 }
@@ -1363,41 +1368,17 @@ __device__ void big_rows_tilde(double* Rl, const double* Pl, int n, int lo, int 
     }
 }
 
-// A -= C~ R~ on lo <= i <= j (C~ stored transposed, Cl[q n + i]; R~ in Rl[q n + j]).
-// complex64: the vector ALU, each thread four rows of one column per round.
-__device__ void big_update_sym(cx<float>* __restrict__ A, int n, int lo, int b, const cx<float>* Cl,
-                               const cx<float>* Rl) {
-    using Fo = Fld<cx<float>>;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const int nl = n - lo;
-    const int rows4 = (nl + 3) / 4;
-    for (int idx = tid; idx < rows4 * nl; idx += nt) {
-        const int i0 = lo + (idx / nl) * 4, j = lo + idx - (idx / nl) * nl;
-        if (i0 > j) continue;
-        cx<float> a[4];
-#pragma unroll
-        for (int s = 0; s < 4; ++s) a[s] = (i0 + s <= j && i0 + s < n) ? A[(int64_t)(i0 + s) * n + j] : Fo::zero();
-        for (int q = 0; q < b; ++q) {
-            const cx<float> r = Rl[q * n + j];
-#pragma unroll
-            for (int s = 0; s < 4; ++s)
-                if (i0 + s <= j && i0 + s < n) a[s] = Fo::fms(a[s], Cl[q * n + i0 + s], r);
-        }
-#pragma unroll
-        for (int s = 0; s < 4; ++s)
-            if (i0 + s <= j && i0 + s < n) A[(int64_t)(i0 + s) * n + j] = a[s];
-    }
-}
-
-// complex128: on the matrix cores, one 16 x 16 tile (I <= J) of the upper triangle per
-// wave at a time, four v_mfma_f64_16x16x4f64 per complex product and k-step of four
-// panel columns.  Layouts (gfx950): accumulator lane l, register r -> row l / 16 + 4 r,
-// column l % 16; A operand lane l -> A[l % 16][l / 16]; B operand lane l -> B[l / 16][l % 16].
-// Operand rows / columns beyond n read as zero; rows or columns below lo (not kept in
-// the panels) only reach output entries below lo, which are not stored.
-__device__ void big_update_sym(cx<double>* __restrict__ A, int n, int lo, int b, const cx<double>* Cl,
-                               const cx<double>* Rl) {
-    typedef double V4 __attribute__((ext_vector_type(4)));
+// A -= C~ R~ on lo <= i <= j (C~ stored transposed, Cl[q n + i]; R~ in Rl[q n + j]) on
+// the matrix cores: one 16 x 16 tile (I <= J) of the upper triangle per wave at a time,
+// four real MFMAs per complex product and k-step of four panel columns, the next tile's
+// entries loaded while this tile's products run.  Operand rows / columns beyond n read
+// as zero; rows or columns below lo (not kept in the panels) only reach output entries
+// below lo, which are not stored.
+template <typename R>
+__device__ void big_update_sym(cx<R>* __restrict__ A, int n, int lo, int b, const cx<R>* Cl, const cx<R>* Rl) {
+    using Mf = Mfma16<R>;
+    typedef typename Mf::V4 V4;
+    const cx<R> zero = {R(0), R(0)};
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int li = lane & 15, lg = lane >> 4;
     const int t0 = lo >> 4, ntile = ((n + 15) >> 4) - t0;
@@ -1416,9 +1397,9 @@ __device__ void big_update_sym(cx<double>* __restrict__ A, int n, int lo, int b,
     auto load = [&](int r0, int c0, V4& re, V4& im) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int row = r0 + lg + 4 * r, col = c0 + li;
+            const int row = r0 + Mf::row(lg, r), col = c0 + li;
             const bool ok = row >= lo && row <= col && col < n;
-            const cx<double> a = ok ? A[(int64_t)row * n + col] : cx<double>{0.0, 0.0};
+            const cx<R> a = ok ? A[(int64_t)row * n + col] : zero;
             re[r] = a.re;
             im[r] = a.im;
         }
@@ -1432,7 +1413,7 @@ __device__ void big_update_sym(cx<double>* __restrict__ A, int n, int lo, int b,
     for (; t < tiles; t += nw) {
         // the next tile's entries are loaded while this tile's products run
         int r1 = 0, c1 = 0;
-        V4 bre = {0.0, 0.0, 0.0, 0.0}, bim = {0.0, 0.0, 0.0, 0.0};
+        V4 bre = {R(0), R(0), R(0), R(0)}, bim = {R(0), R(0), R(0), R(0)};
         const bool more = t + nw < tiles;
         if (more) {
             tile_of(t + nw, r1, c1);
@@ -1441,18 +1422,18 @@ __device__ void big_update_sym(cx<double>* __restrict__ A, int n, int lo, int b,
         for (int kk = 0; kk < nk; ++kk) {
             const int q = 4 * kk + lg;
             const int ra = r0 + li, cb = c0 + li;
-            const cx<double> ca = (q < b && ra < n) ? Cl[q * n + ra] : cx<double>{0.0, 0.0};
-            const cx<double> rb = (q < b && cb < n) ? Rl[q * n + cb] : cx<double>{0.0, 0.0};
+            const cx<R> ca = (q < b && ra < n) ? Cl[q * n + ra] : zero;
+            const cx<R> rb = (q < b && cb < n) ? Rl[q * n + cb] : zero;
             // A -= C R:  re += (-Cr) Rr + Ci Ri,  im += (-Cr) Ri + (-Ci) Rr
-            are = __builtin_amdgcn_mfma_f64_16x16x4f64(-ca.re, rb.re, are, 0, 0, 0);
-            are = __builtin_amdgcn_mfma_f64_16x16x4f64(ca.im, rb.im, are, 0, 0, 0);
-            aim = __builtin_amdgcn_mfma_f64_16x16x4f64(-ca.re, rb.im, aim, 0, 0, 0);
-            aim = __builtin_amdgcn_mfma_f64_16x16x4f64(-ca.im, rb.re, aim, 0, 0, 0);
+            are = Mf::op(-ca.re, rb.re, are);
+            are = Mf::op(ca.im, rb.im, are);
+            aim = Mf::op(-ca.re, rb.im, aim);
+            aim = Mf::op(-ca.im, rb.re, aim);
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int row = r0 + lg + 4 * r, col = c0 + li;
-            if (row >= lo && row <= col && col < n) A[(int64_t)row * n + col] = cx<double>{are[r], aim[r]};
+            const int row = r0 + Mf::row(lg, r), col = c0 + li;
+            if (row >= lo && row <= col && col < n) A[(int64_t)row * n + col] = cx<R>{are[r], aim[r]};
         }
         if (!more) break;
         r0 = r1;
