@@ -78,7 +78,7 @@ hipError_t launch_inertia_big(int bmax, int nmax, const lres::WinDesc* wins, con
 #define LRES_MFMA_PART_DECL(r, nt)                                                                              \
     hipError_t lres_mfma_part_##r##_##nt(int nwc, const lres::WinDesc* wins, const int32_t* ids, int nids,     \
                                          const double* SW, const double* HW, int M, int ngroup, double c,     \
-                                         double Rr, double* Dpart, double* Fpart, hipStream_t st);
+                                         double Rr, double* Dpart, double* Fpart, int acc, hipStream_t st);
 #define LRES_MFMA_PARTS(X) X(float, 2) X(float, 3) X(float, 4) X(float, 5) X(float, 6) X(float, 7) X(float, 8) \
     X(float, 9) X(float, 10) X(float, 11) X(float, 12) X(float, 13) X(float, 14) X(float, 15) X(float, 16)     \
     X(double, 2) X(double, 3) X(double, 4) X(double, 5) X(double, 6) X(double, 7) X(double, 8)
@@ -93,9 +93,11 @@ LRES_MFMA_INERTIA_PARTS(LRES_MFMA_INERTIA_DECL)
 hipError_t launch_inertia_mfma(int nt, const lres::WinDesc* wins, const int32_t* ids, int nids, const double* SW,
                                const double* HW, const double* sigma, int nshift, int32_t* counts,
                                unsigned long long* total, hipStream_t st);
+// acc > 0 (fp64): the group partials are accumulated over the group's poles in LDS, as
+// packed upper triangles of acc entries (at least k(k+1)/2 of every window launched)
 hipError_t launch_pole_mfma(int fp32, int nt, int nwc, const lres::WinDesc* wins, const int32_t* ids, int nids,
                             const double* SW, const double* HW, int M, int ngroup, double c, double Rr,
-                            double* Dpart, double* Fpart, hipStream_t st);
+                            double* Dpart, double* Fpart, int acc, hipStream_t st);
 // accumulator registers per lane of one wave of the MFMA pole kernel
 int mfma_acc_regs(int fp32, int nt, int nwc);
 // scratch bytes per lane of an MFMA variant (register spills; -1: no such variant)

@@ -941,11 +941,11 @@ static bool mfma_nwc_ok(int nt, int nwc) {
 
 hipError_t launch_pole_mfma(int fp32, int nt, int nwc, const WinDesc* wins, const int32_t* ids, int nids,
                             const double* SW, const double* HW, int M, int ngroup, double c, double Rr,
-                            double* Dpart, double* Fpart, hipStream_t st) {
+                            double* Dpart, double* Fpart, int acc, hipStream_t st) {
     if (!mfma_nwc_ok(nt, nwc)) return hipErrorInvalidValue;
 #define LRES_MFMA_CALL(r, NT)                                                                                  \
     if (nt == NT && (fp32 != 0) == (sizeof(r) == 4))                                                           \
-        return lres_mfma_part_##r##_##NT(nwc, wins, ids, nids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart, st);
+        return lres_mfma_part_##r##_##NT(nwc, wins, ids, nids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart, acc, st);
     LRES_MFMA_PARTS(LRES_MFMA_CALL)
 #undef LRES_MFMA_CALL
     return hipErrorInvalidValue;
@@ -966,7 +966,7 @@ int mfma_scratch_bytes(int fp32, int nt, int nwc) {
 #define LRES_MFMA_CALL(r, NT)                                                                                  \
     if (nt == NT && (fp32 != 0) == (sizeof(r) == 4))                                                           \
         return (int)lres_mfma_part_##r##_##NT(-nwc, nullptr, nullptr, 0, nullptr, nullptr, 0, 0, 0.0, 0.0,    \
-                                              nullptr, nullptr, nullptr);
+                                              nullptr, nullptr, 0, nullptr);
     LRES_MFMA_PARTS(LRES_MFMA_CALL)
 #undef LRES_MFMA_CALL
     return -1;

@@ -143,7 +143,7 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
                                                         const double* __restrict__ SW,
                                                         const double* __restrict__ HW, int M, int ngroup, double c,
                                                         double Rr, double* __restrict__ Dpart,
-                                                        double* __restrict__ Fpart) {
+                                                        double* __restrict__ Fpart, int acc) {
     using MF = Mfma<R>;
     using V = typename MF::V;
     constexpr int TJ = (NT + NWC - 1) / NWC;   // tile columns per wave
@@ -157,6 +157,10 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
     constexpr int RS = sizeof(R) == 8 ? 5 : 4;
     __shared__ __attribute__((aligned(16))) R rpr[2][NR][RS];
     __shared__ __attribute__((aligned(16))) R rpi[2][NR][RS];
+    // fp64 with acc > 0: the group's D and F partial sums over its poles, packed upper
+    // triangles of acc entries each (dynamic LDS), stored to Dpart / Fpart once at the
+    // end instead of a read-modify-write of global memory per pole
+    extern __shared__ double pacc[];
     const int w = ids[blockIdx.x];
     const int gq = blockIdx.y;
     const WinDesc d = wins[w];
@@ -375,6 +379,15 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
                         const int64_t q = (int64_t)row * kk + col;
                         if constexpr (sizeof(R) == 4) {
                             gs[(int64_t)jp * kk * kk + q] = make_float2((float)dv, (float)fv);
+                        } else if (acc > 0) {
+                            const int u = row * kk - ((row * (row - 1)) >> 1) + (col - row);
+                            if (first) {
+                                pacc[u] = dv;
+                                pacc[acc + u] = fv;
+                            } else {
+                                pacc[u] += dv;
+                                pacc[acc + u] += fv;
+                            }
                         } else if (first) {
                             dp[q] = dv;
                             fp[q] = fv;
@@ -386,6 +399,29 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
                 }
             }
         first = false;
+    }
+    if constexpr (sizeof(R) == 8) {
+        // each lane stores the entries it accumulated (no barrier: the same lane wrote them)
+        if (acc > 0) {
+            const int ei = lane & 15, eg = lane >> 4;
+#pragma unroll
+            for (int I = 0; I < NT; ++I)
+#pragma unroll
+                for (int j = 0; j < TJ; ++j) {
+                    if (I >= held(j) || !owned(j)) continue;
+                    const int col = 16 * (wv + NWC * j) + ei - m;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = 16 * I + MF::row(eg, r) - m;
+                        if (row >= 0 && row <= col && col < k) {
+                            const int u = row * k - ((row * (row - 1)) >> 1) + (col - row);
+                            const int64_t q = (int64_t)row * k + col;
+                            dp[q] = pacc[u];
+                            fp[q] = pacc[acc + u];
+                        }
+                    }
+                }
+        }
     }
 }
 
@@ -562,10 +598,22 @@ constexpr int mfma_waves() {
 }
 
 template <typename R, int NT, int NWC>
-void launch_one(const WinDesc* wins, const int32_t* ids, int nids, const double* SW, const double* HW, int M,
-                int ngroup, double c, double Rr, double* Dpart, double* Fpart, hipStream_t st) {
-    hipLaunchKernelGGL((k_pole_mfma<R, NT, NWC, mfma_waves<R, NT, NWC>()>), dim3(nids, ngroup), dim3(64 * NWC), 0,
-                       st, wins, ids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart);
+hipError_t launch_one(const WinDesc* wins, const int32_t* ids, int nids, const double* SW, const double* HW, int M,
+                      int ngroup, double c, double Rr, double* Dpart, double* Fpart, int acc, hipStream_t st) {
+    const auto kern = k_pole_mfma<R, NT, NWC, mfma_waves<R, NT, NWC>()>;
+    const size_t lds = sizeof(R) == 8 && acc > 0 ? (size_t)2 * acc * sizeof(double) : 0;
+    if (lds > 0) {
+        static bool raised = false;   // the dynamic LDS limit, raised once per variant
+        if (!raised) {
+            const hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                     (int)lds > 65536 ? (int)lds : 65536);
+            if (e != hipSuccess) return e;
+            raised = true;
+        }
+    }
+    hipLaunchKernelGGL(kern, dim3(nids, ngroup), dim3(64 * NWC), lds, st, wins, ids, SW, HW, M, ngroup, c, Rr, Dpart,
+                       Fpart, lds > 0 ? acc : 0);
+    return hipGetLastError();
 }
 
 // private (scratch) bytes per lane of a variant: nonzero means register spills
@@ -598,15 +646,13 @@ int scratch_one() {
 // scratch bytes per lane of the -nwc variant (-1 if it does not exist)
 hipError_t LRES_MFMA_NAME(LRES_MFMA_R, LRES_MFMA_NT)(int nwc, const lres::WinDesc* wins, const int32_t* ids, int nids,
                                                      const double* SW, const double* HW, int M, int ngroup, double c,
-                                                     double Rr, double* Dpart, double* Fpart, hipStream_t st) {
+                                                     double Rr, double* Dpart, double* Fpart, int acc, hipStream_t st) {
     using R = LRES_MFMA_R;
     constexpr int NT = LRES_MFMA_NT;
 #define LRES_MFMA_CASE(W)                                                                                     \
     if constexpr (W <= NT) {                                                                                  \
-        if (nwc == W) {                                                                                       \
-            lres::launch_one<R, NT, W>(wins, ids, nids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart, st);          \
-            return hipGetLastError();                                                                         \
-        }                                                                                                     \
+        if (nwc == W)                                                                                         \
+            return lres::launch_one<R, NT, W>(wins, ids, nids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart, acc, st); \
         if (nwc == -W) return (hipError_t)lres::scratch_one<R, NT, W>();                                      \
     }
     LRES_MFMA_FOR_NWC(LRES_MFMA_CASE)

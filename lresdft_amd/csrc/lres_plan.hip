@@ -130,9 +130,9 @@ Shape default_pole_shape(int nmax, int fp32) {
 
 hipError_t launch_pole(const Shape& sh, int fp32, const WinDesc* wins, const int32_t* ids, int nids, const double* SW,
                        const double* HW, int M, int ngroup, double c, double Rr, double* Dpart, double* Fpart,
-                       hipStream_t st) {
+                       hipStream_t st, int acc = 0) {
     if (sh.wide == kMfma)
-        return launch_pole_mfma(fp32, sh.T, sh.G, wins, ids, nids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart, st);
+        return launch_pole_mfma(fp32, sh.T, sh.G, wins, ids, nids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart, acc, st);
     return launch_pole_gj(sh.T, sh.G, sh.wide, fp32, wins, ids, nids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart, st);
 }
 
@@ -165,8 +165,10 @@ struct lres_plan {
         Shape ishape;          // inertia kernel shape
         bool big = false;      // n above the register-resident limit: blocked sweep in a workspace
         int bmax = 0;          // panel width of the blocked sweep
+        int kmax = 0;          // largest |T_B| of the bucket
     };
     std::vector<Bucket> buckets;
+    bool acc_lds = true;       // fp64 group partials accumulated in LDS (LRES_POLE_ACC != "global")
     bool autotune = true;
     bool tune_log = false;
     DevBuf<WinDesc> d_wins;
@@ -285,6 +287,17 @@ double* pole_partials(lres_plan* P, const Shape& sh, int fp32) {
     return reinterpret_cast<double*>(P->d_slots.p);
 }
 
+// fp64 matrix-core launches with more than one pole per group accumulate the group's
+// D and F partial sums in LDS (packed upper triangles, 16 B per entry) and store them
+// once, when that fits beside a second workgroup on the CU (at most 64 KB: |T_B| <= 89);
+// the same additions in the same order as the read-modify-write of global memory, so
+// the same bits.  LRES_POLE_ACC=global keeps the read-modify-write.
+int pole_acc_entries(const lres_plan* P, const lres_plan::Bucket& b, const Shape& sh, int fp32) {
+    if (fp32 || sh.wide != kMfma || P->M / 2 <= P->ngroup || !P->acc_lds) return 0;
+    const int u = b.kmax * (b.kmax + 1) / 2;
+    return (size_t)u * 16 <= 64 * 1024 ? u : 0;
+}
+
 // Autotuner: time every valid shape of a bucket on the loaded H (two launches
 // after one warm-up, HIP events) and keep the fastest.  All shapes apply the
 // same operations to every matrix entry, so the choice never changes results.
@@ -302,7 +315,7 @@ void tune_bucket(lres_plan* P, lres_plan::Bucket& b, int fp32, double c, double 
         auto launch = [&] {
             hip_check(launch_pole(sh, fp32, P->d_wins.p, P->d_bucket_ids.p + b.id_off, (int)b.ids.size(), P->d_SW.p,
                                   P->d_HW.p, P->M, P->ngroup, c, R, pole_partials(P, sh, fp32), P->d_Fpart.p,
-                                  P->st),
+                                  P->st, pole_acc_entries(P, b, sh, fp32)),
                       "pole kernel (tuning)");
         };
         launch();
@@ -345,7 +358,7 @@ void run_pass(lres_plan* P, double mu, double e_lo, double* d_dens, double* d_NB
         if (!b.tuned[fp32]) tune_bucket(P, b, fp32, c, R);
         hip_check(launch_pole(b.shape[fp32], fp32, P->d_wins.p, P->d_bucket_ids.p + b.id_off, (int)b.ids.size(),
                               P->d_SW.p, P->d_HW.p, P->M, P->ngroup, c, R, pole_partials(P, b.shape[fp32], fp32),
-                              P->d_Fpart.p, P->st),
+                              P->d_Fpart.p, P->st, pole_acc_entries(P, b, b.shape[fp32], fp32)),
                   "pole kernel");
         if (fp32 && b.shape[fp32].wide == kMfma)
             for (int32_t w : b.ids) by_pole[w] = 1;
@@ -755,6 +768,7 @@ int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, 
                 auto& b = P->buckets.back();
                 b.ids.push_back(w);
                 b.nmax = std::max(b.nmax, P->n_of[w]);
+                b.kmax = std::max(b.kmax, P->k_of[w]);
             }
         }
         std::vector<int32_t> ids;
@@ -789,6 +803,8 @@ int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, 
         for (const auto& b : P->buckets)
             if (b.big) ws_len = std::max<int64_t>(ws_len, (int64_t)b.ids.size() * b.nmax * b.nmax);
         if (ws_len) P->d_work.alloc((size_t)ws_len);
+        const char* pa = std::getenv("LRES_POLE_ACC");
+        P->acc_lds = !(pa && std::string(pa) == "global");
         const char* at = std::getenv("LRES_AUTOTUNE");
         P->autotune = !(at && at[0] == '0');
         const char* tl = std::getenv("LRES_TUNE_LOG");

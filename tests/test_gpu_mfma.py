@@ -135,6 +135,33 @@ def test_every_wave_split_gives_the_same_bits(sys3, precision):
             assert ns == ref[3] and es == ref[4]
 
 
+def test_fp64_group_sums_in_lds_equal_global_read_modify_write():
+    """216 H2 (n_W 84, |T_B| <= 84), M 64: 10 pole groups per window, 3-4 poles each, so the
+    fp64 matrix-core kernel accumulates each group's D / F partial sums in LDS and stores them
+    once.  The pass equals the one with the read-modify-write of global memory per pole
+    (LRES_POLE_ACC=global) bit for bit, at three wave splits."""
+    print("[gpu] fp64 group sums in LDS vs global, n=6 M=64", flush=True)
+    cr = h2_crystal((6, 6, 6), a=6.0, pts_per_cell=10, rc=0.6)
+    b = LCAOBasis(cr, **BASIS)
+    v = guess_v(cr)
+    cells = cell_centres((6, 6, 6), 6.0)
+    H = b.H(v)
+    W, _ = build_windows(b, cells, 8.0)
+    e_lo, mu = _mu_window(H, b.S, W[:8])
+    ref = None
+    for nwc in (2, 3, 4):
+        outs = [_pass(b, cells, H, mu, e_lo, M=64, env={"LRES_MFMA_WAVES": str(nwc), "LRES_POLE_KERNEL": "mfma",
+                                                        **extra})
+                for extra in ({}, {"LRES_POLE_ACC": "global"})]
+        for a, g in zip(outs[0][:3], outs[1][:3]):
+            assert np.array_equal(a, g), nwc
+        assert outs[0][3] == outs[1][3] and outs[0][4] == outs[1][4]
+        if ref is None:
+            ref = outs[0]
+        else:
+            assert all(np.array_equal(x, y) for x, y in zip(outs[0][:3], ref[:3])), nwc
+
+
 def test_mfma_uneven_windows_match_oracle():
     """Random centres: windows of 36 to 96 functions, several buckets and tile counts."""
     from lresdft_amd import system
