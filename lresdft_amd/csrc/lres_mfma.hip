@@ -141,8 +141,8 @@ template <typename R, int NT, int NWC, int WPE>
 __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(WPE))) k_pole_mfma(const WinDesc* __restrict__ wins,
                                                         const int32_t* __restrict__ ids,
                                                         const double* __restrict__ SW,
-                                                        const double* __restrict__ HW, int M, int ngroup, double c,
-                                                        double Rr, double* __restrict__ Dpart,
+                                                        const double* __restrict__ HW, int nids, int M, int ngroup,
+                                                        double c, double Rr, double* __restrict__ Dpart,
                                                         double* __restrict__ Fpart, int acc) {
     using MF = Mfma<R>;
     using V = typename MF::V;
@@ -161,8 +161,15 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
     // triangles of acc entries each (dynamic LDS), stored to Dpart / Fpart once at the
     // end instead of a read-modify-write of global memory per pole
     extern __shared__ double pacc[];
-    const int w = ids[blockIdx.x];
-    const int gq = blockIdx.y;
+    // XCD-aware order: workgroups are dealt to the 8 XCDs round-robin by linear id, so
+    // id L -> window slot 8 (L / (8 ngroup)) + L % 8, group (L / 8) % ngroup puts every
+    // group of a window on one XCD, dispatched together: the window's S_W and H_W are
+    // then read from HBM about once per pass and from that XCD's L2 by its other groups
+    const int L = blockIdx.x;
+    const int slot = (L / (8 * ngroup)) * 8 + (L & 7);
+    const int gq = (L >> 3) % ngroup;
+    if (slot >= nids) return;
+    const int w = ids[slot];
     const WinDesc d = wins[w];
     const int n = d.n, m = d.m, k = d.k;
     const int lane = threadIdx.x & 63;
@@ -611,7 +618,8 @@ hipError_t launch_one(const WinDesc* wins, const int32_t* ids, int nids, const d
             raised = true;
         }
     }
-    hipLaunchKernelGGL(kern, dim3(nids, ngroup), dim3(64 * NWC), lds, st, wins, ids, SW, HW, M, ngroup, c, Rr, Dpart,
+    const int nblk = (nids + 7) / 8 * 8 * ngroup;   // one id per (window slot, group); slots past nids exit
+    hipLaunchKernelGGL(kern, dim3(nblk), dim3(64 * NWC), lds, st, wins, ids, SW, HW, nids, M, ngroup, c, Rr, Dpart,
                        Fpart, lds > 0 ? acc : 0);
     return hipGetLastError();
 }
