@@ -1602,14 +1602,19 @@ constexpr int kBigThreads = 1024;
 
 template <typename R>
 __global__ void __launch_bounds__(kBigThreads) k_pole_big(const WinDesc* __restrict__ wins, const int32_t* __restrict__ ids,
-                                                  const double* __restrict__ SW, const double* __restrict__ HW,
+                                                  int nids, const double* __restrict__ SW, const double* __restrict__ HW,
                                                   int M, int ngroup, double c, double Rr, int bmax,
                                                   cx<double>* __restrict__ work, const int64_t* __restrict__ ws_off,
                                                   double* __restrict__ Dpart, double* __restrict__ Fpart) {
     extern __shared__ double lds_raw[];
     using E = cx<R>;
-    const int w = ids[blockIdx.x];
-    const int g = blockIdx.y;
+    // XCD-aware order (workgroups go to the 8 XCDs round-robin by id): every pole group
+    // of a window on one XCD, dispatched together, so S_W and H_W come from its L2
+    const int L = blockIdx.x;
+    const int slot = (L / (8 * ngroup)) * 8 + (L & 7);
+    const int g = (L >> 3) % ngroup;
+    if (slot >= nids) return;
+    const int w = ids[slot];
     const WinDesc d = wins[w];
     const int n = d.n, m = d.m, k = d.k;
     E* A = reinterpret_cast<E*>(work + ws_off[w] + (int64_t)g * n * n);
@@ -1706,17 +1711,17 @@ hipError_t launch_pole_big(int fp32, int bmax, int nmax, const WinDesc* wins, co
                            const double* SW, const double* HW, int M, int ngroup, double c, double Rr, void* work,
                            const int64_t* ws_off, double* Dpart, double* Fpart, hipStream_t st) {
     const size_t lds = big_lds_bytes(nmax, bmax, fp32 ? 8 : 16);
-    dim3 grid(nids, ngroup), blk(kBigThreads);
+    dim3 grid((nids + 7) / 8 * 8 * ngroup), blk(kBigThreads);   // ids past nids exit at once
     hipError_t e;
     if (fp32) {
         e = hipFuncSetAttribute((const void*)k_pole_big<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_pole_big<float>, grid, blk, lds, st, wins, ids, SW, HW, M, ngroup, c, Rr, bmax,
+        hipLaunchKernelGGL(k_pole_big<float>, grid, blk, lds, st, wins, ids, nids, SW, HW, M, ngroup, c, Rr, bmax,
                            (cx<double>*)work, ws_off, Dpart, Fpart);
     } else {
         e = hipFuncSetAttribute((const void*)k_pole_big<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_pole_big<double>, grid, blk, lds, st, wins, ids, SW, HW, M, ngroup, c, Rr, bmax,
+        hipLaunchKernelGGL(k_pole_big<double>, grid, blk, lds, st, wins, ids, nids, SW, HW, M, ngroup, c, Rr, bmax,
                            (cx<double>*)work, ws_off, Dpart, Fpart);
     }
     return hipGetLastError();
