@@ -48,7 +48,7 @@ def parse():
                     help="launch every SCF iteration's mixing/potential/H(v) step eagerly instead of "
                          "replaying it from a captured hipGraph")
     ap.add_argument("--no-forces", action="store_true",
-                    help="skip the adjoint / band-energy forces at the converged potential (after the timed region)")
+                    help="skip the adjoint and the total-energy forces at the converged SCF (after the timed region)")
     ap.add_argument("--forces-fd", type=int, default=0, metavar="K",
                     help="also check the forces of K seeded random atoms against central differences of the "
                          "fixed-N band energy (steps 1e-4 / 5e-5 bohr, Richardson)")
@@ -223,10 +223,10 @@ def forces_fd(C, v, F, cells, args, K, seed=11):
 
 
 def adjoint_timing(C, grid, vion, res, cells, args):
-    """The adjoint kernels (fp64) and the fixed-N band-energy forces at the converged
-    input potential of an SCF result, timed after the benchmark's timed region."""
+    """The adjoint kernels (fp64) and the total-energy forces at the converged input
+    density of an SCF result, timed after the benchmark's timed region."""
     import torch
-    from lresdft_amd.forces import band_forces
+    from lresdft_amd.forces import total_forces
     C.set_precision("fp64")
     v = grid.potential(res["dens_in"], vion)
     C.plan.set_potential(v)
@@ -236,15 +236,20 @@ def adjoint_timing(C, grid, vion, res, cells, args):
     C.plan.adjoint(C.mu, C.e_lo)
     torch.cuda.synchronize()
     t_adj = time.perf_counter() - t
+    total_forces(C, grid, vion, res["dens_in"])   # first call builds the per-atom boxes
+    torch.cuda.synchronize()
     t = time.perf_counter()
-    vh = v.cpu().numpy()
-    F, parts = band_forces(C, vh)
+    F, parts = total_forces(C, grid, vion, res["dens_in"])
+    torch.cuda.synchronize()
     t_f = time.perf_counter() - t
-    fd = forces_fd(C, vh, F, cells, args, args.forces_fd) if args.forces_fd else None
-    return {"adjoint_ms": 1e3 * t_adj, "forces_s": t_f, "max_abs_force": float(abs(F).max()), "fd_check": fd,
-            "note": ("adjoint = d(2 sum E_B)/d(H_W, S_W) on the GPU (fp64, all windows); forces = fixed-N "
-                     "band-energy forces with the converged grid potential frozen: adjoint + D_B/F_B on the GPU, "
-                     "chain rule to atoms on the host")}
+    fd = forces_fd(C, v.cpu().numpy(), parts["band"], cells, args, args.forces_fd) if args.forces_fd else None
+    return {"adjoint_ms": 1e3 * t_adj, "forces_s": t_f, "max_abs_force": float(abs(F).max()),
+            "max_abs_band_force": float(abs(parts["band"]).max()), "fd_check": fd,
+            "note": ("adjoint = d(2 sum E_B)/d(H_W, S_W) on the GPU (fp64, all windows); forces = -dE/dR of the "
+                     "Harris-Foulkes SCF energy at the converged input density: fixed-N band-energy forces with the "
+                     "potential frozen (beta from two passes, adjoint, contraction with the basis derivatives in "
+                     "per-atom boxes, all on the GPU) plus the ion-potential response (FFT); fd_check: central "
+                     "differences of the band energy along random directions")}
 
 
 def profile_passes(C, cr, grid, vion, K):
