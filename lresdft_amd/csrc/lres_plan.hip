@@ -215,6 +215,12 @@ struct lres_plan {
     DevBuf<int32_t> d_fpA, d_fpB, d_fent_n, d_fowner, d_frow, d_ftpos;
     DevBuf<double> d_fpos, d_fexps, d_fL, d_fX, d_fDX, d_fTX, d_fGH, d_fGS, d_fpair, d_fq;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // side streams (default priority) over which independent per-bucket launches of a
+    // pass or an inertia count are spread, with their fork and join events; created on
+    // first use on the plan's device
+    static constexpr int kSide = 4;
+    hipStream_t side[kSide] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[kSide] = {};
     double timing[3] = {0, 0, 0};
     // cumulative pole-kernel statistics per precision: passes and milliseconds (HIP events)
     int64_t stat_passes[2] = {0, 0};
@@ -230,11 +236,39 @@ struct lres_plan {
     ~lres_plan() {
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
+        for (auto& e : ev_join)
+            if (e) (void)hipEventDestroy(e);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        for (auto& q : side)
+            if (q) (void)hipStreamDestroy(q);
         if (h_pin) (void)hipHostFree(h_pin);
     }
 };
 
 namespace {
+
+// Independent launches spread over the plan's side streams: side_fork makes every
+// side stream wait for the work queued so far on the plan stream, side_stream(P, i) is
+// the stream of the i-th launch, side_join makes the plan stream wait for the first
+// `used` side streams.  Each launch writes outputs of its own windows only.
+void side_fork(lres_plan* P) {
+    if (!P->ev_fork) {
+        hip_check(hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming), "event");
+        for (int i = 0; i < lres_plan::kSide; ++i) {
+            hip_check(hipStreamCreateWithFlags(&P->side[i], hipStreamNonBlocking), "stream");
+            hip_check(hipEventCreateWithFlags(&P->ev_join[i], hipEventDisableTiming), "event");
+        }
+    }
+    hip_check(hipEventRecord(P->ev_fork, P->st), "event");
+    for (int i = 0; i < lres_plan::kSide; ++i) hip_check(hipStreamWaitEvent(P->side[i], P->ev_fork, 0), "wait");
+}
+hipStream_t side_stream(const lres_plan* P, int i) { return P->side[i % lres_plan::kSide]; }
+void side_join(lres_plan* P, int used) {
+    for (int i = 0; i < std::min(used, lres_plan::kSide); ++i) {
+        hip_check(hipEventRecord(P->ev_join[i], P->side[i]), "event");
+        hip_check(hipStreamWaitEvent(P->st, P->ev_join[i], 0), "wait");
+    }
+}
 
 int fail_from(const Fail& f) {
     g_err = f.msg;
@@ -339,6 +373,40 @@ void tune_bucket(lres_plan* P, lres_plan::Bucket& b, int fp32, double c, double 
     (void)hipEventDestroy(e1);
 }
 
+// The buckets whose pole solves run on the blocked sweep, as one span of the size-ordered
+// id list: first = index of the first such bucket (-1 when they are not a tail of the
+// bucket list, or there are none), with the span's ids, largest n and narrowest panel.
+struct BigSpan {
+    int first = -1;
+    int64_t id_off = 0;
+    int nids = 0, nmax = 0, bmax = 0;
+};
+BigSpan big_span(const lres_plan* P, int fp32) {
+    BigSpan sp;
+    const int nb = (int)P->buckets.size();
+    int first = nb;
+    while (first > 0) {
+        const auto& b = P->buckets[first - 1];
+        if (!(b.big && !big_on_mfma(b.nmax, fp32))) break;
+        --first;
+    }
+    if (first == nb) return sp;
+    for (int i = 0; i < first; ++i) {
+        const auto& b = P->buckets[i];
+        if (b.big && !big_on_mfma(b.nmax, fp32)) return sp;   // not a tail: per-bucket launches
+    }
+    sp.first = first;
+    sp.id_off = P->buckets[first].id_off;
+    sp.bmax = P->buckets[first].bmax;
+    for (int i = first; i < nb; ++i) {
+        const auto& b = P->buckets[i];
+        sp.nids += (int)b.ids.size();
+        sp.nmax = std::max(sp.nmax, b.nmax);
+        sp.bmax = std::min(sp.bmax, b.bmax);
+    }
+    return sp;
+}
+
 // pole solves + reduction (+ density) at (mu, e_lo); sums to host if requested
 void run_pass(lres_plan* P, double mu, double e_lo, double* d_dens, double* d_NB, double* d_EB, double* h_sums) {
     require(P->have_H, "no Hamiltonian loaded: call lres_set_hamiltonian_csr first");
@@ -347,22 +415,50 @@ void run_pass(lres_plan* P, double mu, double e_lo, double* d_dens, double* d_NB
     hip_check(hipEventRecord(P->ev[2], P->st), "event");
     const int fp32 = P->precision == LRES_MIXED ? 1 : 0;
     std::vector<int8_t> by_pole(P->nwin, 0);
+    // the buckets on the blocked sweep (size-generic: every window's n comes from its
+    // descriptor) go in one launch when they are the tail of the size-ordered id list, so
+    // that a spread of window sizes still fills the chip
+    const BigSpan big = big_span(P, fp32);
+    // the register-kernel buckets, tuned first (the autotuner times on the plan stream);
+    // with more than one launch in the pass, the buckets run concurrently on the side
+    // streams (each writes its own windows' partial sums) beside the blocked launch
+    std::vector<lres_plan::Bucket*> reg;
+    int nlaunch = big.first >= 0 ? 1 : 0;
     for (auto& b : P->buckets) {
         if (b.big && !big_on_mfma(b.nmax, fp32)) {
-            hip_check(launch_pole_big(fp32, b.bmax, b.nmax, P->d_wins.p, P->d_bucket_ids.p + b.id_off,
-                                      (int)b.ids.size(), P->d_SW.p, P->d_HW.p, P->M, P->ngroup, c, R, P->d_work.p,
-                                      P->d_ws_off.p, P->d_Dpart.p, P->d_Fpart.p, P->st),
-                      "blocked pole kernel");
+            if (big.first < 0) ++nlaunch;
             continue;
         }
         if (!b.tuned[fp32]) tune_bucket(P, b, fp32, c, R);
+        reg.push_back(&b);
+        ++nlaunch;
+    }
+    const bool spread = nlaunch > 1 && !reg.empty();
+    if (spread) side_fork(P);
+    if (big.first >= 0)
+        hip_check(launch_pole_big(fp32, big.bmax, big.nmax, P->d_wins.p, P->d_bucket_ids.p + big.id_off, big.nids,
+                                  P->d_SW.p, P->d_HW.p, P->M, P->ngroup, c, R, P->d_work.p, P->d_ws_off.p,
+                                  P->d_Dpart.p, P->d_Fpart.p, P->st),
+                  "blocked pole kernel");
+    else
+        for (auto& b : P->buckets)
+            if (b.big && !big_on_mfma(b.nmax, fp32))
+                hip_check(launch_pole_big(fp32, b.bmax, b.nmax, P->d_wins.p, P->d_bucket_ids.p + b.id_off,
+                                          (int)b.ids.size(), P->d_SW.p, P->d_HW.p, P->M, P->ngroup, c, R,
+                                          P->d_work.p, P->d_ws_off.p, P->d_Dpart.p, P->d_Fpart.p, P->st),
+                          "blocked pole kernel");
+    int used = 0;
+    for (auto* bp : reg) {
+        auto& b = *bp;
+        const hipStream_t sq = spread ? side_stream(P, used++) : P->st;
         hip_check(launch_pole(b.shape[fp32], fp32, P->d_wins.p, P->d_bucket_ids.p + b.id_off, (int)b.ids.size(),
                               P->d_SW.p, P->d_HW.p, P->M, P->ngroup, c, R, pole_partials(P, b.shape[fp32], fp32),
-                              P->d_Fpart.p, P->st, pole_acc_entries(P, b, b.shape[fp32], fp32)),
+                              P->d_Fpart.p, sq, pole_acc_entries(P, b, b.shape[fp32], fp32)),
                   "pole kernel");
         if (fp32 && b.shape[fp32].wide == kMfma)
             for (int32_t w : b.ids) by_pole[w] = 1;
     }
+    if (spread) side_join(P, used);
     // complex64 passes only (fp64 partials are always group sums); changes only
     // when a bucket's kernel family does
     if (fp32 && by_pole != P->by_pole) {
@@ -432,24 +528,58 @@ void inertia_launch(lres_plan* P, const double* sig, int ns, int32_t* d_counts,
         if (sub.empty()) sub.push_back(0);   // keeps the upload non-empty; no launch reads it
         P->d_cand.upload(sub.data(), sub.size(), P->st);
     }
+    // the big buckets (a tail of the size order) in one launch on the plan stream, as in
+    // run_pass; the other buckets concurrently on the side streams when there are several
+    // launches (integer totals by atomics, per-window counts disjoint)
+    size_t first_big = P->buckets.size();
+    while (first_big > 0 && P->buckets[first_big - 1].big) --first_big;
+    bool merged = first_big < P->buckets.size();
+    for (size_t bi = 0; bi < first_big; ++bi) merged = merged && !P->buckets[bi].big;
+    int nother = merged ? 1 : 0;
+    for (size_t bi = 0; bi < P->buckets.size(); ++bi) {
+        const int nids = cand ? (int)(sub_off[bi + 1] - sub_off[bi]) : (int)P->buckets[bi].ids.size();
+        if (nids > 0 && !(merged && P->buckets[bi].big)) ++nother;
+    }
+    const bool spread = nother > 1;
+    if (spread) side_fork(P);
+    if (merged) {
+        int nmax = 0, bmax = 1 << 30;
+        for (size_t bi = first_big; bi < P->buckets.size(); ++bi) {
+            nmax = std::max(nmax, P->buckets[bi].nmax);
+            bmax = std::min(bmax, P->buckets[bi].bmax);
+        }
+        const int32_t* ids = cand ? P->d_cand.p + sub_off[first_big] : P->d_bucket_ids.p + P->buckets[first_big].id_off;
+        int nids = 0;
+        for (size_t bi = first_big; bi < P->buckets.size(); ++bi)
+            nids += cand ? (int)(sub_off[bi + 1] - sub_off[bi]) : (int)P->buckets[bi].ids.size();
+        if (nids > 0)
+            hip_check(launch_inertia_big(bmax, nmax, P->d_wins.p, ids, nids, P->d_SW.p, P->d_HW.p, d_sig, ns,
+                                         P->d_work.p, (int64_t)P->d_work.n, d_counts, P->d_itot.p, P->st),
+                      "blocked inertia kernel");
+    }
+    int used = 0;
     for (size_t bi = 0; bi < P->buckets.size(); ++bi) {
         const auto& b = P->buckets[bi];
         const int32_t* ids = cand ? P->d_cand.p + sub_off[bi] : P->d_bucket_ids.p + b.id_off;
         const int nids = cand ? (int)(sub_off[bi + 1] - sub_off[bi]) : (int)b.ids.size();
-        if (nids == 0) continue;
-        if (b.big)
+        if (nids == 0 || (merged && b.big)) continue;
+        if (b.big) {   // blocked buckets share the workspace: they stay on the plan stream
             hip_check(launch_inertia_big(b.bmax, b.nmax, P->d_wins.p, ids, nids, P->d_SW.p, P->d_HW.p, d_sig,
                                          ns, P->d_work.p, (int64_t)P->d_work.n, d_counts, P->d_itot.p, P->st),
                       "blocked inertia kernel");
-        else if (inertia_on_mfma(b.nmax))
+            continue;
+        }
+        const hipStream_t sq = spread ? side_stream(P, used++) : P->st;
+        if (inertia_on_mfma(b.nmax))
             hip_check(launch_inertia_mfma((b.nmax + 15) / 16, P->d_wins.p, ids, nids, P->d_SW.p, P->d_HW.p,
-                                          d_sig, ns, d_counts, P->d_itot.p, P->st),
+                                          d_sig, ns, d_counts, P->d_itot.p, sq),
                       "matrix-core inertia kernel");
         else
             hip_check(launch_inertia(b.ishape.T, b.ishape.G, P->d_wins.p, ids, nids, P->d_SW.p, P->d_HW.p,
-                                     d_sig, ns, d_counts, P->d_itot.p, P->st),
+                                     d_sig, ns, d_counts, P->d_itot.p, sq),
                       "inertia kernel");
     }
+    if (spread) side_join(P, used);
 }
 
 void inertia(lres_plan* P, const double* sig, int ns, int64_t* h_total, int32_t* d_counts,
@@ -797,11 +927,18 @@ int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, 
             }
         }
         P->d_ws_off.upload(ws_off.data(), nwin, P->st);
-        // the blocked inertia kernel cuts the same workspace into real nmax^2 slots and
-        // needs one per window of a bucket at least
+        // the blocked inertia kernel cuts the same workspace into real slots of the largest
+        // blocked window's n^2 and needs one per blocked window at least (its launch takes
+        // every blocked bucket at once)
         int64_t ws_len = ws_total * 2;
+        int64_t nbig = 0;
+        int nmax_big = 0;
         for (const auto& b : P->buckets)
-            if (b.big) ws_len = std::max<int64_t>(ws_len, (int64_t)b.ids.size() * b.nmax * b.nmax);
+            if (b.big) {
+                nbig += (int64_t)b.ids.size();
+                nmax_big = std::max(nmax_big, b.nmax);
+            }
+        ws_len = std::max<int64_t>(ws_len, nbig * nmax_big * nmax_big);
         if (ws_len) P->d_work.alloc((size_t)ws_len);
         const char* pa = std::getenv("LRES_POLE_ACC");
         P->acc_lds = !(pa && std::string(pa) == "global");
