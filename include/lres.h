@@ -224,6 +224,34 @@ int lres_mu_solve(lres_plan* plan, double N, double e_lo, double lo, double hi, 
 int lres_mu_solve_warm(lres_plan* plan, double N, double e_lo, double mu0, double f0, double slope, double ftol,
                        double xtol, double* h_mu, double* h_info);
 
+/* The warm density step of LRFTLCAOCpp._density (lcao3d.py:412-448: keep the filter
+ * edge, keep mu, or re-solve mu, then the density) queued on the plan stream with the
+ * decisions taken on the device, so that the host waits once per step instead of once
+ * per trial pass:
+ *   - the keep-edge test: inertia counts at e_lo + margin / 3 and e_lo + 2 margin
+ *     (lcao3d.py:420; the edge is kept when the first is 0 and the second positive);
+ *   - a pass at mu0, accepted when |count| <= keep_tol (or lag != 0 and |count| < 0.25,
+ *     lcao3d.py:423-427);
+ *   - else the search of lres_mu_solve_warm (same code, run by one device thread
+ *     between passes) with slope, ftol, xtol;
+ *   - the density of the accepted mu (no spin factor) into d_dens, from the last
+ *     trial's D_B or from a pass of its own.
+ * `depth` (1..8) pass slots are queued, the keep pass first; those the search does
+ * not need leave at once, and lres_step_finish queues further batches (of at least 4)
+ * when it needs more.  Work queued after this call on the same stream reads d_dens as
+ * formed by the first batch (it holds the step's density unless a further batch ran,
+ * which h_out reports); until lres_step_finish returns, no other call may use the plan. */
+int lres_step_enqueue(lres_plan* plan, double N, double mu0, double e_lo, double margin, double keep_tol,
+                      double ftol, double xtol, double slope, int lag, int depth, double* d_dens);
+
+/* Waits for the queued step (queuing more trial batches when the search needs them)
+ * and returns h_out[10] = {status (0: done, d_dens holds the density; 1: the keep-edge
+ * test failed and nothing was computed; 2: no mu bracket within 60 secant steps),
+ * outcome (0 kept mu, 1 lagged mu, 2 re-solved), passes run, mu, count at mu,
+ * sum N_B and sum E_B at mu, slope after the search, 1 if the root came from Brent,
+ * batches}.  A search that does not converge in 200 Brent steps is an error. */
+int lres_step_finish(lres_plan* plan, double* d_dens, double* h_out);
+
 /* One-shot form with lrft_windows' argument list (src/lrft_cpp.cpp:129-132):
  * host CSR H, S, X, window tables, nb, dV, mu, e_lo, M; outputs h_dens (ng, may be
  * NULL), h_NB, h_EB (nwin).  Builds and destroys a plan on the null stream. */

@@ -58,6 +58,9 @@ SIGNATURES = {
                                 C.c_double, C.c_double, _vp, _f64p, _f64p]),
     "lres_mu_solve_warm": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                      C.c_double, _f64p, _f64p]),
+    "lres_step_enqueue": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                    C.c_double, C.c_double, C.c_int, C.c_int, _vp]),
+    "lres_step_finish": (C.c_int, [_vp, _vp, _f64p]),
     "lres_windows": (C.c_int, [_i64p, _i64p, _f64p, _i64p, _i64p, _f64p, _i64p, _i64p, _f64p,
                                _i64p, _i64p, C.c_int64, _i64p, _i64p, C.c_int64, C.c_int64, C.c_double,
                                C.c_double, C.c_double, C.c_int, _f64p, _f64p, _f64p]),

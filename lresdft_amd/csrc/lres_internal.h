@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "lres_musearch.h"
+
 namespace lres {
 
 // One window of a plan (all offsets in elements).
@@ -18,22 +20,39 @@ struct WinDesc {
 hipError_t launch_gather_csr(const lres::WinDesc* wins, int nwin, const int64_t* Wflat, const int64_t* Woff,
                              const int32_t* pos, const int64_t* Hp, const int64_t* Hi, const double* Hv,
                              double* out, hipStream_t st);
+// The pole kernels read the circle of their pass from device memory: circ = {c, R,
+// skip, skip_density}; skip != 0 makes every workgroup leave at once (a trial pass the
+// device-side mu search did not need), skip_density != 0 does the same for the density
+// kernel.  k_set_circle writes it for a pass decided on the host.
+hipError_t launch_set_circle(double* circ, double c, double R, double skip, double skip_density, hipStream_t st);
+// The device side of the queued warm density step (lres_step_enqueue): k_mu_begin
+// starts the search of lres_musearch.h (keep test at mu0, after the keep-edge test on
+// the inertia totals itot if given); k_sum_step sums a slot's N_B and E_B into out and
+// runs one controller stage (StepStage) on them; kStepPublish copies the state to `pin`
+// (mapped host memory); k_mu_step runs a stage without sums (kStepResume).
+hipError_t launch_mu_begin(lres::StepState* st, const unsigned long long* itot, double N, double e_lo, double mu0,
+                           double slope, double keep_tol, double ftol, double xtol, int lag, double* circ,
+                           hipStream_t stream);
+hipError_t launch_mu_step(lres::StepState* st, int stage, double* circ, void* pin, hipStream_t stream);
+hipError_t launch_sum_step(const double* a, const double* b, int n, double* out, lres::StepState* st, int stage,
+                           double* circ, void* pin, hipStream_t stream);
 hipError_t launch_pole_gj(int T, int G, int wide, int fp32, const lres::WinDesc* wins, const int32_t* ids, int nids,
-                          const double* SW, const double* HW, int M, int ngroup, double c, double Rr,
+                          const double* SW, const double* HW, int M, int ngroup, const double* circ,
                           double* Dpart, double* Fpart, hipStream_t st);
 // D, F, N_B, E_B from the pole kernels' partials: group sums (Dpart, Fpart) or,
 // for windows with by_pole[w] != 0, nslot per-pole float pairs in `slots`
 hipError_t launch_reduce(const lres::WinDesc* wins, int nwin, int ngroup, const double* Dpart, const double* Fpart,
                          const float* slots, int nslot, const int8_t* by_pole, const double* Q, double* D, double* F,
-                         double* NB, double* EB, hipStream_t st);
+                         double* NB, double* EB, const double* circ, hipStream_t st);
 // lower triangles of D_B (and F_B if F != NULL) from the upper ones the reduction writes
 hipError_t launch_mirror(const lres::WinDesc* wins, int nwin, double* D, double* F, hipStream_t st);
 // blocks6: every owned row of X is whole atom blocks (6 consecutive T-local columns
 // each, ascending), checked on the host; then one lane per point
+// skip (may be null): the density kernel leaves at once when *skip != 0
 hipError_t launch_density(const lres::WinDesc* wins, int nwin, int kmax, const double* D, const int64_t* own_g,
                           const int64_t* xptr, const int32_t* xidx, const double* xval, double* dens, int blocks6,
-                          hipStream_t st);
-hipError_t launch_sum2(const double* a, const double* b, int n, double* out, hipStream_t st);
+                          const double* skip, hipStream_t st);
+hipError_t launch_sum2(const double* a, const double* b, int n, double* out, const double* skip, hipStream_t st);
 hipError_t launch_voronoi_owner(const double* cells, int ncell, const double* box, const int64_t* shape, double dx,
                                 int32_t* owner, hipStream_t st);
 // SCF density mixing: grid and block counts of its fixed-order sum (kMixBlocks partials)
@@ -63,7 +82,7 @@ hipError_t launch_window_hv(int64_t total, const int32_t* hmap, const double* vp
 // Windows above the register-resident limit (blocked sweep in a global workspace).
 size_t big_lds_bytes(int nmax, int b, int elem_bytes);
 hipError_t launch_pole_big(int fp32, int bmax, int nmax, const lres::WinDesc* wins, const int32_t* ids, int nids,
-                           const double* SW, const double* HW, int M, int ngroup, double c, double Rr, void* work,
+                           const double* SW, const double* HW, int M, int ngroup, const double* circ, void* work,
                            const int64_t* ws_off, double* Dpart, double* Fpart, hipStream_t st);
 // Inertia counts of windows above the register-resident limit: work holds work_len
 // doubles, cut into slots of nmax^2; up to work_len / nmax^2 (window, shift) pairs are
@@ -77,8 +96,8 @@ hipError_t launch_inertia_big(int bmax, int nmax, const lres::WinDesc* wins, con
 // compiled in parallel.
 #define LRES_MFMA_PART_DECL(r, nt)                                                                              \
     hipError_t lres_mfma_part_##r##_##nt(int nwc, const lres::WinDesc* wins, const int32_t* ids, int nids,     \
-                                         const double* SW, const double* HW, int M, int ngroup, double c,     \
-                                         double Rr, double* Dpart, double* Fpart, int acc, hipStream_t st);
+                                         const double* SW, const double* HW, int M, int ngroup,               \
+                                         const double* circ, double* Dpart, double* Fpart, int acc, hipStream_t st);
 #define LRES_MFMA_PARTS(X) X(float, 2) X(float, 3) X(float, 4) X(float, 5) X(float, 6) X(float, 7) X(float, 8) \
     X(float, 9) X(float, 10) X(float, 11) X(float, 12) X(float, 13) X(float, 14) X(float, 15) X(float, 16)     \
     X(double, 2) X(double, 3) X(double, 4) X(double, 5) X(double, 6) X(double, 7) X(double, 8)
@@ -96,7 +115,7 @@ hipError_t launch_inertia_mfma(int nt, const lres::WinDesc* wins, const int32_t*
 // acc > 0 (fp64): the group partials are accumulated over the group's poles in LDS, as
 // packed upper triangles of acc entries (at least k(k+1)/2 of every window launched)
 hipError_t launch_pole_mfma(int fp32, int nt, int nwc, const lres::WinDesc* wins, const int32_t* ids, int nids,
-                            const double* SW, const double* HW, int M, int ngroup, double c, double Rr,
+                            const double* SW, const double* HW, int M, int ngroup, const double* circ,
                             double* Dpart, double* Fpart, int acc, hipStream_t st);
 // accumulator registers per lane of one wave of the MFMA pole kernel
 int mfma_acc_regs(int fp32, int nt, int nwc);

@@ -127,10 +127,12 @@ __device__ __forceinline__ void publish_pivot(const cx<R> (&A)[T][T], int PQ, in
 template <int T, int MAXT, typename R>
 __global__ void __launch_bounds__(MAXT) k_pole_gj(const WinDesc* __restrict__ wins, const int32_t* __restrict__ ids,
                                                   const double* __restrict__ SW, const double* __restrict__ HW,
-                                                  int M, int ngroup, int G, double c, double Rr,
+                                                  int M, int ngroup, int G, const double* __restrict__ circ,
                                                   double* __restrict__ Dpart, double* __restrict__ Fpart) {
     __shared__ cx<R> rowbuf[2][kMaxG * T];
     __shared__ cx<R> invbuf[2];
+    if (circ[2] != 0.0) return;   // a trial pass the device-side mu search did not need
+    const double c = circ[0], Rr = circ[1];
     const int w = ids[blockIdx.x];
     const int g = blockIdx.y;
     const WinDesc d = wins[w];
@@ -288,8 +290,9 @@ __global__ void __launch_bounds__(1024) k_reduce(const WinDesc* __restrict__ win
                                                 const int8_t* __restrict__ by_pole,
                                                 const double* __restrict__ Q, double* __restrict__ D,
                                                 double* __restrict__ F, double* __restrict__ NB,
-                                                double* __restrict__ EB) {
+                                                double* __restrict__ EB, const double* __restrict__ circ) {
     __shared__ double red[16];
+    if (circ != nullptr && circ[2] != 0.0) return;   // the pass was skipped
     const int w = blockIdx.x;
     const WinDesc d = wins[w];
     const int k = d.k;
@@ -370,7 +373,8 @@ __global__ void __launch_bounds__(1024) k_reduce(const WinDesc* __restrict__ win
 
 // Lower triangle of D_B (and F_B if given) from the upper one, per window.
 __global__ void __launch_bounds__(256) k_mirror(const WinDesc* __restrict__ wins, double* __restrict__ D,
-                                                double* __restrict__ F) {
+                                                double* __restrict__ F, const double* __restrict__ skip) {
+    if (skip != nullptr && *skip != 0.0) return;
     const WinDesc d = wins[blockIdx.x];
     const int k = d.k;
     double* Dw = D + d.kk_off;
@@ -419,8 +423,10 @@ __global__ void __launch_bounds__(64 * kDensWaves) k_density(const WinDesc* __re
                                                              const int64_t* __restrict__ xptr,
                                                              const int32_t* __restrict__ xidx,
                                                              const double* __restrict__ xval,
-                                                             double* __restrict__ dens) {
+                                                             double* __restrict__ dens,
+                                                             const double* __restrict__ skip) {
     extern __shared__ double Dl[];
+    if (skip != nullptr && *skip != 0.0) return;
     const int w = blockIdx.x;
     const WinDesc d = wins[w];
     const int k = d.k;
@@ -525,8 +531,10 @@ __global__ void __launch_bounds__(kDensBlkThreads) k_density_blk(const WinDesc* 
                                                                  const int64_t* __restrict__ xptr,
                                                                  const int32_t* __restrict__ xidx,
                                                                  const double* __restrict__ xval,
-                                                                 double* __restrict__ dens) {
+                                                                 double* __restrict__ dens,
+                                                                 const double* __restrict__ skip) {
     extern __shared__ double Dl[];
+    if (skip != nullptr && *skip != 0.0) return;
     const int w = blockIdx.x;
     const WinDesc d = wins[w];
     const int k = d.k;
@@ -760,8 +768,10 @@ __global__ void __launch_bounds__(kMixThreads) k_rows_total(const double* __rest
 }
 
 // sum of nwin values in a fixed order into out[0] (one block)
-__global__ void k_sum2(const double* __restrict__ a, const double* __restrict__ b, int n, double* __restrict__ out) {
+__global__ void k_sum2(const double* __restrict__ a, const double* __restrict__ b, int n, double* __restrict__ out,
+                       const double* __restrict__ skip) {
     __shared__ double red[16];
+    if (skip != nullptr && *skip != 0.0) return;
     double sa = 0.0, sb = 0.0;
     for (int q = threadIdx.x; q < n; q += blockDim.x) {
         sa += a[q];
@@ -773,6 +783,132 @@ __global__ void k_sum2(const double* __restrict__ a, const double* __restrict__ 
         out[0] = ta;
         out[1] = tb;
         __threadfence_system();   // out may be mapped host memory
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The circle of a pass in device memory and the warm mu search between the
+// queued passes of a device step: one thread runs the search of lres_musearch.h
+// (the host loop's code) on the sums the pass left in device memory and writes the
+// next pass's circle, or marks the remaining queued passes as skipped.
+// ---------------------------------------------------------------------------
+__global__ void k_set_circle(double* __restrict__ circ, double c, double R, double skip, double skip_density) {
+    circ[0] = c;
+    circ[1] = R;
+    circ[2] = skip;
+    circ[3] = skip_density;
+}
+
+// a trial pass at mu (the same c and R as the host's run_pass); trials form no density
+__device__ void circle_at(double* circ, double mu, double e_lo) {
+    circ[0] = 0.5 * (mu + e_lo);
+    circ[1] = 0.5 * (mu - e_lo);
+    circ[2] = 0.0;
+    circ[3] = 1.0;
+}
+
+__device__ void circle_off(double* circ) {
+    circ[2] = 1.0;
+    circ[3] = 1.0;
+}
+
+// itot (may be null): inertia totals at (e_lo + margin / 3, e_lo + 2 margin); the
+// filter edge is kept only when the first is zero and the second positive
+// (lcao3d.py:420), else the step stops with kWarmEdge before any pass
+__global__ void k_mu_begin(StepState* __restrict__ st, const unsigned long long* __restrict__ itot, double N,
+                           double e_lo, double mu0, double slope, double keep_tol, double ftol, double xtol, int lag,
+                           double* __restrict__ circ) {
+    WarmSearch& s = st->s;
+    warm_begin_keep(s, N, e_lo, mu0, slope, keep_tol, ftol, xtol, lag);
+    st->final_pass = 0;
+    st->final_ran = 0;
+    if (itot != nullptr && !(itot[0] == 0 && itot[1] > 0)) s.phase = kWarmEdge;
+    if (warm_terminal(s))
+        circle_off(circ);
+    else
+        circle_at(circ, s.x, s.e_lo);
+}
+
+__device__ void step_final(StepState* st, double* circ) {
+    WarmSearch& s = st->s;
+    if (s.phase == kWarmDone) {
+        if (s.mu == s.last_mu) {
+            circ[2] = 1.0;     // the density from the last trial's D_B
+        } else {
+            circle_at(circ, s.mu, s.e_lo);
+            st->final_pass = 1;
+        }
+        circ[3] = 0.0;
+    } else {
+        circle_off(circ);      // still searching (resumed by the next batch) or failed
+    }
+}
+
+// one controller stage on the sums (sn, se) of the slot's pass (valid when circ[2] was 0
+// before it ran: `ran`)
+__device__ void step_stage(StepState* st, int stage, bool ran, double sn, double se, double* circ,
+                           unsigned long long* pin) {
+    WarmSearch& s = st->s;
+    if (stage == kStepFeed || stage == kStepFeedFinal) {
+        if (ran) {
+            warm_feed(s, sn, se);
+            if (warm_terminal(s))
+                circle_off(circ);
+            else
+                circle_at(circ, s.x, s.e_lo);
+        }
+        if (stage == kStepFeedFinal) step_final(st, circ);
+    } else if (stage == kStepPublish) {
+        if (st->final_pass && ran) {
+            s.last_mu = s.mu;
+            s.last_nsum = sn;
+            s.last_esum = se;
+            s.last_f = 2.0 * sn - s.N;
+            st->final_pass = 0;
+            st->final_ran += 1;
+        }
+        // the state to mapped host memory, read after the stream's wait
+        const unsigned long long* src = reinterpret_cast<const unsigned long long*>(st);
+        for (int i = 0; i < (int)(sizeof(StepState) / 8); ++i) pin[i] = src[i];
+        __threadfence_system();
+    } else if (stage == kStepResume) {
+        if (warm_terminal(s))
+            circle_off(circ);
+        else
+            circle_at(circ, s.x, s.e_lo);
+    }
+}
+
+__global__ void k_mu_step(StepState* __restrict__ st, int stage, double* __restrict__ circ,
+                          unsigned long long* __restrict__ pin) {
+    step_stage(st, stage, false, 0.0, 0.0, circ, pin);
+}
+
+// The sums of N_B and E_B of a device step's slot (the same fixed-order sum as k_sum2)
+// followed by the controller stage in thread 0; every thread reads circ[2] before
+// thread 0 rewrites it.
+__global__ void k_sum_step(const double* __restrict__ a, const double* __restrict__ b, int n,
+                           double* __restrict__ out, StepState* __restrict__ st, int stage, double* __restrict__ circ,
+                           unsigned long long* __restrict__ pin) {
+    __shared__ double red[16];
+    const bool ran = circ[2] == 0.0;
+    double ta = 0.0, tb = 0.0;
+    if (ran) {
+        double sa = 0.0, sb = 0.0;
+        for (int q = threadIdx.x; q < n; q += blockDim.x) {
+            sa += a[q];
+            sb += b[q];
+        }
+        ta = block_sum(sa, red);
+        tb = block_sum(sb, red);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (ran) {
+            out[0] = ta;
+            out[1] = tb;
+        }
+        step_stage(st, stage, ran, ta, tb, circ, pin);
     }
 }
 
@@ -893,7 +1029,7 @@ hipError_t launch_gather_csr(const WinDesc* wins, int nwin, const int64_t* Wflat
 // 512 (fewer, fatter waves).  The autotuner times both.
 template <typename R>
 static hipError_t launch_pole_t(int T, int G, int wide, const WinDesc* wins, const int32_t* ids, int nids,
-                                const double* SW, const double* HW, int M, int ngroup, double c, double Rr,
+                                const double* SW, const double* HW, int M, int ngroup, const double* circ,
                                 double* Dpart, double* Fpart, hipStream_t st) {
     if (G < 1 || G > kMaxG) return hipErrorInvalidValue;
     if (wide && G * G > 256) return hipErrorInvalidValue;
@@ -901,10 +1037,10 @@ static hipError_t launch_pole_t(int T, int G, int wide, const WinDesc* wins, con
 #define LRES_POLE_CASE(TT)                                                                                    \
     case TT:                                                                                                  \
         if (wide)                                                                                             \
-            hipLaunchKernelGGL((k_pole_gj<TT, 256, R>), grid, blk, 0, st, wins, ids, SW, HW, M, ngroup, G, c, Rr, \
+            hipLaunchKernelGGL((k_pole_gj<TT, 256, R>), grid, blk, 0, st, wins, ids, SW, HW, M, ngroup, G, circ,  \
                                Dpart, Fpart);                                                                 \
         else                                                                                                  \
-            hipLaunchKernelGGL((k_pole_gj<TT, 1024, R>), grid, blk, 0, st, wins, ids, SW, HW, M, ngroup, G, c, Rr, \
+            hipLaunchKernelGGL((k_pole_gj<TT, 1024, R>), grid, blk, 0, st, wins, ids, SW, HW, M, ngroup, G, circ, \
                                Dpart, Fpart);                                                                 \
         break;
     switch (T) {
@@ -920,10 +1056,10 @@ static hipError_t launch_pole_t(int T, int G, int wide, const WinDesc* wins, con
 }
 
 hipError_t launch_pole_gj(int T, int G, int wide, int fp32, const WinDesc* wins, const int32_t* ids, int nids,
-                          const double* SW, const double* HW, int M, int ngroup, double c, double Rr,
+                          const double* SW, const double* HW, int M, int ngroup, const double* circ,
                           double* Dpart, double* Fpart, hipStream_t st) {
-    if (fp32) return launch_pole_t<float>(T, G, wide, wins, ids, nids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart, st);
-    return launch_pole_t<double>(T, G, wide, wins, ids, nids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart, st);
+    if (fp32) return launch_pole_t<float>(T, G, wide, wins, ids, nids, SW, HW, M, ngroup, circ, Dpart, Fpart, st);
+    return launch_pole_t<double>(T, G, wide, wins, ids, nids, SW, HW, M, ngroup, circ, Dpart, Fpart, st);
 }
 
 int mfma_acc_regs(int fp32, int nt, int nwc) {
@@ -940,12 +1076,12 @@ static bool mfma_nwc_ok(int nt, int nwc) {
 }
 
 hipError_t launch_pole_mfma(int fp32, int nt, int nwc, const WinDesc* wins, const int32_t* ids, int nids,
-                            const double* SW, const double* HW, int M, int ngroup, double c, double Rr,
+                            const double* SW, const double* HW, int M, int ngroup, const double* circ,
                             double* Dpart, double* Fpart, int acc, hipStream_t st) {
     if (!mfma_nwc_ok(nt, nwc)) return hipErrorInvalidValue;
 #define LRES_MFMA_CALL(r, NT)                                                                                  \
     if (nt == NT && (fp32 != 0) == (sizeof(r) == 4))                                                           \
-        return lres_mfma_part_##r##_##NT(nwc, wins, ids, nids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart, acc, st);
+        return lres_mfma_part_##r##_##NT(nwc, wins, ids, nids, SW, HW, M, ngroup, circ, Dpart, Fpart, acc, st);
     LRES_MFMA_PARTS(LRES_MFMA_CALL)
 #undef LRES_MFMA_CALL
     return hipErrorInvalidValue;
@@ -965,7 +1101,7 @@ int mfma_scratch_bytes(int fp32, int nt, int nwc) {
     if (!mfma_nwc_ok(nt, nwc)) return -1;
 #define LRES_MFMA_CALL(r, NT)                                                                                  \
     if (nt == NT && (fp32 != 0) == (sizeof(r) == 4))                                                           \
-        return (int)lres_mfma_part_##r##_##NT(-nwc, nullptr, nullptr, 0, nullptr, nullptr, 0, 0, 0.0, 0.0,    \
+        return (int)lres_mfma_part_##r##_##NT(-nwc, nullptr, nullptr, 0, nullptr, nullptr, 0, 0, nullptr,      \
                                               nullptr, nullptr, 0, nullptr);
     LRES_MFMA_PARTS(LRES_MFMA_CALL)
 #undef LRES_MFMA_CALL
@@ -974,41 +1110,41 @@ int mfma_scratch_bytes(int fp32, int nt, int nwc) {
 
 hipError_t launch_reduce(const WinDesc* wins, int nwin, int ngroup, const double* Dpart, const double* Fpart,
                          const float* slots, int nslot, const int8_t* by_pole, const double* Q, double* D, double* F,
-                         double* NB, double* EB, hipStream_t st) {
+                         double* NB, double* EB, const double* circ, hipStream_t st) {
     // 16 waves per window: enough loads in flight for the per-pole partials (16 float
     // pairs per entry)
     hipLaunchKernelGGL(k_reduce, dim3(nwin), dim3(1024), 0, st, wins, ngroup, Dpart, Fpart,
-                       reinterpret_cast<const float2*>(slots), nslot, by_pole, Q, D, F, NB, EB);
+                       reinterpret_cast<const float2*>(slots), nslot, by_pole, Q, D, F, NB, EB, circ);
     return hipGetLastError();
 }
 
 hipError_t launch_mirror(const WinDesc* wins, int nwin, double* D, double* F, hipStream_t st) {
-    hipLaunchKernelGGL(k_mirror, dim3(nwin), dim3(256), 0, st, wins, D, F);
+    hipLaunchKernelGGL(k_mirror, dim3(nwin), dim3(256), 0, st, wins, D, F, nullptr);
     return hipGetLastError();
 }
 
 hipError_t launch_density(const WinDesc* wins, int nwin, int kmax, const double* D, const int64_t* own_g,
                           const int64_t* xptr, const int32_t* xidx, const double* xval, double* dens, int blocks6,
-                          hipStream_t st) {
+                          const double* skip, hipStream_t st) {
     const size_t lds = (size_t)kmax * kmax * sizeof(double);
     // D_B in LDS within the 64 KB default of dynamic LDS
     const int stage = lds <= 56 * 1024 ? 1 : 0;
     if (blocks6) {
         if (stage)
             hipLaunchKernelGGL(k_density_blk<true>, dim3(nwin), dim3(kDensBlkThreads), lds, st, wins, D, own_g, xptr,
-                               xidx, xval, dens);
+                               xidx, xval, dens, skip);
         else
             hipLaunchKernelGGL(k_density_blk<false>, dim3(nwin), dim3(kDensBlkThreads), 0, st, wins, D, own_g, xptr,
-                               xidx, xval, dens);
+                               xidx, xval, dens, skip);
         return hipGetLastError();
     }
-    hipLaunchKernelGGL(k_mirror, dim3(nwin), dim3(256), 0, st, wins, const_cast<double*>(D), nullptr);
+    hipLaunchKernelGGL(k_mirror, dim3(nwin), dim3(256), 0, st, wins, const_cast<double*>(D), nullptr, skip);
     if (stage)
         hipLaunchKernelGGL(k_density<true>, dim3(nwin), dim3(64 * kDensWaves), lds, st, wins, D, own_g, xptr, xidx,
-                           xval, dens);
+                           xval, dens, skip);
     else
         hipLaunchKernelGGL(k_density<false>, dim3(nwin), dim3(64 * kDensWaves), 0, st, wins, D, own_g, xptr, xidx,
-                           xval, dens);
+                           xval, dens, skip);
     return hipGetLastError();
 }
 
@@ -1081,8 +1217,34 @@ hipError_t launch_diag_ratio(const WinDesc* wins, int nwin, const double* SW, co
     return hipGetLastError();
 }
 
-hipError_t launch_sum2(const double* a, const double* b, int n, double* out, hipStream_t st) {
-    hipLaunchKernelGGL(k_sum2, dim3(1), dim3(256), 0, st, a, b, n, out);
+hipError_t launch_set_circle(double* circ, double c, double R, double skip, double skip_density, hipStream_t st) {
+    hipLaunchKernelGGL(k_set_circle, dim3(1), dim3(1), 0, st, circ, c, R, skip, skip_density);
+    return hipGetLastError();
+}
+
+hipError_t launch_mu_begin(StepState* st, const unsigned long long* itot, double N, double e_lo, double mu0,
+                           double slope, double keep_tol, double ftol, double xtol, int lag, double* circ,
+                           hipStream_t stream) {
+    hipLaunchKernelGGL(k_mu_begin, dim3(1), dim3(1), 0, stream, st, itot, N, e_lo, mu0, slope, keep_tol, ftol, xtol,
+                       lag, circ);
+    return hipGetLastError();
+}
+
+hipError_t launch_mu_step(StepState* st, int stage, double* circ, void* pin, hipStream_t stream) {
+    hipLaunchKernelGGL(k_mu_step, dim3(1), dim3(1), 0, stream, st, stage, circ,
+                       reinterpret_cast<unsigned long long*>(pin));
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_step(const double* a, const double* b, int n, double* out, StepState* st, int stage,
+                           double* circ, void* pin, hipStream_t stream) {
+    hipLaunchKernelGGL(k_sum_step, dim3(1), dim3(256), 0, stream, a, b, n, out, st, stage, circ,
+                       reinterpret_cast<unsigned long long*>(pin));
+    return hipGetLastError();
+}
+
+hipError_t launch_sum2(const double* a, const double* b, int n, double* out, const double* skip, hipStream_t st) {
+    hipLaunchKernelGGL(k_sum2, dim3(1), dim3(256), 0, st, a, b, n, out, skip);
     return hipGetLastError();
 }
 
@@ -1603,11 +1765,13 @@ constexpr int kBigThreads = 1024;
 template <typename R>
 __global__ void __launch_bounds__(kBigThreads) k_pole_big(const WinDesc* __restrict__ wins, const int32_t* __restrict__ ids,
                                                   int nids, const double* __restrict__ SW, const double* __restrict__ HW,
-                                                  int M, int ngroup, double c, double Rr, int bmax,
+                                                  int M, int ngroup, const double* __restrict__ circ, int bmax,
                                                   cx<double>* __restrict__ work, const int64_t* __restrict__ ws_off,
                                                   double* __restrict__ Dpart, double* __restrict__ Fpart) {
     extern __shared__ double lds_raw[];
     using E = cx<R>;
+    if (circ[2] != 0.0) return;   // a trial pass the device-side mu search did not need
+    const double c = circ[0], Rr = circ[1];
     // XCD-aware order (workgroups go to the 8 XCDs round-robin by id): every pole group
     // of a window on one XCD, dispatched together, so S_W and H_W come from its L2
     const int L = blockIdx.x;
@@ -1708,7 +1872,7 @@ size_t big_lds_bytes(int nmax, int b, int elem_bytes) {
 }
 
 hipError_t launch_pole_big(int fp32, int bmax, int nmax, const WinDesc* wins, const int32_t* ids, int nids,
-                           const double* SW, const double* HW, int M, int ngroup, double c, double Rr, void* work,
+                           const double* SW, const double* HW, int M, int ngroup, const double* circ, void* work,
                            const int64_t* ws_off, double* Dpart, double* Fpart, hipStream_t st) {
     const size_t lds = big_lds_bytes(nmax, bmax, fp32 ? 8 : 16);
     dim3 grid((nids + 7) / 8 * 8 * ngroup), blk(kBigThreads);   // ids past nids exit at once
@@ -1716,12 +1880,12 @@ hipError_t launch_pole_big(int fp32, int bmax, int nmax, const WinDesc* wins, co
     if (fp32) {
         e = hipFuncSetAttribute((const void*)k_pole_big<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_pole_big<float>, grid, blk, lds, st, wins, ids, nids, SW, HW, M, ngroup, c, Rr, bmax,
+        hipLaunchKernelGGL(k_pole_big<float>, grid, blk, lds, st, wins, ids, nids, SW, HW, M, ngroup, circ, bmax,
                            (cx<double>*)work, ws_off, Dpart, Fpart);
     } else {
         e = hipFuncSetAttribute((const void*)k_pole_big<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_pole_big<double>, grid, blk, lds, st, wins, ids, nids, SW, HW, M, ngroup, c, Rr, bmax,
+        hipLaunchKernelGGL(k_pole_big<double>, grid, blk, lds, st, wins, ids, nids, SW, HW, M, ngroup, circ, bmax,
                            (cx<double>*)work, ws_off, Dpart, Fpart);
     }
     return hipGetLastError();

@@ -142,7 +142,7 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
                                                         const int32_t* __restrict__ ids,
                                                         const double* __restrict__ SW,
                                                         const double* __restrict__ HW, int nids, int M, int ngroup,
-                                                        double c, double Rr, double* __restrict__ Dpart,
+                                                        const double* __restrict__ circ, double* __restrict__ Dpart,
                                                         double* __restrict__ Fpart, int acc) {
     using MF = Mfma<R>;
     using V = typename MF::V;
@@ -161,6 +161,10 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
     // triangles of acc entries each (dynamic LDS), stored to Dpart / Fpart once at the
     // end instead of a read-modify-write of global memory per pole
     extern __shared__ double pacc[];
+    // the circle (c, R) of the pass; circ[2] != 0 marks a pass the device-side mu search
+    // did not need (every workgroup leaves at once)
+    if (circ[2] != 0.0) return;
+    const double c = circ[0], Rr = circ[1];
     // XCD-aware order: workgroups are dealt to the 8 XCDs round-robin by linear id, so
     // id L -> window slot 8 (L / (8 ngroup)) + L % 8, group (L / 8) % ngroup puts every
     // group of a window on one XCD, dispatched together: the window's S_W and H_W are
@@ -606,7 +610,7 @@ constexpr int mfma_waves() {
 
 template <typename R, int NT, int NWC>
 hipError_t launch_one(const WinDesc* wins, const int32_t* ids, int nids, const double* SW, const double* HW, int M,
-                      int ngroup, double c, double Rr, double* Dpart, double* Fpart, int acc, hipStream_t st) {
+                      int ngroup, const double* circ, double* Dpart, double* Fpart, int acc, hipStream_t st) {
     const auto kern = k_pole_mfma<R, NT, NWC, mfma_waves<R, NT, NWC>()>;
     const size_t lds = sizeof(R) == 8 && acc > 0 ? (size_t)2 * acc * sizeof(double) : 0;
     if (lds > 0) {
@@ -619,7 +623,7 @@ hipError_t launch_one(const WinDesc* wins, const int32_t* ids, int nids, const d
         }
     }
     const int nblk = (nids + 7) / 8 * 8 * ngroup;   // one id per (window slot, group); slots past nids exit
-    hipLaunchKernelGGL(kern, dim3(nblk), dim3(64 * NWC), lds, st, wins, ids, SW, HW, nids, M, ngroup, c, Rr, Dpart,
+    hipLaunchKernelGGL(kern, dim3(nblk), dim3(64 * NWC), lds, st, wins, ids, SW, HW, nids, M, ngroup, circ, Dpart,
                        Fpart, lds > 0 ? acc : 0);
     return hipGetLastError();
 }
@@ -653,14 +657,15 @@ int scratch_one() {
 // nwc > 0: launch with nwc waves per (window, pole group); nwc < 0: return the
 // scratch bytes per lane of the -nwc variant (-1 if it does not exist)
 hipError_t LRES_MFMA_NAME(LRES_MFMA_R, LRES_MFMA_NT)(int nwc, const lres::WinDesc* wins, const int32_t* ids, int nids,
-                                                     const double* SW, const double* HW, int M, int ngroup, double c,
-                                                     double Rr, double* Dpart, double* Fpart, int acc, hipStream_t st) {
+                                                     const double* SW, const double* HW, int M, int ngroup,
+                                                     const double* circ, double* Dpart, double* Fpart, int acc,
+                                                     hipStream_t st) {
     using R = LRES_MFMA_R;
     constexpr int NT = LRES_MFMA_NT;
 #define LRES_MFMA_CASE(W)                                                                                     \
     if constexpr (W <= NT) {                                                                                  \
         if (nwc == W)                                                                                         \
-            return lres::launch_one<R, NT, W>(wins, ids, nids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart, acc, st); \
+            return lres::launch_one<R, NT, W>(wins, ids, nids, SW, HW, M, ngroup, circ, Dpart, Fpart, acc, st); \
         if (nwc == -W) return (hipError_t)lres::scratch_one<R, NT, W>();                                      \
     }
     LRES_MFMA_FOR_NWC(LRES_MFMA_CASE)

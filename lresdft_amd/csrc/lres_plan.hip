@@ -129,11 +129,11 @@ Shape default_pole_shape(int nmax, int fp32) {
 }
 
 hipError_t launch_pole(const Shape& sh, int fp32, const WinDesc* wins, const int32_t* ids, int nids, const double* SW,
-                       const double* HW, int M, int ngroup, double c, double Rr, double* Dpart, double* Fpart,
+                       const double* HW, int M, int ngroup, const double* circ, double* Dpart, double* Fpart,
                        hipStream_t st, int acc = 0) {
     if (sh.wide == kMfma)
-        return launch_pole_mfma(fp32, sh.T, sh.G, wins, ids, nids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart, acc, st);
-    return launch_pole_gj(sh.T, sh.G, sh.wide, fp32, wins, ids, nids, SW, HW, M, ngroup, c, Rr, Dpart, Fpart, st);
+        return launch_pole_mfma(fp32, sh.T, sh.G, wins, ids, nids, SW, HW, M, ngroup, circ, Dpart, Fpart, acc, st);
+    return launch_pole_gj(sh.T, sh.G, sh.wide, fp32, wins, ids, nids, SW, HW, M, ngroup, circ, Dpart, Fpart, st);
 }
 
 Shape inertia_shape(int nmax) {
@@ -231,12 +231,27 @@ struct lres_plan {
     // their shifts from the block at kPinSlots; every call that uses it waits for its
     // stream before returning
     static constexpr int kPinSlots = 2 + 1024;
+    // after the two blocks above: the device step's state (StepState), published there
+    // by its last controller stage
+    static constexpr int kStepPin = 64;
     double* h_pin = nullptr;
     double* d_pin = nullptr;
+    // the circle of the pass in flight in device memory: {c, R, skip, skip_density}
+    // (written by k_set_circle for host-decided passes, by the mu controller in a
+    // device step)
+    DevBuf<double> d_circ;
+    // queued warm density step (lres_step_enqueue / lres_step_finish): its state, the
+    // trial slots per batch and a pair of timing events per slot
+    DevBuf<lres::StepState> d_step;
+    bool step_queued = false;
+    int step_depth = 0;
+    std::vector<hipEvent_t> ev_slot;
     ~lres_plan() {
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         for (auto& e : ev_join)
+            if (e) (void)hipEventDestroy(e);
+        for (auto& e : ev_slot)
             if (e) (void)hipEventDestroy(e);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         for (auto& q : side)
@@ -345,10 +360,11 @@ void tune_bucket(lres_plan* P, lres_plan::Bucket& b, int fp32, double c, double 
     hip_check(hipEventCreate(&e0), "event");
     hip_check(hipEventCreate(&e1), "event");
     float best = 1e30f;
+    hip_check(launch_set_circle(P->d_circ.p, c, R, 0.0, 1.0, P->st), "circle");
     for (const auto& sh : cands) {
         auto launch = [&] {
             hip_check(launch_pole(sh, fp32, P->d_wins.p, P->d_bucket_ids.p + b.id_off, (int)b.ids.size(), P->d_SW.p,
-                                  P->d_HW.p, P->M, P->ngroup, c, R, pole_partials(P, sh, fp32), P->d_Fpart.p,
+                                  P->d_HW.p, P->M, P->ngroup, P->d_circ.p, pole_partials(P, sh, fp32), P->d_Fpart.p,
                                   P->st, pole_acc_entries(P, b, sh, fp32)),
                       "pole kernel (tuning)");
         };
@@ -407,21 +423,43 @@ BigSpan big_span(const lres_plan* P, int fp32) {
     return sp;
 }
 
-// pole solves + reduction (+ density) at (mu, e_lo); sums to host if requested
-void run_pass(lres_plan* P, double mu, double e_lo, double* d_dens, double* d_NB, double* d_EB, double* h_sums) {
-    require(P->have_H, "no Hamiltonian loaded: call lres_set_hamiltonian_csr first");
-    require(mu > e_lo, "mu must lie above e_lo");
-    const double c = 0.5 * (mu + e_lo), R = 0.5 * (mu - e_lo);
-    hip_check(hipEventRecord(P->ev[2], P->st), "event");
-    const int fp32 = P->precision == LRES_MIXED ? 1 : 0;
+// Before the passes of one precision: every register-kernel bucket tuned (the
+// autotuner times its variants at the circle (c, R) on the plan stream) and the
+// reduction's table of windows whose partials are per-pole slots (complex64
+// matrix-core buckets) on the device; both change only on a bucket's first pass.
+void prepare_passes(lres_plan* P, int fp32, double c, double R) {
     std::vector<int8_t> by_pole(P->nwin, 0);
+    for (auto& b : P->buckets) {
+        if (b.big && !big_on_mfma(b.nmax, fp32)) continue;
+        if (!b.tuned[fp32]) tune_bucket(P, b, fp32, c, R);
+        if (fp32 && b.shape[fp32].wide == kMfma)
+            for (int32_t w : b.ids) by_pole[w] = 1;
+    }
+    // complex64 passes only (fp64 partials are always group sums)
+    if (fp32 && by_pole != P->by_pole) {
+        P->d_by_pole.upload(by_pole.data(), by_pole.size(), P->st);
+        hip_check(hipStreamSynchronize(P->st), "sync");
+        P->by_pole = by_pole;
+    }
+}
+
+// Queue one pass at the circle in P->d_circ: pole solves, reduction, the density into
+// d_dens if given (skipped on the device when d_circ[3] != 0) and the sums of N_B and
+// E_B into `sums` (device or mapped host memory).  Every kernel leaves at once when
+// d_circ[2] != 0.  The buckets must be prepared (prepare_passes).  e0 / e1 (optional):
+// events recorded on the plan stream around the pole kernels.  step_stage >= 0: a slot
+// of a device step, whose sum kernel then runs that controller stage (k_sum_step).
+void queue_pass(lres_plan* P, double* d_dens, double* d_NB, double* d_EB, double* sums, hipEvent_t e0 = nullptr,
+                hipEvent_t e1 = nullptr, int step_stage = -1) {
+    const int fp32 = P->precision == LRES_MIXED ? 1 : 0;
+    const double* circ = P->d_circ.p;
     // the buckets on the blocked sweep (size-generic: every window's n comes from its
     // descriptor) go in one launch when they are the tail of the size-ordered id list, so
     // that a spread of window sizes still fills the chip
     const BigSpan big = big_span(P, fp32);
-    // the register-kernel buckets, tuned first (the autotuner times on the plan stream);
-    // with more than one launch in the pass, the buckets run concurrently on the side
-    // streams (each writes its own windows' partial sums) beside the blocked launch
+    // the register-kernel buckets; with more than one launch in the pass, they run
+    // concurrently on the side streams (each writes its own windows' partial sums)
+    // beside the blocked launch
     std::vector<lres_plan::Bucket*> reg;
     int nlaunch = big.first >= 0 ? 1 : 0;
     for (auto& b : P->buckets) {
@@ -429,22 +467,23 @@ void run_pass(lres_plan* P, double mu, double e_lo, double* d_dens, double* d_NB
             if (big.first < 0) ++nlaunch;
             continue;
         }
-        if (!b.tuned[fp32]) tune_bucket(P, b, fp32, c, R);
+        require(b.tuned[fp32], "pass queued before its buckets were tuned");
         reg.push_back(&b);
         ++nlaunch;
     }
     const bool spread = nlaunch > 1 && !reg.empty();
+    if (e0) hip_check(hipEventRecord(e0, P->st), "event");
     if (spread) side_fork(P);
     if (big.first >= 0)
         hip_check(launch_pole_big(fp32, big.bmax, big.nmax, P->d_wins.p, P->d_bucket_ids.p + big.id_off, big.nids,
-                                  P->d_SW.p, P->d_HW.p, P->M, P->ngroup, c, R, P->d_work.p, P->d_ws_off.p,
+                                  P->d_SW.p, P->d_HW.p, P->M, P->ngroup, circ, P->d_work.p, P->d_ws_off.p,
                                   P->d_Dpart.p, P->d_Fpart.p, P->st),
                   "blocked pole kernel");
     else
         for (auto& b : P->buckets)
             if (b.big && !big_on_mfma(b.nmax, fp32))
                 hip_check(launch_pole_big(fp32, b.bmax, b.nmax, P->d_wins.p, P->d_bucket_ids.p + b.id_off,
-                                          (int)b.ids.size(), P->d_SW.p, P->d_HW.p, P->M, P->ngroup, c, R,
+                                          (int)b.ids.size(), P->d_SW.p, P->d_HW.p, P->M, P->ngroup, circ,
                                           P->d_work.p, P->d_ws_off.p, P->d_Dpart.p, P->d_Fpart.p, P->st),
                           "blocked pole kernel");
     int used = 0;
@@ -452,32 +491,40 @@ void run_pass(lres_plan* P, double mu, double e_lo, double* d_dens, double* d_NB
         auto& b = *bp;
         const hipStream_t sq = spread ? side_stream(P, used++) : P->st;
         hip_check(launch_pole(b.shape[fp32], fp32, P->d_wins.p, P->d_bucket_ids.p + b.id_off, (int)b.ids.size(),
-                              P->d_SW.p, P->d_HW.p, P->M, P->ngroup, c, R, pole_partials(P, b.shape[fp32], fp32),
+                              P->d_SW.p, P->d_HW.p, P->M, P->ngroup, circ, pole_partials(P, b.shape[fp32], fp32),
                               P->d_Fpart.p, sq, pole_acc_entries(P, b, b.shape[fp32], fp32)),
                   "pole kernel");
-        if (fp32 && b.shape[fp32].wide == kMfma)
-            for (int32_t w : b.ids) by_pole[w] = 1;
     }
     if (spread) side_join(P, used);
-    // complex64 passes only (fp64 partials are always group sums); changes only
-    // when a bucket's kernel family does
-    if (fp32 && by_pole != P->by_pole) {
-        P->d_by_pole.upload(by_pole.data(), by_pole.size(), P->st);
-        hip_check(hipStreamSynchronize(P->st), "sync");
-        P->by_pole = by_pole;
-    }
-    hip_check(hipEventRecord(P->ev[3], P->st), "event");
+    if (e1) hip_check(hipEventRecord(e1, P->st), "event");
     double* NB = d_NB ? d_NB : P->d_NB.p;
     double* EB = d_EB ? d_EB : P->d_EB.p;
     hip_check(launch_reduce(P->d_wins.p, (int)P->nwin, P->ngroup, P->d_Dpart.p, P->d_Fpart.p, P->d_slots.p, P->M / 2,
-                            fp32 ? P->d_by_pole.p : nullptr, P->d_Q.p, P->d_D.p,
-                            P->d_F.p, NB, EB, P->st),
+                            fp32 ? P->d_by_pole.p : nullptr, P->d_Q.p, P->d_D.p, P->d_F.p, NB, EB, circ, P->st),
               "reduce kernel");
     if (d_dens)
         hip_check(launch_density(P->d_wins.p, (int)P->nwin, P->kmax, P->d_D.p, P->d_own_g.p, P->d_xptr.p, P->d_xidx.p,
-                                 P->d_xval.p, d_dens, density_blocks(P), P->st),
+                                 P->d_xval.p, d_dens, density_blocks(P), circ + 3, P->st),
                   "density kernel");
-    hip_check(launch_sum2(NB, EB, (int)P->nwin, h_sums ? P->d_pin : P->d_sums.p, P->st), "sum kernel");
+    if (step_stage >= 0)
+        hip_check(launch_sum_step(NB, EB, (int)P->nwin, sums, P->d_step.p, step_stage, P->d_circ.p,
+                                  P->d_pin + 2 * lres_plan::kPinSlots, P->st),
+                  "sum kernel");
+    else
+        hip_check(launch_sum2(NB, EB, (int)P->nwin, sums, circ + 2, P->st), "sum kernel");
+}
+
+// pole solves + reduction (+ density) at (mu, e_lo) decided on the host; sums to host
+// if requested
+void run_pass(lres_plan* P, double mu, double e_lo, double* d_dens, double* d_NB, double* d_EB, double* h_sums) {
+    require(P->have_H, "no Hamiltonian loaded: call lres_set_hamiltonian_csr first");
+    require(mu > e_lo, "mu must lie above e_lo");
+    require(!P->step_queued, "a queued density step is pending: call lres_step_finish first");
+    const double c = 0.5 * (mu + e_lo), R = 0.5 * (mu - e_lo);
+    const int fp32 = P->precision == LRES_MIXED ? 1 : 0;
+    prepare_passes(P, fp32, c, R);
+    hip_check(launch_set_circle(P->d_circ.p, c, R, 0.0, d_dens ? 0.0 : 1.0, P->st), "circle");
+    queue_pass(P, d_dens, d_NB, d_EB, h_sums ? P->d_pin : P->d_sums.p, P->ev[2], P->ev[3]);
     if (h_sums) {
         hip_check(hipStreamSynchronize(P->st), "sync");
         h_sums[0] = P->h_pin[0];
@@ -598,60 +645,26 @@ void inertia(lres_plan* P, const double* sig, int ns, int64_t* h_total, int32_t*
     for (int s = 0; s < ns; ++s) h_total[s] = (int64_t)tot[s];
 }
 
-// Brent's root finder (R. P. Brent, Algorithms for Minimization without
-// Derivatives, 1973, ch. 4): inverse quadratic interpolation / secant steps
-// safeguarded by bisection; tolerance 2 eps |b| + xtol/2 as in the textbook.
-template <typename Fn>
-double brent_root(Fn f, double a, double b, double fa, double fb, double xtol, double ftol, int maxit, int* evals) {
-    const double eps = std::numeric_limits<double>::epsilon();
-    require(fa * fb <= 0.0, "mu bracket does not change sign");
-    if (std::fabs(fa) <= ftol) return a;
-    if (std::fabs(fb) <= ftol) return b;
-    double c = a, fc = fa, d = b - a, e = d;
-    for (int it = 0; it < maxit; ++it) {
-        if ((fb > 0 && fc > 0) || (fb < 0 && fc < 0)) {
-            c = a;
-            fc = fa;
-            d = e = b - a;
-        }
-        if (std::fabs(fc) < std::fabs(fb)) {
-            a = b; b = c; c = a;
-            fa = fb; fb = fc; fc = fa;
-        }
-        const double tol = 2.0 * eps * std::fabs(b) + 0.5 * xtol;
-        const double mid = 0.5 * (c - b);
-        if (std::fabs(mid) <= tol || std::fabs(fb) <= ftol) return b;
-        if (std::fabs(e) >= tol && std::fabs(fa) > std::fabs(fb)) {
-            double p, q, r;
-            const double s = fb / fa;
-            if (a == c) {
-                p = 2.0 * mid * s;
-                q = 1.0 - s;
-            } else {
-                q = fa / fc;
-                r = fb / fc;
-                p = s * (2.0 * mid * q * (q - r) - (b - a) * (r - 1.0));
-                q = (q - 1.0) * (r - 1.0) * (s - 1.0);
-            }
-            if (p > 0) q = -q; else p = -p;
-            if (2.0 * p < std::min(3.0 * mid * q - std::fabs(tol * q), std::fabs(e * q))) {
-                e = d;
-                d = p / q;
-            } else {
-                d = mid;
-                e = d;
-            }
-        } else {
-            d = mid;
-            e = d;
-        }
-        a = b;
-        fa = fb;
-        b += (std::fabs(d) > tol) ? d : (mid > 0 ? tol : -tol);
-        fb = f(b);
-        ++*evals;
+// Pass slots per batch of a device step (lres_step_enqueue), at most; batches after the
+// first hold at least kMoreStepDepth.
+constexpr int kMaxStepDepth = 8, kMoreStepDepth = 4;
+
+// One batch of a queued warm density step: `depth` trial slots (a pass at the
+// controller's circle, then its feed stage), the final stage (a pass at the accepted mu
+// when it was not the last trial; the density either way) and the publish stage, which
+// copies the search state to the mapped staging.  The slots the search does not need
+// leave at once.  Pole-kernel events per slot for the pass statistics.
+void step_batch(lres_plan* P, int depth, double* d_dens) {
+    if ((int)P->ev_slot.size() < 2 * (kMaxStepDepth + 1)) {
+        P->ev_slot.resize(2 * (kMaxStepDepth + 1), nullptr);
+        for (auto& e : P->ev_slot)
+            if (!e) hip_check(hipEventCreate(&e), "hipEventCreate");
     }
-    throw Fail{"mu solve did not converge"};
+    for (int i = 0; i < depth; ++i)
+        queue_pass(P, nullptr, nullptr, nullptr, P->d_sums.p, P->ev_slot[2 * i], P->ev_slot[2 * i + 1],
+                   i + 1 < depth ? lres::kStepFeed : lres::kStepFeedFinal);
+    queue_pass(P, d_dens, nullptr, nullptr, P->d_sums.p, P->ev_slot[2 * depth], P->ev_slot[2 * depth + 1],
+               lres::kStepPublish);
 }
 
 }  // namespace
@@ -801,7 +814,9 @@ int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, 
         P->precision = precision;
         P->st = (hipStream_t)stream;
         for (auto& e : P->ev) hip_check(hipEventCreate(&e), "hipEventCreate");
-        hip_check(hipHostMalloc((void**)&P->h_pin, 2 * lres_plan::kPinSlots * sizeof(double),
+        static_assert(sizeof(lres::StepState) <= lres_plan::kStepPin * sizeof(double), "step state staging");
+        static_assert(sizeof(lres::StepState) % 8 == 0, "step state copied as 8-byte words");
+        hip_check(hipHostMalloc((void**)&P->h_pin, (2 * lres_plan::kPinSlots + lres_plan::kStepPin) * sizeof(double),
                                 hipHostMallocMapped | hipHostMallocCoherent),
                   "hipHostMalloc");
         hip_check(hipHostGetDevicePointer((void**)&P->d_pin, P->h_pin, 0), "hipHostGetDevicePointer");
@@ -964,6 +979,8 @@ int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, 
         P->d_NB.alloc(nwin);
         P->d_EB.alloc(nwin);
         P->d_sums.alloc(2);
+        P->d_circ.alloc(4);
+        P->d_step.alloc(1);
         // whole atom blocks in every owned row: 6 consecutive T-local columns per
         // block, blocks ascending (the density kernel's lane-per-point form)
         P->dens_blocks6 = 1;
@@ -1344,8 +1361,9 @@ int lres_density_last(lres_plan* P, double* d_dens) {
     return guarded([&] {
         require(P != nullptr, "null plan");
         require(d_dens != nullptr, "null density buffer");
+        require(!P->step_queued, "a queued density step is pending: call lres_step_finish first");
         hip_check(launch_density(P->d_wins.p, (int)P->nwin, P->kmax, P->d_D.p, P->d_own_g.p, P->d_xptr.p, P->d_xidx.p,
-                                 P->d_xval.p, d_dens, density_blocks(P), P->st),
+                                 P->d_xval.p, d_dens, density_blocks(P), nullptr, P->st),
                   "density kernel");
     });
 }
@@ -1451,15 +1469,22 @@ int lres_mu_solve(lres_plan* P, double N, double e_lo, double lo, double hi, dou
         double last_mu = NAN, last_f = NAN, last_e = NAN;
         auto f = [&](double mu) {
             run_pass(P, mu, e_lo, d_dens, nullptr, nullptr, sums);
+            ++evals;
             last_mu = mu;
             last_f = 2.0 * sums[0] - N;
             last_e = sums[1];
             return last_f;
         };
         double flo = f_lo, fhi = f_hi;
-        if (std::isnan(flo)) { flo = f(lo); ++evals; }
-        if (std::isnan(fhi)) { fhi = f(hi); ++evals; }
-        const double mu = brent_root(f, lo, hi, flo, fhi, xtol, ftol, 200, &evals);
+        if (std::isnan(flo)) flo = f(lo);
+        if (std::isnan(fhi)) fhi = f(hi);
+        lres::BrentRC br;
+        double x = 0.0;
+        int r = lres::brent_begin(br, lo, hi, flo, fhi, xtol, ftol, 200, &x);
+        require(r != lres::kRcFail, "mu bracket does not change sign");
+        while (r == lres::kRcEval) r = lres::brent_feed(br, f(x), &x);
+        require(r == lres::kRcDone, "mu solve did not converge");
+        const double mu = x;
         *h_mu = mu;
         if (h_info) {
             const bool last = mu == last_mu;
@@ -1476,80 +1501,112 @@ int lres_mu_solve_warm(lres_plan* P, double N, double e_lo, double mu0, double f
     return guarded([&] {
         require(P != nullptr, "null plan");
         require(h_mu && h_info, "null output");
-        int evals = 0;
         double sums[2] = {0.0, 0.0};
-        double last_mu = NAN, last_f = NAN, last_e = NAN;
         // LRES_TRACE_MU=1: one line per trial pass on stderr (mu, count, tolerance)
         static const bool trace = std::getenv("LRES_TRACE_MU") != nullptr;
         if (trace)
             fprintf(stderr, "[lres mu] warm search from mu %.12f f %.3e slope %.3e ftol %.3e (%s)\n", mu0, f0, slope,
                     ftol, P->precision == LRES_MIXED ? "c64" : "f64");
-        auto f = [&](double mu) {
-            run_pass(P, mu, e_lo, nullptr, nullptr, nullptr, sums);
-            ++evals;
-            last_mu = mu;
-            last_f = 2.0 * sums[0] - N;
-            last_e = sums[1];
-            if (trace) fprintf(stderr, "[lres mu]   pass mu %.12f f %.3e\n", mu, last_f);
-            return last_f;
-        };
-        // slope of the count between two trial points when it is positive and finite
-        auto note = [&](double m0, double g0, double m1, double g1) {
-            if (m1 != m0 && std::isfinite(g1) && (g1 - g0) / (m1 - m0) > 0) slope = (g1 - g0) / (m1 - m0);
-        };
-        double a = mu0, fa = f0;
-        double step = slope != 0.0 ? -fa / slope : (fa < 0 ? 1e-3 : -1e-3);
-        double mu = NAN;
-        bool found = false, by_brent = false;
-        for (int it = 0; it < 60 && !found; ++it) {
-            step = std::min(std::max(step, -0.25), 0.25);
-            if (std::fabs(step) < 1e-12) step = fa < 0 ? 1e-12 : -1e-12;
-            double b = a + step;
-            if (b <= e_lo + 1e-6) b = 0.5 * (a + e_lo + 1e-6);
-            const double fb = f(b);
-            note(a, fa, b, fb);
-            if (std::fabs(fb) <= ftol) {
-                mu = b;
-                found = true;
-                break;
-            }
-            if ((fa < 0) != (fb < 0)) {
-                const bool lo_a = a < b;
-                const double lo = lo_a ? a : b, hi = lo_a ? b : a, flo = lo_a ? fa : fb, fhi = lo_a ? fb : fa;
-                // where the bracket's high end lies far up the count's exponential rise
-                // towards the conduction band (f(hi) > 10 |f(lo)|, e.g. f in [-0.06, +400]),
-                // interpolation on f creeps along the low end: Brent then runs on
-                // g = log(1 + f / A), A = 1.25 |f(lo)|, which has the root and monotonicity
-                // of f (f >= f(lo) on the bracket keeps 1 + f / A > 0) and is nearly linear
-                // there; on a nearly linear bracket Brent runs on f itself
-                int bevals = 0;
-                if (fhi > 10.0 * std::fabs(flo)) {
-                    const double Ab = 1.25 * std::fabs(flo);
-                    auto g = [&](double m) { return std::log1p(f(m) / Ab); };
-                    const double gtol = std::log1p(ftol / Ab);    // |g| <= gtol implies |f| <= ftol
-                    mu = brent_root(g, lo, hi, std::log1p(flo / Ab), std::log1p(fhi / Ab), xtol, gtol, 200, &bevals);
-                } else {
-                    mu = brent_root(f, lo, hi, flo, fhi, xtol, ftol, 200, &bevals);
-                }
-                found = by_brent = true;
-                break;
-            }
-            const double sl = (fb - fa) / (b - a);
-            // same side of the root: secant extrapolation with 20% overshoot to bracket it
-            step = sl > 0 ? -1.2 * fb / sl : 4.0 * step;
-            a = b;
-            fa = fb;
+        lres::WarmSearch s;
+        lres::warm_begin(s, N, e_lo, mu0, f0, slope, ftol, xtol);
+        while (!lres::warm_terminal(s)) {
+            run_pass(P, s.x, e_lo, nullptr, nullptr, nullptr, sums);
+            if (trace) fprintf(stderr, "[lres mu]   pass mu %.12f f %.3e\n", s.x, 2.0 * sums[0] - N);
+            lres::warm_feed(s, sums[0], sums[1]);
         }
-        *h_mu = mu;
-        const bool last = found && mu == last_mu;
-        h_info[0] = evals;
-        h_info[1] = last ? last_f : NAN;
+        require(s.phase != lres::kWarmNoConv, "mu solve did not converge");
+        const bool found = s.phase == lres::kWarmDone;
+        const bool last = found && s.mu == s.last_mu;
+        *h_mu = found ? s.mu : NAN;
+        h_info[0] = s.evals;
+        h_info[1] = last ? s.last_f : NAN;
         h_info[2] = last ? 1.0 : 0.0;
-        h_info[3] = last ? last_e : NAN;
-        h_info[4] = slope;
+        h_info[3] = last ? s.last_esum : NAN;
+        h_info[4] = s.slope;
         h_info[5] = found ? 0.0 : 1.0;
-        h_info[6] = last ? sums[0] : NAN;
-        h_info[7] = by_brent ? 1.0 : 0.0;
+        h_info[6] = last ? s.last_nsum : NAN;
+        h_info[7] = s.by_brent ? 1.0 : 0.0;
+    });
+}
+
+int lres_step_enqueue(lres_plan* P, double N, double mu0, double e_lo, double margin, double keep_tol, double ftol,
+                      double xtol, double slope, int lag, int depth, double* d_dens) {
+    return guarded([&] {
+        require(P != nullptr, "null plan");
+        require(P->have_H, "no Hamiltonian loaded");
+        require(!P->step_queued, "a queued density step is pending: call lres_step_finish first");
+        require(mu0 > e_lo, "mu must lie above e_lo");
+        require(depth >= 1 && depth <= kMaxStepDepth, "depth must be in 1.." + std::to_string(kMaxStepDepth));
+        require(d_dens != nullptr, "null density buffer");
+        const int fp32 = P->precision == LRES_MIXED ? 1 : 0;
+        prepare_passes(P, fp32, 0.5 * (mu0 + e_lo), 0.5 * (mu0 - e_lo));
+        // the keep-edge test (lcao3d.py:420) on the device: counts at e_lo + margin / 3 and
+        // e_lo + 2 margin, read by the first controller stage
+        const double sig[2] = {e_lo + margin / 3, e_lo + 2 * margin};
+        inertia_launch(P, sig, 2, nullptr);
+        hip_check(launch_mu_begin(P->d_step.p, P->d_itot.p, N, e_lo, mu0, slope, keep_tol, ftol, xtol, lag,
+                                  P->d_circ.p, P->st),
+                  "mu controller");
+        step_batch(P, depth, d_dens);
+        P->step_queued = true;
+        P->step_depth = depth;
+    });
+}
+
+int lres_step_finish(lres_plan* P, double* d_dens, double* h_out) {
+    return guarded([&] {
+        require(P != nullptr, "null plan");
+        require(P->step_queued, "no density step queued");
+        require(h_out != nullptr, "null output");
+        P->step_queued = false;
+        const int fp32 = P->precision == LRES_MIXED ? 1 : 0;
+        // later batches serve searches longer than the first batch held: deeper ones
+        int depth = P->step_depth;
+        const int more = std::max(depth, kMoreStepDepth);
+        const lres::StepState* st = reinterpret_cast<const lres::StepState*>(P->h_pin + 2 * lres_plan::kPinSlots);
+        int evals_before = 0, final_before = 0, batches = 1;
+        for (;;) {
+            hip_check(hipStreamSynchronize(P->st), "sync");
+            // pole-kernel time of the slots that ran: the trials first, then the final pass
+            const int ran = st->s.evals - evals_before;
+            for (int i = 0; i < ran && i < depth; ++i) {
+                float ms = 0;
+                if (hipEventElapsedTime(&ms, P->ev_slot[2 * i], P->ev_slot[2 * i + 1]) == hipSuccess) {
+                    P->stat_passes[fp32] += 1;
+                    P->stat_ms[fp32] += ms;
+                }
+            }
+            if (st->final_ran > final_before) {
+                float ms = 0;
+                if (hipEventElapsedTime(&ms, P->ev_slot[2 * depth], P->ev_slot[2 * depth + 1]) == hipSuccess) {
+                    P->stat_passes[fp32] += 1;
+                    P->stat_ms[fp32] += ms;
+                }
+            }
+            evals_before = st->s.evals;
+            final_before = st->final_ran;
+            if (lres::warm_terminal(st->s)) break;
+            // the search needs more trials than were queued: another batch
+            require(++batches <= 64, "device mu search: too many trial passes");
+            hip_check(launch_mu_step(P->d_step.p, lres::kStepResume, P->d_circ.p, P->d_pin + 2 * lres_plan::kPinSlots,
+                                     P->st),
+                      "mu controller");
+            depth = more;
+            step_batch(P, depth, d_dens);
+        }
+        const lres::WarmSearch& s = st->s;
+        const bool done = s.phase == lres::kWarmDone;
+        h_out[0] = s.phase == lres::kWarmDone ? 0.0 : s.phase == lres::kWarmEdge ? 1.0 : 2.0;
+        h_out[1] = s.outcome;
+        h_out[2] = s.evals + st->final_ran;
+        h_out[3] = done ? s.mu : NAN;
+        h_out[4] = done ? s.last_f : NAN;
+        h_out[5] = done ? s.last_nsum : NAN;
+        h_out[6] = done ? s.last_esum : NAN;
+        h_out[7] = s.slope;
+        h_out[8] = s.by_brent;
+        h_out[9] = batches;
+        require(s.phase != lres::kWarmNoConv, "mu solve did not converge");
     });
 }
 

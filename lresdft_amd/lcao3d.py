@@ -286,7 +286,7 @@ class LRFTLCAOCpp:
     (one GPU); ``lag_mu`` keeps the reference's meaning (lcao3d.py:423-427)."""
 
     def __init__(self, basis, cells, R_w=12.1, M=64, margin=0.3, nthreads=1, lag_mu=False, reuse_vectors=True,
-                 precision="fp64", mu_xtol=1e-12, device_h=True, mu_ftol=1e-11):
+                 precision="fp64", mu_xtol=1e-12, device_h=True, mu_ftol=1e-11, device_step=True, step_depth=2):
         if M % 2:
             raise ValueError("M must be even")
         self.b, self.M, self.margin = basis, M, margin
@@ -314,6 +314,16 @@ class LRFTLCAOCpp:
         self._slope = None
         self.pass_log = []      # passes per density call
         self.count_tol = None   # optional looser mu tolerance (relative to N) set by the SCF driver
+        # warm steps (a previous mu) queued on the device with the mu search between the
+        # passes (lres_step_enqueue): one host wait per step; step_depth trial passes per
+        # queued batch after the keep pass.  device_step=False: the search on the host.
+        self.device_step = bool(device_step)
+        self.step_depth = int(step_depth)
+        self._queued = None     # N of the queued step
+        self._p_step = None     # plan.passes when the step was queued
+        self._new_edge = False  # the queued step's keep-edge test failed: go to a new edge
+        self.step_batches = 0   # trial batches of the last device step (more than one: work
+                                # queued behind the step ran before its density)
 
     @property
     def passes(self):
@@ -421,6 +431,85 @@ class LRFTLCAOCpp:
             self.plan.last = None
         return mu
 
+    def _keep_tol(self, N):
+        # the reference's keep-mu rule |count| <= 1e-8 N (lcao3d.py:412, 428), floored at
+        # the pass precision; under an SCF driver that sets count_tol (relative to its
+        # density residual) the kept count error also stays below count_tol, else an SCF
+        # asked for a residual below ~1e-8 would stall on the kept count error
+        keep = 1e-8 if self.count_tol is None else min(1e-8, self.count_tol)
+        return max(keep * N, self._ftol(N))
+
+    def _step_queue(self, N):
+        """Queue the warm step at the current (mu, e_lo) on the device: keep-edge test,
+        keep pass, search, density into plan.dens (lres_step_enqueue)."""
+        nat.check(nat.lib().lres_step_enqueue(self.plan.h, float(N), float(self.mu), float(self.e_lo),
+                                              float(self.margin), float(self._keep_tol(N)), float(self._ftol(N)),
+                                              float(self.mu_xtol), float(self._slope or 0.0), int(bool(self.lag_mu)),
+                                              int(self.step_depth), nat.C.c_void_p(self.plan.dens.data_ptr())))
+        self._queued = N
+
+    def _step_collect(self):
+        """Wait for the queued step: (plan density without spin factor, e_band, mu), or
+        None when its keep-edge test failed (a new filter edge is needed); StaleMu when
+        the search found no bracket."""
+        self._queued = None
+        out = np.zeros(10)
+        nat.check(nat.lib().lres_step_finish(self.plan.h, nat.C.c_void_p(self.plan.dens.data_ptr()), nat.pf(out)))
+        self.plan.passes += int(out[2])
+        self.step_batches = int(out[9])
+        if out[7] > 0:
+            self._slope = float(out[7])
+        status = int(out[0])
+        if status != 0:
+            self.plan.last = None
+            if status == 1:
+                return None
+            raise StaleMu("no electron-count bracket around the previous mu")
+        mu = float(out[3])
+        self.plan.last = (mu, float(self.e_lo), float(out[5]), float(out[6]), True)
+        self.history.append(("warm-keep", "warm-lag", "warm-resolve")[int(out[1])])
+        if _TRACE_MU:
+            print(f"[lres mu] device step: mu {mu:.12f} count {out[4]:.3e} passes {int(out[2])} "
+                  f"batches {int(out[9])} ({self.precision})", file=sys.stderr, flush=True)
+        self.mu = mu
+        return self.plan.dens, 2.0 * float(out[6]), mu
+
+    def density_queue(self, v_dev, N=None, potential_set=False):
+        """Queue density_device's warm step without waiting, so that a caller can queue
+        work that reads plan.dens (the SCF residual) behind it and wait once
+        (density_collect).  Returns False, queuing nothing, when there is no previous mu
+        or device steps are off; then call density_device."""
+        if not (self.device_h and self.device_step) or self.mu is None:
+            return False
+        N = self.b.cr.n_elec if N is None else N
+        if potential_set:
+            self.plan.last = None
+            self.plan._v_keep = v_dev
+        else:
+            self.plan.set_potential(v_dev)
+        self._p_step = self.plan.passes
+        self._step_queue(N)
+        return True
+
+    def density_collect(self):
+        """Result of density_queue's step: (plan density without the spin factor, e_band,
+        mu), or None when the step needs the cold paths (a new filter edge, or no mu
+        bracket); then call density_device(v, potential_set=True, ...), which runs them
+        on the same potential.  When the search needed more trial passes than the first
+        queued batch held (step_batches > 1), work queued behind the step ran before the
+        density was formed and must be queued again."""
+        try:
+            out = self._step_collect()
+        except StaleMu:
+            self.mu = None
+            out = None
+        if out is None:
+            self._new_edge = self.mu is not None
+            return None
+        self.pass_log.append(self.plan.passes - self._p_step)
+        self._p_step = None
+        return out
+
     def refine(self, N=None):
         """fp64 finish of a converged step (on the H of the last call): an fp64
         pass at the current (mu, e_lo) and, if its count misses N by more than
@@ -494,7 +583,8 @@ class LRFTLCAOCpp:
             self.plan._v_keep = v_dev
         else:
             self.plan.set_potential(v_dev)
-        p0 = self.plan.passes
+        p0 = self.plan.passes if self._p_step is None else self._p_step
+        self._p_step = None
         try:
             out = self._density_loaded(N)
         except StaleMu:
@@ -504,14 +594,16 @@ class LRFTLCAOCpp:
         return (2.0 * out[0],) + out[1:] if spin_scaled else out
 
     def _density_loaded(self, N):
-        # the reference's keep-mu rule |count| <= 1e-8 N (lcao3d.py:412, 428), floored at
-        # the pass precision; under an SCF driver that sets count_tol (relative to its
-        # density residual) the kept count error also stays below count_tol, else an SCF
-        # asked for a residual below ~1e-8 would stall on the kept count error
-        keep = 1e-8 if self.count_tol is None else min(1e-8, self.count_tol)
-        tol = max(keep * N, self._ftol(N))
+        tol = self._keep_tol(N)
         m = self.margin
-        if self.mu is not None:
+        new_edge, self._new_edge = self._new_edge, False
+        if self.mu is not None and not new_edge and self.device_h and self.device_step:
+            # the warm step on the device, one wait (lres_step_enqueue / lres_step_finish)
+            self._step_queue(N)
+            out = self._step_collect()
+            if out is not None:
+                return out
+        elif self.mu is not None and not new_edge:
             mu, e_lo = self.mu, self.e_lo
             c_lo, c_hi = self.plan.inertia([e_lo + m / 3, e_lo + 2 * m])
             if c_lo == 0 and c_hi > 0:            # keep the filter edge (lcao3d.py:420)

@@ -16,6 +16,34 @@ import numpy as np
 from . import _native as nat
 
 
+_GELSD = {}
+
+
+def diis_coefficients(gram, slots):
+    """Pulay coefficients of the stored residuals `slots` (oldest first) from their Gram
+    matrix: the least-squares solution of [[B, 1], [1^T, 0]] [c; lambda] = [0; 1]
+    (lcao3d.py:268-273).  LAPACK's dgelsd with numpy.linalg.lstsq's rcond (eps times
+    the order), called directly: the same solution without lstsq's wrapper, which
+    costs more than the solve at this size (the host sits between two iterations of
+    GPU work while it runs)."""
+    from scipy.linalg import lapack
+    k = len(slots)
+    idx = np.asarray(slots)
+    Bm = np.ones((k + 1, k + 1))
+    Bm[k, k] = 0.0
+    Bm[:k, :k] = gram[idx[:, None], idx]
+    rhs = np.zeros((k + 1, 1))
+    rhs[k, 0] = 1.0
+    lw = _GELSD.get(k)
+    if lw is None:
+        w = lapack.dgelsd_lwork(k + 1, k + 1, 1)
+        lw = _GELSD[k] = (int(w[0]), int(w[1]))
+    x, _, _, info = lapack.dgelsd(Bm, rhs, lw[0], lw[1], np.finfo(np.float64).eps * (k + 1))
+    if info != 0:
+        return np.linalg.lstsq(Bm, rhs[:, 0], rcond=None)[0][:k]
+    return x[:k, 0]
+
+
 class GridOps:
     """FFT Hartree, ion potential and PZ81 LDA on the device for one crystal."""
 
@@ -234,8 +262,11 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
     converge slowly.  Measured at 1024 electrons (13 iterations in each case,
     14 at 1e-5): 45.7 ms with the hand-over at 1e-4, 41.2 ms at 3e-5, so the
     default floor is 3e-5.  The energy and the returned density are always fp64 results at the
-    converged input density.  One device-to-host transfer per iteration (the
-    residual norm and the new row of the DIIS Gram matrix together).  The
+    converged input density.  A warm step (a previous mu) is queued with its mu
+    search on the device (solver.density_queue) and the residual behind it, so
+    the iteration waits once: for the step's outcome and, in the same transfer,
+    the residual norm and the new row of the DIIS Gram matrix (a new filter edge
+    or a stale mu falls back to the host-driven paths).  The
     mixing, potential and H(v) of every iteration after the first form one
     PotentialStep, replayed from a captured graph when ``graph`` is true (same
     results as eager launches, bit for bit)."""
@@ -276,8 +307,12 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
     cfull = None
     if step is not None:
         stats_dev = torch.empty(ndiis + 1, dtype=torch.float64, device=grid.device)
+        stats_pin = torch.empty(ndiis + 1, dtype=torch.float64, pin_memory=True)
         if ndiis > nat.LRES_MAX_HIST:
             raise ValueError(f"ndiis must be at most {nat.LRES_MAX_HIST}")
+        # warm steps queued on the device with the residual behind them: one wait per
+        # iteration (needs the solver's plan on this stream)
+        queue_ok = hasattr(solver, "density_queue") and solver.plan.stream == torch.cuda.current_stream()
     for it in range(1, maxiter + 1):
         if cfull is None:
             v = grid.potential(dens, vion)
@@ -307,13 +342,36 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
         if step is not None:
             # density without the spin factor; one fused pass forms the residual
             # 2 n_out - n_in, stores the history row and the Gram row
-            nraw, eband, mu = solver.density_device(v, potential_set=preset, spin_scaled=False)
+            din = dens.reshape(-1).contiguous()
+
+            def residual(nraw):
+                nat.check(nat.lib().lres_scf_residual(
+                    nraw.data_ptr(), 2.0, din.data_ptr(), cr.ng, ndiis, slot, Nh.data_ptr(), Rh.data_ptr(),
+                    grid._workspace().data_ptr(), stats_dev.data_ptr(),
+                    nat.C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+            out = None
+            queued = queue_ok and solver.density_queue(v, potential_set=preset)
+            if queued:
+                # the warm step and the residual of its density queued together, one wait
+                residual(solver.plan.dens)
+                stats_pin.copy_(stats_dev, non_blocking=True)
+                out = solver.density_collect()
+                if out is not None:
+                    nraw, eband, mu = out
+                    if solver.step_batches > 1:
+                        # the search took a further batch: the residual above ran
+                        # before this density was formed
+                        residual(nraw)
+                        stats = stats_dev.cpu().numpy()
+                    else:
+                        stats = stats_pin.numpy().copy()
+            if out is None:
+                # first step, new filter edge or stale mu: the host-driven paths
+                nraw, eband, mu = solver.density_device(v, potential_set=preset or queued, spin_scaled=False)
+                residual(nraw)
+                stats = stats_dev.cpu().numpy()
             t_solver += time.perf_counter() - ts
-            nat.check(nat.lib().lres_scf_residual(
-                nraw.data_ptr(), 2.0, dens.reshape(-1).contiguous().data_ptr(), cr.ng, ndiis, slot, Nh.data_ptr(),
-                Rh.data_ptr(), grid._workspace().data_ptr(), stats_dev.data_ptr(),
-                nat.C.c_void_p(torch.cuda.current_stream().cuda_stream)))
-            stats = stats_dev.cpu().numpy()
             nout = None
         else:
             nout, eband, mu = solver.density_device(v)
@@ -335,17 +393,12 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
             solver.set_precision("fp64")
             handed_over = True
         slots = live
-        for sq in slots:
-            gram[slot, sq] = gram[sq, slot] = stats[1 + sq]
+        ls = np.asarray(slots)
+        gram[slot, ls] = gram[ls, slot] = stats[1 + ls]
         k = len(slots)
         cfull = np.zeros(ndiis)
         if k > 1:
-            Bm = np.zeros((k + 1, k + 1))
-            Bm[:k, :k] = gram[np.ix_(slots, slots)]
-            Bm[k, :k] = Bm[:k, k] = 1.0
-            rhs = np.zeros(k + 1)
-            rhs[k] = 1.0
-            cfull[slots] = np.linalg.lstsq(Bm, rhs, rcond=None)[0][:k]
+            cfull[ls] = diis_coefficients(gram, slots)
         else:
             cfull[slot] = 1.0      # plain mixing: dens + mix * resid
     if adaptive:
