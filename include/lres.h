@@ -81,9 +81,11 @@ int lres_plan_sizes(const lres_plan* plan, int32_t* h_n, int32_t* h_k);
  * the first pass in that precision has run the autotuner).  kind 0 / 1: vector
  * ALU Gauss-Jordan with a T x T tile per thread on a G x G grid and a register
  * budget of 128 / 512; kind 2: matrix cores (MFMA), T = 16 x 16 tiles per
- * dimension, G = waves per (window, pole group).  By default buckets of up to
- * 128 functions use kind 2 in both precisions and larger ones the vector-ALU
- * kernels; the environment variable LRES_POLE_KERNEL=valu, read at plan
+ * dimension, G = waves per (window, pole group); G = 0 (complex64, chosen only
+ * with LRES_POLE_FUSED=1 or =2) is the window-fused form: four waves per window, one
+ * pole each, the pole sums and the reduction in the kernel, the same bits.  By default
+ * buckets of up to 128 functions use kind 2 in both precisions and larger ones the
+ * vector-ALU kernels; the environment variable LRES_POLE_KERNEL=valu, read at plan
  * creation and tuning, selects the vector-ALU family for both. */
 int lres_plan_buckets(const lres_plan* plan, int32_t* h_out, int cap, int* h_nbucket);
 

@@ -15,6 +15,17 @@ struct WinDesc {
     int32_t n, k, m, nown;
 };
 
+// Outputs of a pole kernel that also does the reduction of its windows (the complex64
+// matrix-core kernel with all poles of a window in one workgroup): D_B, F_B (upper
+// triangles, as k_reduce writes them), Q_B read, N_B and E_B written.
+struct WinOut {
+    double* D;
+    double* F;
+    const double* Q;
+    double* NB;
+    double* EB;
+};
+
 }  // namespace lres
 
 hipError_t launch_gather_csr(const lres::WinDesc* wins, int nwin, const int64_t* Wflat, const int64_t* Woff,
@@ -40,7 +51,8 @@ hipError_t launch_pole_gj(int T, int G, int wide, int fp32, const lres::WinDesc*
                           const double* SW, const double* HW, int M, int ngroup, const double* circ,
                           double* Dpart, double* Fpart, hipStream_t st);
 // D, F, N_B, E_B from the pole kernels' partials: group sums (Dpart, Fpart) or,
-// for windows with by_pole[w] != 0, nslot per-pole float pairs in `slots`
+// for windows with by_pole[w] == 1, nslot per-pole float pairs in `slots`; windows with
+// by_pole[w] == 2 were reduced by the window-fused pole kernel and are left alone
 hipError_t launch_reduce(const lres::WinDesc* wins, int nwin, int ngroup, const double* Dpart, const double* Fpart,
                          const float* slots, int nslot, const int8_t* by_pole, const double* Q, double* D, double* F,
                          double* NB, double* EB, const double* circ, hipStream_t st);
@@ -94,10 +106,14 @@ hipError_t launch_inertia_big(int bmax, int nmax, const lres::WinDesc* wins, con
 // tiles per dimension (2..8), nwc = waves per (window, pole group) splitting the
 // tile columns (1..nt).  One object per (precision, nt),
 // compiled in parallel.
+// nwc > 0: waves per (window, pole group); nwc == 0 (fp32 only): the window-fused form,
+// four waves per window each on its own pole, the poles' D and F summed in LDS in pole
+// order and reduced in place into `out` (no partials, no k_reduce for those windows).
 #define LRES_MFMA_PART_DECL(r, nt)                                                                              \
     hipError_t lres_mfma_part_##r##_##nt(int nwc, const lres::WinDesc* wins, const int32_t* ids, int nids,     \
                                          const double* SW, const double* HW, int M, int ngroup,               \
-                                         const double* circ, double* Dpart, double* Fpart, int acc, hipStream_t st);
+                                         const double* circ, double* Dpart, double* Fpart, int acc,           \
+                                         lres::WinOut out, hipStream_t st);
 #define LRES_MFMA_PARTS(X) X(float, 2) X(float, 3) X(float, 4) X(float, 5) X(float, 6) X(float, 7) X(float, 8) \
     X(float, 9) X(float, 10) X(float, 11) X(float, 12) X(float, 13) X(float, 14) X(float, 15) X(float, 16)     \
     X(double, 2) X(double, 3) X(double, 4) X(double, 5) X(double, 6) X(double, 7) X(double, 8)
@@ -116,7 +132,7 @@ hipError_t launch_inertia_mfma(int nt, const lres::WinDesc* wins, const int32_t*
 // packed upper triangles of acc entries (at least k(k+1)/2 of every window launched)
 hipError_t launch_pole_mfma(int fp32, int nt, int nwc, const lres::WinDesc* wins, const int32_t* ids, int nids,
                             const double* SW, const double* HW, int M, int ngroup, const double* circ,
-                            double* Dpart, double* Fpart, int acc, hipStream_t st);
+                            double* Dpart, double* Fpart, int acc, lres::WinOut out, hipStream_t st);
 // accumulator registers per lane of one wave of the MFMA pole kernel
 int mfma_acc_regs(int fp32, int nt, int nwc);
 // scratch bytes per lane of an MFMA variant (register spills; -1: no such variant)

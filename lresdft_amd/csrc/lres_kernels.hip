@@ -299,7 +299,8 @@ __global__ void __launch_bounds__(1024) k_reduce(const WinDesc* __restrict__ win
     const int64_t kk = (int64_t)k * k;
     const double* dp = Dpart + d.kk_off * ngroup;
     const double* fp = Fpart + d.kk_off * ngroup;
-    const bool poles = by_pole != nullptr && by_pole[w] != 0;
+    if (by_pole != nullptr && by_pole[w] == 2) return;   // reduced by the window-fused pole kernel
+    const bool poles = by_pole != nullptr && by_pole[w] == 1;
     const float2* gs = slots + d.kk_off * nslot;
     double* Dw = D + d.kk_off;
     double* Fw = F + d.kk_off;
@@ -359,8 +360,8 @@ __global__ void __launch_bounds__(1024) k_reduce(const WinDesc* __restrict__ win
             Fw[q] = fv;
             double qv = Qw[q];
             if (j != i) qv *= 2.0;
-            sn += dv * qv;
-            se += fv * qv;
+            sn = fma(dv, qv, sn);
+            se = fma(fv, qv, se);
         }
     }
     const double tn = block_sum(sn, red);
@@ -1063,6 +1064,7 @@ hipError_t launch_pole_gj(int T, int G, int wide, int fp32, const WinDesc* wins,
 }
 
 int mfma_acc_regs(int fp32, int nt, int nwc) {
+    if (nwc == 0) nwc = 1;   // the window-fused form: one wave per matrix
     // the k-th tile column of a wave keeps the tile rows below min(nt, nwc (k + 1))
     const int tj = (nt + nwc - 1) / nwc;
     int tiles = 0;
@@ -1070,18 +1072,21 @@ int mfma_acc_regs(int fp32, int nt, int nwc) {
     return tiles * (fp32 ? 8 : 16);
 }
 
-static bool mfma_nwc_ok(int nt, int nwc) {
+// nwc 0: the window-fused complex64 form (up to 8 tile columns)
+static bool mfma_nwc_ok(int fp32, int nt, int nwc) {
+    if (nwc == 0) return fp32 && nt >= 2 && nt <= 8;
     if (nt > 8) return nt <= 16 && nwc == 8;
     return nt >= 2 && nwc >= 1 && nwc <= nt;
 }
 
 hipError_t launch_pole_mfma(int fp32, int nt, int nwc, const WinDesc* wins, const int32_t* ids, int nids,
                             const double* SW, const double* HW, int M, int ngroup, const double* circ,
-                            double* Dpart, double* Fpart, int acc, hipStream_t st) {
-    if (!mfma_nwc_ok(nt, nwc)) return hipErrorInvalidValue;
+                            double* Dpart, double* Fpart, int acc, WinOut out, hipStream_t st) {
+    if (!mfma_nwc_ok(fp32, nt, nwc)) return hipErrorInvalidValue;
 #define LRES_MFMA_CALL(r, NT)                                                                                  \
     if (nt == NT && (fp32 != 0) == (sizeof(r) == 4))                                                           \
-        return lres_mfma_part_##r##_##NT(nwc, wins, ids, nids, SW, HW, M, ngroup, circ, Dpart, Fpart, acc, st);
+        return lres_mfma_part_##r##_##NT(nwc, wins, ids, nids, SW, HW, M, ngroup, circ, Dpart, Fpart, acc, out, \
+                                         st);
     LRES_MFMA_PARTS(LRES_MFMA_CALL)
 #undef LRES_MFMA_CALL
     return hipErrorInvalidValue;
@@ -1098,11 +1103,11 @@ hipError_t launch_inertia_mfma(int nt, const WinDesc* wins, const int32_t* ids, 
 }
 
 int mfma_scratch_bytes(int fp32, int nt, int nwc) {
-    if (!mfma_nwc_ok(nt, nwc)) return -1;
+    if (!mfma_nwc_ok(fp32, nt, nwc)) return -1;
 #define LRES_MFMA_CALL(r, NT)                                                                                  \
     if (nt == NT && (fp32 != 0) == (sizeof(r) == 4))                                                           \
-        return (int)lres_mfma_part_##r##_##NT(-nwc, nullptr, nullptr, 0, nullptr, nullptr, 0, 0, nullptr,      \
-                                              nullptr, nullptr, 0, nullptr);
+        return (int)lres_mfma_part_##r##_##NT(nwc == 0 ? -100 : -nwc, nullptr, nullptr, 0, nullptr, nullptr, 0, \
+                                              0, nullptr, nullptr, nullptr, 0, WinOut{}, nullptr);
     LRES_MFMA_PARTS(LRES_MFMA_CALL)
 #undef LRES_MFMA_CALL
     return -1;

@@ -137,15 +137,25 @@ __device__ __forceinline__ void static_for(F&& f) {
     }
 }
 
-template <typename R, int NT, int NWC, int WPE>
-__global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(WPE))) k_pole_mfma(const WinDesc* __restrict__ wins,
+// NP > 1 (complex64, NWC = 1): the window-fused form.  One workgroup of NP waves per
+// window, wave q sweeping poles q, q + NP, ... with a row panel of its own in LDS (one
+// wave per matrix: its LDS operations complete in order, so one buffer and a wave-level
+// wait replace the double buffer and the workgroup barrier).  After each round of NP
+// poles the waves add their Re(w G) and Re(w z G), rounded to float as the per-pole
+// slots hold them, to double accumulators in LDS in wave (= pole) order; at the end the
+// workgroup forms D_B, F_B, N_B and E_B exactly as k_reduce does from the slots (the
+// same additions in the same order, the trace sums over k_reduce's 1024 virtual threads
+// and reduction tree), so both forms give the same bits.
+template <typename R, int NT, int NWC, int WPE, int NP>
+__global__ void __launch_bounds__(64 * NWC * NP) __attribute__((amdgpu_waves_per_eu(WPE))) k_pole_mfma(const WinDesc* __restrict__ wins,
                                                         const int32_t* __restrict__ ids,
                                                         const double* __restrict__ SW,
                                                         const double* __restrict__ HW, int nids, int M, int ngroup,
                                                         const double* __restrict__ circ, double* __restrict__ Dpart,
-                                                        double* __restrict__ Fpart, int acc) {
+                                                        double* __restrict__ Fpart, int acc, WinOut wout) {
     using MF = Mfma<R>;
     using V = typename MF::V;
+    static_assert(NP == 1 || (NWC == 1 && sizeof(R) == 4), "the window-fused form is complex64, one wave per matrix");
     constexpr int TJ = (NT + NWC - 1) / NWC;   // tile columns per wave
     constexpr int NR = 16 * NT;                 // padded order
     constexpr bool kEven = NT % NWC == 0;
@@ -155,11 +165,13 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
     // B operands read one column per lane, and at a 32-byte stride lanes j and j + 8
     // would share LDS banks
     constexpr int RS = sizeof(R) == 8 ? 5 : 4;
-    __shared__ __attribute__((aligned(16))) R rpr[2][NR][RS];
-    __shared__ __attribute__((aligned(16))) R rpi[2][NR][RS];
+    constexpr int NBUF = NP > 1 ? NP : 2;
+    __shared__ __attribute__((aligned(16))) R rpr[NBUF][NR][RS];
+    __shared__ __attribute__((aligned(16))) R rpi[NBUF][NR][RS];
     // fp64 with acc > 0: the group's D and F partial sums over its poles, packed upper
     // triangles of acc entries each (dynamic LDS), stored to Dpart / Fpart once at the
-    // end instead of a read-modify-write of global memory per pole
+    // end instead of a read-modify-write of global memory per pole; NP > 1: the window's
+    // D and F sums over all poles
     extern __shared__ double pacc[];
     // the circle (c, R) of the pass; circ[2] != 0 marks a pass the device-side mu search
     // did not need (every workgroup leaves at once)
@@ -170,14 +182,30 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
     // group of a window on one XCD, dispatched together: the window's S_W and H_W are
     // then read from HBM about once per pass and from that XCD's L2 by its other groups
     const int L = blockIdx.x;
-    const int slot = (L / (8 * ngroup)) * 8 + (L & 7);
-    const int gq = (L >> 3) % ngroup;
+    const int slot = NP > 1 ? L : (L / (8 * ngroup)) * 8 + (L & 7);
+    const int gq = NP > 1 ? 0 : (L >> 3) % ngroup;
     if (slot >= nids) return;
     const int w = ids[slot];
     const WinDesc d = wins[w];
     const int n = d.n, m = d.m, k = d.k;
     const int lane = threadIdx.x & 63;
-    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    // NP > 1: wave `mat` holds its own matrix (pole), every wave is the matrix's wave 0
+    const int mat = NP > 1 ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) : 0;
+    const int wv = NP > 1 ? 0 : __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    // the panel buffer of block parity b: NP > 1 one buffer per wave
+    auto bsel = [&](int b) __attribute__((always_inline)) { return NP > 1 ? mat : b; };
+    // the panel written by some lanes is read by others: a workgroup barrier, or for a
+    // wave alone with its panel, the completion of its LDS writes
+    auto panel_sync = [&]() __attribute__((always_inline)) {
+        if constexpr (NP > 1) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_s_waitcnt(0xc07f);   // lgkmcnt(0)
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        } else {
+            __syncthreads();
+        }
+    };
     const double* S = SW + d.mat_off;
     const double* H = HW + d.mat_off;
     // fp64: this group's partial sums (Dpart, Fpart); fp32: Dpart is the per-pole
@@ -207,8 +235,8 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
             for (int I = 0; I < JPc; ++I)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    rpr[b][16 * I + MF::row(lg, r)][sc] = are[I][jc][r];
-                    rpi[b][16 * I + MF::row(lg, r)][sc] = aim[I][jc][r];
+                    rpr[bsel(b)][16 * I + MF::row(lg, r)][sc] = are[I][jc][r];
+                    rpi[bsel(b)][16 * I + MF::row(lg, r)][sc] = aim[I][jc][r];
                 }
         }
 #pragma unroll
@@ -218,14 +246,14 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
             const int col = 16 * (wv + NWC * j) + li;
             if constexpr (sizeof(R) == 4) {   // lane group pq holds rows 4 pq + s in register s
                 if (lg == pq) {
-                    *reinterpret_cast<V*>(&rpr[b][col][0]) = are[JPc][j];
-                    *reinterpret_cast<V*>(&rpi[b][col][0]) = aim[JPc][j];
+                    *reinterpret_cast<V*>(&rpr[bsel(b)][col][0]) = are[JPc][j];
+                    *reinterpret_cast<V*>(&rpi[bsel(b)][col][0]) = aim[JPc][j];
                 }
             } else {                          // register pq holds row 4 pq + lg
                 const V& xr = are[JPc][j];
                 const V& xi = aim[JPc][j];
-                rpr[b][col][lg] = pq == 0 ? xr[0] : pq == 1 ? xr[1] : pq == 2 ? xr[2] : xr[3];
-                rpi[b][col][lg] = pq == 0 ? xi[0] : pq == 1 ? xi[1] : pq == 2 ? xi[2] : xi[3];
+                rpr[bsel(b)][col][lg] = pq == 0 ? xr[0] : pq == 1 ? xr[1] : pq == 2 ? xr[2] : xr[3];
+                rpi[bsel(b)][col][lg] = pq == 0 ? xi[0] : pq == 1 ? xi[1] : pq == 2 ? xi[2] : xi[3];
             }
         }
         // distinct per tile: stops the two publish branches of a block from being
@@ -235,9 +263,16 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
     };
 
     bool first = true;
-    for (int jp = gq; jp < M / 2; jp += ngroup) {
+    const int npole = M / 2;
+    // NP > 1: rounds of NP poles (wave mat takes pole NP r + mat; a wave without a pole in
+    // the last round still takes part in its barriers); else the group's poles in turn
+    const int nround = NP > 1 ? (npole + NP - 1) / NP : (npole - gq + ngroup - 1) / ngroup;
+    for (int rd = 0; rd < nround; ++rd) {
+        const int jp = NP > 1 ? NP * rd + mat : gq + ngroup * rd;
+        const bool active = NP == 1 || jp < npole;
         cx<double> z, wt, wz;
-        pole(jp, M, c, Rr, z, wt, wz);
+        pole(active ? jp : 0, M, c, Rr, z, wt, wz);
+        if (active) {
         // the order made opaque per pole: keeps the per-entry offsets and bounds
         // tests of the load from being held in registers across the sweep
         int nn = n, ln0 = lane;
@@ -283,7 +318,7 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
             const int ppend = min(4, nst - 4 * JP);
             for (int pp = 0; pp < ppend; ++pp) {
                 const int p = 4 * JP + pp, b = p & 1;
-                __syncthreads();
+                panel_sync();
                 // lane indices made opaque per block: per-lane LDS addresses and
                 // masks are recomputed here rather than held across the sweep
                 int ln = lane;
@@ -291,7 +326,7 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
                 const int li = ln & 15, lg = ln >> 4;
                 // Pi = A[P,P]^-1 on 16 lanes (four identical copies per wave)
                 const int pi = li >> 2, pj = li & 3;
-                cx<R> a = {rpr[b][4 * p + pj][pi], rpi[b][4 * p + pj][pi]};
+                cx<R> a = {rpr[bsel(b)][4 * p + pj][pi], rpi[bsel(b)][4 * p + pj][pi]};
                 pivot4<0>(a, pi, pj);
                 pivot4<1>(a, pi, pj);
                 pivot4<2>(a, pi, pj);
@@ -306,8 +341,8 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
                 R car[NT], cai[NT];
 #pragma unroll
                 for (int I = 0; I < NT; ++I) {
-                    R xr = rpr[b][16 * I + li][lg];
-                    R xi = rpi[b][16 * I + li][lg];
+                    R xr = rpr[bsel(b)][16 * I + li][lg];
+                    R xi = rpi[bsel(b)][16 * I + li][lg];
                     if (I == JP && li < 4 * pp) {
                         xr = -xr;
                         xi = -xi;
@@ -327,8 +362,8 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
                     R tr[4], ti[4];
 #pragma unroll
                     for (int s = 0; s < 4; ++s) {
-                        tr[s] = rpr[b][col][s];
-                        ti[s] = rpi[b][col][s];
+                        tr[s] = rpr[bsel(b)][col][s];
+                        ti[s] = rpi[bsel(b)][col][s];
                     }
                     if (j == JP / NWC && wv == JP % NWC) {
 #pragma unroll
@@ -367,7 +402,52 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
                 }
             }
         });
-        __syncthreads();   // the next pole's first publish rewrites buffer 0
+        panel_sync();   // the next pole's first publish rewrites buffer 0
+        }   // active
+
+        if constexpr (NP > 1) {
+            // this round's poles into the window's sums, in wave (= pole) order, each
+            // value rounded to float as a per-pole slot holds it
+            const int ei = lane & 15, eg = lane >> 4;
+            for (int q = 0; q < NP; ++q) {
+                __syncthreads();
+                // the accumulators made opaque per phase: their weighted values are then not
+                // hoisted out of the phase loop (which would hold every entry's pair at once)
+#pragma unroll
+                for (int I = 0; I < NT; ++I)
+#pragma unroll
+                    for (int j = 0; j < TJ; ++j)
+                        if (I < held(j)) asm volatile("" : "+v"(are[I][j]), "+v"(aim[I][j]));
+                if (q != mat || !active) continue;
+#pragma unroll
+                for (int I = 0; I < NT; ++I)
+#pragma unroll
+                    for (int j = 0; j < TJ; ++j) {
+                        if (I >= held(j)) continue;
+                        const int col = 16 * j + ei - m;
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int row = 16 * I + MF::row(eg, r) - m;
+                            if (row >= 0 && row <= col && col < k) {
+                                const double gr = (double)are[I][j][r], gi = (double)aim[I][j][r];
+                                const double gd = (double)(float)fma(wt.re, gr, -(wt.im * gi));
+                                const double gf = (double)(float)fma(wz.re, gr, -(wz.im * gi));
+                                const int u = row * k - ((row * (row - 1)) >> 1) + (col - row);
+                                if (rd == 0 && q == 0) {
+                                    pacc[u] = 0.0 + gd;
+                                    pacc[acc + u] = 0.0 + gf;
+                                } else {
+                                    pacc[u] += gd;
+                                    pacc[acc + u] += gf;
+                                }
+                            }
+                        }
+                        // one tile's read-modify-writes at a time (bounded live registers)
+                        asm volatile("" ::: "memory");
+                    }
+            }
+            continue;
+        }
 
         // Re(w G) and Re(w z G) on T x T: fp64 adds them to this group's partial
         // sums; fp32 stores them once, as a float pair, in this pole's own slot (no
@@ -410,6 +490,57 @@ __global__ void __launch_bounds__(64 * NWC) __attribute__((amdgpu_waves_per_eu(W
                 }
             }
         first = false;
+    }
+    if constexpr (NP > 1) {
+        // D_B, F_B (upper triangles), N_B, E_B as k_reduce forms them from the slots: the
+        // entries dealt to 1024 virtual threads (thread t of this workgroup plays t + 64 NP v,
+        // v < 1024 / (64 NP)), fma trace sums, each virtual wave's shuffle tree, then the 16
+        // wave partials in order
+        static_assert(NP == 4, "the virtual-thread mapping assumes four waves");
+        __shared__ double red[2][16];
+        __syncthreads();
+        const int nup = k * (k + 1) / 2;
+        auto row_start = [k](int r) { return r * k - (r * (r - 1)) / 2; };
+        double* Dw = wout.D + d.kk_off;
+        double* Fw = wout.F + d.kk_off;
+        const double* Qw = wout.Q + d.kk_off;
+#pragma unroll
+        for (int vq = 0; vq < 4; ++vq) {
+            double sn = 0.0, se = 0.0;
+            for (int e = threadIdx.x + 256 * vq; e < nup; e += 1024) {
+                const float b2 = 2.0f * (float)k + 1.0f;
+                int i = (int)(0.5f * (b2 - sqrtf(fmaxf(b2 * b2 - 8.0f * (float)e, 0.0f))));
+                i = min(max(i, 0), k - 1);
+                while (i > 0 && row_start(i) > e) --i;
+                while (i + 1 < k && row_start(i + 1) <= e) ++i;
+                const int j = i + (e - row_start(i));
+                const int64_t q = (int64_t)i * k + j;
+                const double dv = pacc[e] * 2.0, fv = pacc[acc + e] * 2.0;
+                Dw[q] = dv;
+                Fw[q] = fv;
+                double qv = Qw[q];
+                if (j != i) qv *= 2.0;
+                sn = fma(dv, qv, sn);
+                se = fma(fv, qv, se);
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                sn += __shfl_down(sn, o, 64);
+                se += __shfl_down(se, o, 64);
+            }
+            if (lane == 0) {
+                red[0][mat + 4 * vq] = sn;
+                red[1][mat + 4 * vq] = se;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double tn = 0.0, te = 0.0;
+            for (int q = 0; q < 16; ++q) tn += red[0][q];
+            for (int q = 0; q < 16; ++q) te += red[1][q];
+            if (wout.NB) wout.NB[w] = tn;
+            if (wout.EB) wout.EB[w] = te;
+        }
+        return;
     }
     if constexpr (sizeof(R) == 8) {
         // each lane stores the entries it accumulated (no barrier: the same lane wrote them)
@@ -608,11 +739,15 @@ constexpr int mfma_waves() {
     return acc <= 48 ? 4 : acc <= 80 ? 3 : acc <= 168 ? 2 : 1;
 }
 
-template <typename R, int NT, int NWC>
+template <typename R, int NT, int NWC, int NP = 1>
 hipError_t launch_one(const WinDesc* wins, const int32_t* ids, int nids, const double* SW, const double* HW, int M,
-                      int ngroup, const double* circ, double* Dpart, double* Fpart, int acc, hipStream_t st) {
-    const auto kern = k_pole_mfma<R, NT, NWC, mfma_waves<R, NT, NWC>()>;
-    const size_t lds = sizeof(R) == 8 && acc > 0 ? (size_t)2 * acc * sizeof(double) : 0;
+                      int ngroup, const double* circ, double* Dpart, double* Fpart, int acc, WinOut out,
+                      hipStream_t st) {
+    const auto kern = k_pole_mfma<R, NT, NWC, mfma_waves<R, NT, NWC>(), NP>;
+    // dynamic LDS: the fp64 group sums or the window-fused form's window sums (acc
+    // entries each of D and F)
+    const size_t lds = (NP > 1 || sizeof(R) == 8) && acc > 0 ? (size_t)2 * acc * sizeof(double) : 0;
+    if (NP > 1 && acc <= 0) return hipErrorInvalidValue;
     if (lds > 0) {
         static bool raised = false;   // the dynamic LDS limit, raised once per variant
         if (!raised) {
@@ -622,17 +757,18 @@ hipError_t launch_one(const WinDesc* wins, const int32_t* ids, int nids, const d
             raised = true;
         }
     }
-    const int nblk = (nids + 7) / 8 * 8 * ngroup;   // one id per (window slot, group); slots past nids exit
-    hipLaunchKernelGGL(kern, dim3(nblk), dim3(64 * NWC), lds, st, wins, ids, SW, HW, nids, M, ngroup, circ, Dpart,
-                       Fpart, lds > 0 ? acc : 0);
+    // one id per (window slot, group), slots past nids exit; the fused form: one per window
+    const int nblk = NP > 1 ? nids : (nids + 7) / 8 * 8 * ngroup;
+    hipLaunchKernelGGL(kern, dim3(nblk), dim3(64 * NWC * NP), lds, st, wins, ids, SW, HW, nids, M, ngroup, circ, Dpart,
+                       Fpart, lds > 0 ? acc : 0, out);
     return hipGetLastError();
 }
 
 // private (scratch) bytes per lane of a variant: nonzero means register spills
-template <typename R, int NT, int NWC>
+template <typename R, int NT, int NWC, int NP = 1>
 int scratch_one() {
     hipFuncAttributes a;
-    if (hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_pole_mfma<R, NT, NWC, mfma_waves<R, NT, NWC>()>)) !=
+    if (hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_pole_mfma<R, NT, NWC, mfma_waves<R, NT, NWC>(), NP>)) !=
         hipSuccess)
         return -1;
     return (int)a.localSizeBytes;
@@ -654,22 +790,31 @@ int scratch_one() {
 #else
 #define LRES_MFMA_FOR_NWC(X) X(8)
 #endif
-// nwc > 0: launch with nwc waves per (window, pole group); nwc < 0: return the
-// scratch bytes per lane of the -nwc variant (-1 if it does not exist)
+// nwc > 0: launch with nwc waves per (window, pole group); nwc == 0 (fp32, up to 8 tile
+// columns): the window-fused form (four waves per window); nwc < 0: return the scratch
+// bytes per lane of the -nwc variant (kFusedQuery: of the window-fused form; -1 if the
+// variant does not exist)
 hipError_t LRES_MFMA_NAME(LRES_MFMA_R, LRES_MFMA_NT)(int nwc, const lres::WinDesc* wins, const int32_t* ids, int nids,
                                                      const double* SW, const double* HW, int M, int ngroup,
                                                      const double* circ, double* Dpart, double* Fpart, int acc,
-                                                     hipStream_t st) {
+                                                     lres::WinOut out, hipStream_t st) {
     using R = LRES_MFMA_R;
     constexpr int NT = LRES_MFMA_NT;
+    constexpr int kFusedQuery = -100;
 #define LRES_MFMA_CASE(W)                                                                                     \
     if constexpr (W <= NT) {                                                                                  \
         if (nwc == W)                                                                                         \
-            return lres::launch_one<R, NT, W>(wins, ids, nids, SW, HW, M, ngroup, circ, Dpart, Fpart, acc, st); \
+            return lres::launch_one<R, NT, W>(wins, ids, nids, SW, HW, M, ngroup, circ, Dpart, Fpart, acc, out, \
+                                              st);                                                            \
         if (nwc == -W) return (hipError_t)lres::scratch_one<R, NT, W>();                                      \
     }
     LRES_MFMA_FOR_NWC(LRES_MFMA_CASE)
 #undef LRES_MFMA_CASE
+    if constexpr (sizeof(R) == 4 && NT <= 8) {
+        if (nwc == 0)
+            return lres::launch_one<R, NT, 1, 4>(wins, ids, nids, SW, HW, M, ngroup, circ, Dpart, Fpart, acc, out, st);
+        if (nwc == kFusedQuery) return (hipError_t)lres::scratch_one<R, NT, 1, 4>();
+    }
     return nwc < 0 ? (hipError_t)-1 : hipErrorInvalidValue;
 }
 

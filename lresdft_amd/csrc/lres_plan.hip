@@ -81,6 +81,17 @@ bool use_mfma(int /*fp32*/) {
     return true;
 }
 
+// The window-fused complex64 form (G = 0): LRES_POLE_FUSED=1 makes it the only candidate
+// where it exists, =2 adds it to the autotuner's candidates; by default it is left out,
+// as it measured slower than one wave per matrix with the per-pole slots and k_reduce at
+// every size timed (its pole-ordered sums in LDS take four serial phases per round of
+// four poles).  Both forms give the same bits.
+int fused_mode() {
+    const char* e = getenv("LRES_POLE_FUSED");
+    if (!e) return 0;
+    return std::string(e) == "1" ? 1 : std::string(e) == "2" ? 2 : 0;
+}
+
 std::vector<Shape> mfma_shapes(int nmax, int fp32) {
     std::vector<Shape> out;
     const int nt = (nmax + 15) / 16;
@@ -102,6 +113,17 @@ std::vector<Shape> mfma_shapes(int nmax, int fp32) {
         if (scratch < least_bytes) least = nwc, least_bytes = scratch;
     }
     if (out.empty() && least > 0 && least_bytes <= 256) out.push_back({nt, least, kMfma});
+    // complex64: the window-fused form (G = 0: four waves per window, one pole each, the
+    // reduction in the kernel; the same bits as one wave per matrix with k_reduce), when
+    // asked for.  Its spills sit outside the block sweep (values kept across a pole), so
+    // up to 256 bytes of them are allowed
+    if (fp32 && nt <= 8 && force <= 0 && fused_mode() != 0) {
+        const int scratch = mfma_scratch_bytes(1, nt, 0);
+        if (scratch >= 0 && scratch <= 256) {
+            if (fused_mode() == 1) out.clear();
+            out.push_back({nt, 0, kMfma});
+        }
+    }
     return out;
 }
 
@@ -123,6 +145,9 @@ std::vector<Shape> pole_shapes(int nmax, int fp32) {
 
 Shape default_pole_shape(int nmax, int fp32) {
     const auto v = pole_shapes(nmax, fp32);
+    if (fp32 && fused_mode() == 1)
+        for (const auto& s : v)
+            if (s.wide == kMfma && s.G == 0) return s;
     for (const auto& s : v)
         if ((s.wide == kMfma && s.G == 2) || (s.wide == 0 && s.T == 4)) return s;
     return v.empty() ? Shape{} : v.front();
@@ -130,9 +155,10 @@ Shape default_pole_shape(int nmax, int fp32) {
 
 hipError_t launch_pole(const Shape& sh, int fp32, const WinDesc* wins, const int32_t* ids, int nids, const double* SW,
                        const double* HW, int M, int ngroup, const double* circ, double* Dpart, double* Fpart,
-                       hipStream_t st, int acc = 0) {
+                       hipStream_t st, int acc = 0, lres::WinOut out = {}) {
     if (sh.wide == kMfma)
-        return launch_pole_mfma(fp32, sh.T, sh.G, wins, ids, nids, SW, HW, M, ngroup, circ, Dpart, Fpart, acc, st);
+        return launch_pole_mfma(fp32, sh.T, sh.G, wins, ids, nids, SW, HW, M, ngroup, circ, Dpart, Fpart, acc, out,
+                                st);
     return launch_pole_gj(sh.T, sh.G, sh.wide, fp32, wins, ids, nids, SW, HW, M, ngroup, circ, Dpart, Fpart, st);
 }
 
@@ -330,7 +356,7 @@ int density_blocks(const lres_plan* P) {
 // The partial-sum buffer a pole kernel of shape sh writes: the per-pole slots for
 // the complex64 matrix-core kernel (allocated on first use), else the group sums.
 double* pole_partials(lres_plan* P, const Shape& sh, int fp32) {
-    if (!(fp32 && sh.wide == kMfma)) return P->d_Dpart.p;
+    if (!(fp32 && sh.wide == kMfma && sh.G > 0)) return P->d_Dpart.p;
     const size_t need = P->d_Q.n * (size_t)(P->M / 2) * 2;
     if (P->d_slots.n < need) P->d_slots.alloc(need);
     return reinterpret_cast<double*>(P->d_slots.p);
@@ -342,6 +368,7 @@ double* pole_partials(lres_plan* P, const Shape& sh, int fp32) {
 // the same additions in the same order as the read-modify-write of global memory, so
 // the same bits.  LRES_POLE_ACC=global keeps the read-modify-write.
 int pole_acc_entries(const lres_plan* P, const lres_plan::Bucket& b, const Shape& sh, int fp32) {
+    if (fp32 && sh.wide == kMfma && sh.G == 0) return b.kmax * (b.kmax + 1) / 2;   // window-fused: window sums
     if (fp32 || sh.wide != kMfma || P->M / 2 <= P->ngroup || !P->acc_lds) return 0;
     const int u = b.kmax * (b.kmax + 1) / 2;
     return (size_t)u * 16 <= 64 * 1024 ? u : 0;
@@ -365,7 +392,8 @@ void tune_bucket(lres_plan* P, lres_plan::Bucket& b, int fp32, double c, double 
         auto launch = [&] {
             hip_check(launch_pole(sh, fp32, P->d_wins.p, P->d_bucket_ids.p + b.id_off, (int)b.ids.size(), P->d_SW.p,
                                   P->d_HW.p, P->M, P->ngroup, P->d_circ.p, pole_partials(P, sh, fp32), P->d_Fpart.p,
-                                  P->st, pole_acc_entries(P, b, sh, fp32)),
+                                  P->st, pole_acc_entries(P, b, sh, fp32),
+                                  lres::WinOut{P->d_D.p, P->d_F.p, P->d_Q.p, P->d_NB.p, P->d_EB.p}),
                       "pole kernel (tuning)");
         };
         launch();
@@ -379,7 +407,8 @@ void tune_bucket(lres_plan* P, lres_plan::Bucket& b, int fp32, double c, double 
         if (P->tune_log)
             fprintf(stderr, "[lres tune] nmax=%d windows=%zu %s %s T=%d G=%d: %.4f ms/launch\n", b.nmax,
                     b.ids.size(), fp32 ? "fp32" : "fp64",
-                    sh.wide == kMfma ? "mfma" : (sh.wide ? "valu512" : "valu128"), sh.T, sh.G, ms / 2);
+                    sh.wide == kMfma ? (sh.G == 0 ? "mfma-fused" : "mfma") : (sh.wide ? "valu512" : "valu128"), sh.T,
+                    sh.G, ms / 2);
         if (ms < best) {
             best = ms;
             b.shape[fp32] = sh;
@@ -433,7 +462,7 @@ void prepare_passes(lres_plan* P, int fp32, double c, double R) {
         if (b.big && !big_on_mfma(b.nmax, fp32)) continue;
         if (!b.tuned[fp32]) tune_bucket(P, b, fp32, c, R);
         if (fp32 && b.shape[fp32].wide == kMfma)
-            for (int32_t w : b.ids) by_pole[w] = 1;
+            for (int32_t w : b.ids) by_pole[w] = b.shape[fp32].G == 0 ? 2 : 1;
     }
     // complex64 passes only (fp64 partials are always group sums)
     if (fp32 && by_pole != P->by_pole) {
@@ -472,6 +501,8 @@ void queue_pass(lres_plan* P, double* d_dens, double* d_NB, double* d_EB, double
         ++nlaunch;
     }
     const bool spread = nlaunch > 1 && !reg.empty();
+    double* NB = d_NB ? d_NB : P->d_NB.p;
+    double* EB = d_EB ? d_EB : P->d_EB.p;
     if (e0) hip_check(hipEventRecord(e0, P->st), "event");
     if (spread) side_fork(P);
     if (big.first >= 0)
@@ -492,13 +523,12 @@ void queue_pass(lres_plan* P, double* d_dens, double* d_NB, double* d_EB, double
         const hipStream_t sq = spread ? side_stream(P, used++) : P->st;
         hip_check(launch_pole(b.shape[fp32], fp32, P->d_wins.p, P->d_bucket_ids.p + b.id_off, (int)b.ids.size(),
                               P->d_SW.p, P->d_HW.p, P->M, P->ngroup, circ, pole_partials(P, b.shape[fp32], fp32),
-                              P->d_Fpart.p, sq, pole_acc_entries(P, b, b.shape[fp32], fp32)),
+                              P->d_Fpart.p, sq, pole_acc_entries(P, b, b.shape[fp32], fp32),
+                              lres::WinOut{P->d_D.p, P->d_F.p, P->d_Q.p, NB, EB}),
                   "pole kernel");
     }
     if (spread) side_join(P, used);
     if (e1) hip_check(hipEventRecord(e1, P->st), "event");
-    double* NB = d_NB ? d_NB : P->d_NB.p;
-    double* EB = d_EB ? d_EB : P->d_EB.p;
     hip_check(launch_reduce(P->d_wins.p, (int)P->nwin, P->ngroup, P->d_Dpart.p, P->d_Fpart.p, P->d_slots.p, P->M / 2,
                             fp32 ? P->d_by_pole.p : nullptr, P->d_Q.p, P->d_D.p, P->d_F.p, NB, EB, circ, P->st),
               "reduce kernel");

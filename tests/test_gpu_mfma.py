@@ -210,3 +210,67 @@ def test_mfma_inertia_counts_match_valu_and_dense(sys3):
     assert np.array_equal(got_m, ref), (got_m, ref)
     assert np.array_equal(got_v, ref), (got_v, ref)
     assert abs(lm_m - lo) < 1e-10 and abs(lm_v - lo) < 1e-10
+
+
+def _pass_blocks(b, cells, H, mu, e_lo, M, Rw, fused):
+    """complex64 pass with the window-fused form forced on ("1") or left out ("0"): density,
+    N_B, E_B, the sums, D_B and F_B blocks and the bucket shapes."""
+    from lresdft_amd.lcao3d import LRFTLCAOCpp, csr64
+    old = os.environ.get("LRES_POLE_FUSED")
+    os.environ["LRES_POLE_FUSED"] = fused
+    try:
+        C = LRFTLCAOCpp(b, cells, R_w=Rw, M=M, precision="mixed")
+        C.plan.set_h(csr64(H))
+        nsum, esum = C.plan.run(mu, e_lo, want_density=True)
+        D, F = C.plan.pass_blocks()
+        return (C.plan.dens.cpu().numpy().copy(), C.plan.NB.cpu().numpy().copy(), C.plan.EB.cpu().numpy().copy(),
+                D.cpu().numpy(), F.cpu().numpy(), nsum, esum, C.plan.buckets())
+    finally:
+        if old is None:
+            os.environ.pop("LRES_POLE_FUSED", None)
+        else:
+            os.environ["LRES_POLE_FUSED"] = old
+
+
+def _same_bits(a, g):
+    for x, y in zip(a[:5], g[:5]):
+        assert np.array_equal(x, y)
+    assert a[5] == g[5] and a[6] == g[6]
+
+
+@pytest.mark.parametrize("M", [32, 20])
+def test_window_fused_complex64_equals_slot_form(sys3, M):
+    """The complex64 window-fused form (four waves per window, poles summed in LDS, the
+    reduction in the kernel) gives the pass of one wave per matrix with per-pole slots
+    and k_reduce bit for bit: density, N_B, E_B, D_B, F_B.  M = 20: 10 poles, so the
+    last round of four has two waves without a pole."""
+    cr, b, v = sys3
+    print(f"[gpu] complex64 window-fused vs per-pole slots, n=3 R_w 8 M {M}", flush=True)
+    cells = cell_centres((3, 3, 3), 6.0)
+    H = b.H(v)
+    W, own = build_windows(b, cells, 8.0)
+    e_lo, mu = _mu_window(H, b.S, W)
+    slot = _pass_blocks(b, cells, H, mu, e_lo, M, 8.0, "0")
+    fused = _pass_blocks(b, cells, H, mu, e_lo, M, 8.0, "1")
+    assert all(row[7] == 2 and row[6] > 0 for row in slot[7]), slot[7]
+    assert all(row[7] == 2 and row[6] == 0 for row in fused[7]), fused[7]
+    _same_bits(slot, fused)
+    dp, NBo, _ = pole_window_pass(H, b.S, b.Xr, W, own, cr.dV, mu, e_lo, M, True)
+    assert np.max(np.abs(fused[0] - dp)) < 2e-5 and abs(fused[5] - NBo.sum()) < 1e-5 * cr.n_elec
+
+
+def test_window_fused_uneven_windows_equal_slot_form():
+    """Random centres (windows of 36 to 96 functions, several buckets and tile counts):
+    the window-fused form equals the per-pole-slot form bit for bit."""
+    from lresdft_amd import system
+    cr, centres = system.random_h2_box(27, 6.0 * 3, dmin=3.0, seed=7, dx=0.6, rc=0.6)
+    b = LCAOBasis(cr, **BASIS)
+    v = guess_v(cr)
+    H = b.H(v)
+    W, _ = build_windows(b, centres, 6.0)
+    print(f"[gpu] complex64 window-fused vs slots, uneven windows", flush=True)
+    e_lo, mu = _mu_window(H, b.S, W)
+    slot = _pass_blocks(b, centres, H, mu, e_lo, 32, 6.0, "0")
+    fused = _pass_blocks(b, centres, H, mu, e_lo, 32, 6.0, "1")
+    assert len(fused[7]) > 1 and all(row[6] == 0 for row in fused[7]), fused[7]
+    _same_bits(slot, fused)
