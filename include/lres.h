@@ -146,7 +146,9 @@ int lres_lda_potential(const double* d_n, const double* d_vion, const double* d_
  * nhist x ng), and d_stats[0] = sum |r|, d_stats[1 + q] = R_q . r for q < nhist (the
  * new row of DIIS's Gram matrix; rows not in use hold zeros).  nhist <= LRES_MAX_HIST.
  * Fixed-order sums; d_part is a device workspace of (LRES_MAX_HIST + 1) x LRES_MIX_PARTS
- * doubles.  Kernel launches only, on `stream`. */
+ * doubles.  d_stats may be pinned host memory (device-accessible): the totals are then
+ * readable on the host once the stream has been waited for.  Kernel launches only, on
+ * `stream`. */
 #define LRES_MAX_HIST 8
 int lres_scf_residual(const double* d_nout, double scale, const double* d_nin, int64_t ng, int nhist, int slot,
                       double* d_Nh, double* d_Rh, double* d_part, double* d_stats, void* stream);

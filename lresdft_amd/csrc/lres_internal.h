@@ -58,11 +58,21 @@ hipError_t launch_reduce(const lres::WinDesc* wins, int nwin, int ngroup, const 
                          double* NB, double* EB, const double* circ, hipStream_t st);
 // lower triangles of D_B (and F_B if F != NULL) from the upper ones the reduction writes
 hipError_t launch_mirror(const lres::WinDesc* wins, int nwin, double* D, double* F, hipStream_t st);
-// blocks6: every owned row of X is whole atom blocks (6 consecutive T-local columns
-// each, ascending), checked on the host; then one lane per point
+// The owned rows of X restricted to T_B (CSR over owned points, T-local int32 columns)
+// and, when every row is whole 6-function atom blocks, the same rows in 64-point
+// chunks, block-interleaved (chk_off null: no chunked form; the row-per-wave kernel).
+struct DensRows {
+    const int64_t* own_g;
+    const int64_t* xptr;
+    const int32_t* xidx;
+    const double* xval;
+    const int64_t* chk_off;
+    const int64_t* chk_bs;
+    const int32_t* chk_idx;
+    const double* chk_val;
+};
 // skip (may be null): the density kernel leaves at once when *skip != 0
-hipError_t launch_density(const lres::WinDesc* wins, int nwin, int kmax, const double* D, const int64_t* own_g,
-                          const int64_t* xptr, const int32_t* xidx, const double* xval, double* dens, int blocks6,
+hipError_t launch_density(const lres::WinDesc* wins, int nwin, int kmax, const double* D, DensRows rows, double* dens,
                           const double* skip, hipStream_t st);
 hipError_t launch_sum2(const double* a, const double* b, int n, double* out, const double* skip, hipStream_t st);
 hipError_t launch_voronoi_owner(const double* cells, int ncell, const double* box, const int64_t* shape, double dx,

@@ -517,23 +517,27 @@ __global__ void __launch_bounds__(64 * kDensWaves) k_density(const WinDesc* __re
 
 // n(g) = x_g^T D_B x_g when every owned row of X is made of whole atom blocks: a
 // multiple of 6 entries, each 6 in consecutive T-local columns (the 6 functions of
-// one atom share its support), blocks in ascending column order.  One lane per
-// owned point sums x_A^T D[A, B] x_B over the atom blocks A <= B of its row (pairs
-// A < B doubled: D_B is symmetric), reading the 6 x 6 blocks of D_B from LDS (or
-// global memory when D_B does not fit), so only the entries of D_B the point
-// touches are read, all of them on or above the diagonal; each lane adds in a
-// fixed order (bitwise reproducible).
+// one atom share its support), blocks in ascending column order.  The rows come in
+// 64-point chunks, block-interleaved: block slot A of a chunk holds the 64 points'
+// column indices, then their 6 x 64 values, so every load of a wave is 64 consecutive
+// words (one pass over the rows, whole cache lines); a chunk's rows are padded to its
+// longest with zero blocks.  One wave per chunk, one lane per point: the sum of
+// x_A^T D[A, B] x_B over the blocks A <= B of its row (pairs A < B doubled: D_B is
+// symmetric), D_B's 6 x 6 blocks read from LDS (or global memory when D_B does not
+// fit), only entries on or above the diagonal; each lane adds in a fixed order, and
+// the padding adds exact zeros after a row's own terms (bitwise reproducible).
 constexpr int kDensBlkThreads = 512;
 
 template <bool STAGE>
-__global__ void __launch_bounds__(kDensBlkThreads) k_density_blk(const WinDesc* __restrict__ wins,
-                                                                 const double* __restrict__ D,
-                                                                 const int64_t* __restrict__ own_g,
-                                                                 const int64_t* __restrict__ xptr,
-                                                                 const int32_t* __restrict__ xidx,
-                                                                 const double* __restrict__ xval,
-                                                                 double* __restrict__ dens,
-                                                                 const double* __restrict__ skip) {
+__global__ void __launch_bounds__(kDensBlkThreads) k_density_chunk(const WinDesc* __restrict__ wins,
+                                                                   const double* __restrict__ D,
+                                                                   const int64_t* __restrict__ own_g,
+                                                                   const int64_t* __restrict__ chk_off,
+                                                                   const int64_t* __restrict__ chk_bs,
+                                                                   const int32_t* __restrict__ chk_idx,
+                                                                   const double* __restrict__ chk_val,
+                                                                   double* __restrict__ dens,
+                                                                   const double* __restrict__ skip) {
     extern __shared__ double Dl[];
     if (skip != nullptr && *skip != 0.0) return;
     const int w = blockIdx.x;
@@ -560,28 +564,31 @@ __global__ void __launch_bounds__(kDensBlkThreads) k_density_blk(const WinDesc* 
         else
             return Dw[(int64_t)a * k + b];
     };
-    for (int p = threadIdx.x; p < d.nown; p += blockDim.x) {
-        const int64_t o = d.own_off + p;
-        const int64_t r0 = xptr[o];
-        const int nb = (int)((xptr[o + 1] - r0) / 6);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), nwave = blockDim.x >> 6;
+    const int64_t c0 = chk_off[w], c1 = chk_off[w + 1];
+    for (int64_t c = c0 + wave; c < c1; c += nwave) {
+        const int64_t b0 = chk_bs[c];
+        const int nb = (int)(chk_bs[c + 1] - b0);
+        const int p = (int)(c - c0) * 64 + lane;
         double acc = 0.0;
         if (nb > 0) {
-            // each block's column index and values are loaded one step ahead of
-            // their use (the next block's index clamped to the last block), so a
-            // load is in flight while the previous block's terms are summed
-            const int32_t* xi = xidx + r0;
-            const double* xv = xval + r0;
+            // block A of this lane's row: column xi[64 A], values xv[64 (6 A + i)]; each
+            // block is loaded one step ahead of its use (the next block clamped to the
+            // last), so a load is in flight while the previous block's terms are summed
+            const int32_t* xi = chk_idx + b0 * 64 + lane;
+            const double* xv = chk_val + b0 * 6 * 64 + lane;
             int sa = xi[0];
             double xa[6];
 #pragma unroll
-            for (int i = 0; i < 6; ++i) xa[i] = xv[i];
+            for (int i = 0; i < 6; ++i) xa[i] = xv[64 * i];
 #pragma unroll 1
             for (int A = 0; A < nb; ++A) {
                 const int An = min(A + 1, nb - 1);
-                const int sn = xi[6 * An];
+                const int sn = xi[64 * An];
                 double xn[6];
 #pragma unroll
-                for (int i = 0; i < 6; ++i) xn[i] = xv[6 * An + i];
+                for (int i = 0; i < 6; ++i) xn[i] = xv[64 * (6 * An + i)];
                 // diagonal block, every entry read from the upper triangle
                 double t = 0.0;
 #pragma unroll
@@ -599,10 +606,10 @@ __global__ void __launch_bounds__(kDensBlkThreads) k_density_blk(const WinDesc* 
 #pragma unroll 1
                 for (int B = A + 1; B < nb; ++B) {
                     const int Bn = min(B + 1, nb - 1);
-                    const int sbn = xi[6 * Bn];
+                    const int sbn = xi[64 * Bn];
                     double xbn[6];
 #pragma unroll
-                    for (int j = 0; j < 6; ++j) xbn[j] = xv[6 * Bn + j];
+                    for (int j = 0; j < 6; ++j) xbn[j] = xv[64 * (6 * Bn + j)];
                     double u = 0.0;
 #pragma unroll
                     for (int i = 0; i < 6; ++i) {
@@ -621,7 +628,7 @@ __global__ void __launch_bounds__(kDensBlkThreads) k_density_blk(const WinDesc* 
                 for (int i = 0; i < 6; ++i) xa[i] = xn[i];
             }
         }
-        dens[own_g[o]] = acc;
+        if (p < d.nown) dens[own_g[d.own_off + p]] = acc;
     }
 }
 
@@ -766,6 +773,7 @@ __global__ void __launch_bounds__(kMixThreads) k_rows_total(const double* __rest
         const double t = block_sum(s, red);
         if (threadIdx.x == 0) out[q] = t;
     }
+    if (threadIdx.x == 0) __threadfence_system();   // out may be host memory read after the stream's wait
 }
 
 // sum of nwin values in a fixed order into out[0] (one block)
@@ -1128,28 +1136,27 @@ hipError_t launch_mirror(const WinDesc* wins, int nwin, double* D, double* F, hi
     return hipGetLastError();
 }
 
-hipError_t launch_density(const WinDesc* wins, int nwin, int kmax, const double* D, const int64_t* own_g,
-                          const int64_t* xptr, const int32_t* xidx, const double* xval, double* dens, int blocks6,
+hipError_t launch_density(const WinDesc* wins, int nwin, int kmax, const double* D, DensRows rows, double* dens,
                           const double* skip, hipStream_t st) {
     const size_t lds = (size_t)kmax * kmax * sizeof(double);
     // D_B in LDS within the 64 KB default of dynamic LDS
     const int stage = lds <= 56 * 1024 ? 1 : 0;
-    if (blocks6) {
+    if (rows.chk_off) {
         if (stage)
-            hipLaunchKernelGGL(k_density_blk<true>, dim3(nwin), dim3(kDensBlkThreads), lds, st, wins, D, own_g, xptr,
-                               xidx, xval, dens, skip);
+            hipLaunchKernelGGL(k_density_chunk<true>, dim3(nwin), dim3(kDensBlkThreads), lds, st, wins, D, rows.own_g,
+                               rows.chk_off, rows.chk_bs, rows.chk_idx, rows.chk_val, dens, skip);
         else
-            hipLaunchKernelGGL(k_density_blk<false>, dim3(nwin), dim3(kDensBlkThreads), 0, st, wins, D, own_g, xptr,
-                               xidx, xval, dens, skip);
+            hipLaunchKernelGGL(k_density_chunk<false>, dim3(nwin), dim3(kDensBlkThreads), 0, st, wins, D, rows.own_g,
+                               rows.chk_off, rows.chk_bs, rows.chk_idx, rows.chk_val, dens, skip);
         return hipGetLastError();
     }
     hipLaunchKernelGGL(k_mirror, dim3(nwin), dim3(256), 0, st, wins, const_cast<double*>(D), nullptr, skip);
     if (stage)
-        hipLaunchKernelGGL(k_density<true>, dim3(nwin), dim3(64 * kDensWaves), lds, st, wins, D, own_g, xptr, xidx,
-                           xval, dens, skip);
+        hipLaunchKernelGGL(k_density<true>, dim3(nwin), dim3(64 * kDensWaves), lds, st, wins, D, rows.own_g, rows.xptr,
+                           rows.xidx, rows.xval, dens, skip);
     else
-        hipLaunchKernelGGL(k_density<false>, dim3(nwin), dim3(64 * kDensWaves), 0, st, wins, D, own_g, xptr, xidx,
-                           xval, dens, skip);
+        hipLaunchKernelGGL(k_density<false>, dim3(nwin), dim3(64 * kDensWaves), 0, st, wins, D, rows.own_g, rows.xptr,
+                           rows.xidx, rows.xval, dens, skip);
     return hipGetLastError();
 }
 

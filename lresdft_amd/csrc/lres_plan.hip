@@ -211,6 +211,13 @@ struct lres_plan {
     std::vector<int8_t> by_pole;
     DevBuf<int64_t> d_own_g, d_xptr;
     int dens_blocks6 = 0;                        // owned rows of X are whole 6-function atom blocks
+    // the same rows for the density kernel in 64-point chunks, block-interleaved: per
+    // window its first chunk (d_chk_off), per chunk its first block slot (d_chk_bs), per
+    // block slot 64 lanes of the block's T-local column (d_chk_idx) and 6 x 64 values
+    // (d_chk_val); rows shorter than the chunk's longest are padded with zero blocks
+    DevBuf<int64_t> d_chk_off, d_chk_bs;
+    DevBuf<int32_t> d_chk_idx;
+    DevBuf<double> d_chk_val;
     DevBuf<int32_t> d_xidx;
     DevBuf<double> d_xval;
     DevBuf<int64_t> d_Hp, d_Hi;
@@ -346,11 +353,14 @@ bool big_on_mfma(int nmax, int fp32) {
     return fp32 && nmax <= 256 && use_mfma(1) && !mfma_shapes(nmax, 1).empty();
 }
 
-// Density kernel form: one lane per point over atom blocks when the owned rows of X
-// allow it, unless LRES_DENSITY_KERNEL=rows (the row-per-wave kernel, any row shape)
-int density_blocks(const lres_plan* P) {
+// Density kernel input: the owned rows of X, with their chunked form (one lane per point
+// over atom blocks) when the rows allow it, unless LRES_DENSITY_KERNEL=rows (the
+// row-per-wave kernel, any row shape)
+DensRows density_rows(const lres_plan* P) {
     const char* e = std::getenv("LRES_DENSITY_KERNEL");
-    return P->dens_blocks6 && !(e && std::strcmp(e, "rows") == 0);
+    const bool chunks = P->dens_blocks6 && !(e && std::strcmp(e, "rows") == 0);
+    return DensRows{P->d_own_g.p, P->d_xptr.p, P->d_xidx.p, P->d_xval.p, chunks ? P->d_chk_off.p : nullptr,
+                    P->d_chk_bs.p, P->d_chk_idx.p, P->d_chk_val.p};
 }
 
 // The partial-sum buffer a pole kernel of shape sh writes: the per-pole slots for
@@ -533,8 +543,8 @@ void queue_pass(lres_plan* P, double* d_dens, double* d_NB, double* d_EB, double
                             fp32 ? P->d_by_pole.p : nullptr, P->d_Q.p, P->d_D.p, P->d_F.p, NB, EB, circ, P->st),
               "reduce kernel");
     if (d_dens)
-        hip_check(launch_density(P->d_wins.p, (int)P->nwin, P->kmax, P->d_D.p, P->d_own_g.p, P->d_xptr.p, P->d_xidx.p,
-                                 P->d_xval.p, d_dens, density_blocks(P), circ + 3, P->st),
+        hip_check(launch_density(P->d_wins.p, (int)P->nwin, P->kmax, P->d_D.p, density_rows(P), d_dens, circ + 3,
+                                 P->st),
                   "density kernel");
     if (step_stage >= 0)
         hip_check(launch_sum_step(NB, EB, (int)P->nwin, sums, P->d_step.p, step_stage, P->d_circ.p,
@@ -1026,6 +1036,51 @@ int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, 
                 if (q > r0 && xidx[q] < xidx[q - 6] + 6) P->dens_blocks6 = 0;   // ascending blocks
             }
         }
+        if (P->dens_blocks6) {
+            // the chunked form of the owned rows (lane-contiguous loads in the density
+            // kernel): padding blocks repeat the row's last column with zero values, so
+            // they add exact zeros after the row's own terms
+            std::vector<int64_t> chk_off(nwin + 1, 0), bs(1, 0);
+            std::vector<int32_t> cidx;
+            std::vector<double> cval;
+            for (int64_t w = 0; w < nwin; ++w) {
+                const WinDesc& dw = P->wins[w];
+                chk_off[w] = (int64_t)bs.size() - 1;
+                for (int c0 = 0; c0 < dw.nown; c0 += 64) {
+                    const int np = std::min(64, dw.nown - c0);
+                    int nbc = 0;
+                    for (int l = 0; l < np; ++l) {
+                        const int64_t o = dw.own_off + c0 + l;
+                        nbc = std::max(nbc, (int)((xptr[o + 1] - xptr[o]) / 6));
+                    }
+                    const int64_t base = bs.back();
+                    cidx.resize((size_t)(base + nbc) * 64, 0);
+                    cval.resize((size_t)(base + nbc) * 6 * 64, 0.0);
+                    for (int l = 0; l < np; ++l) {
+                        const int64_t o = dw.own_off + c0 + l, r0 = xptr[o];
+                        const int nb = (int)((xptr[o + 1] - r0) / 6);
+                        for (int A = 0; A < nbc; ++A) {
+                            const bool real = A < nb;
+                            const int64_t q = r0 + 6 * (real ? A : std::max(nb - 1, 0));
+                            cidx[(size_t)(base + A) * 64 + l] = nb > 0 ? xidx[q] : 0;
+                            for (int i = 0; i < 6; ++i)
+                                cval[((size_t)(base + A) * 6 + i) * 64 + l] = real ? xval[q + i] : 0.0;
+                        }
+                    }
+                    bs.push_back(base + nbc);
+                }
+            }
+            chk_off[nwin] = (int64_t)bs.size() - 1;
+            if (cidx.empty()) {   // no owned point at all: keep the buffers non-empty
+                cidx.push_back(0);
+                cval.push_back(0.0);
+            }
+            P->d_chk_off.upload(chk_off.data(), chk_off.size(), st);
+            P->d_chk_bs.upload(bs.data(), bs.size(), st);
+            P->d_chk_idx.upload(cidx.data(), cidx.size(), st);
+            P->d_chk_val.upload(cval.data(), cval.size(), st);
+            hip_check(hipStreamSynchronize(st), "sync");   // the host vectors go out of scope
+        }
         P->d_own_g.upload(own_g.data(), own_g.size(), st);
         P->d_xptr.upload(xptr.data(), xptr.size(), st);
         P->d_xidx.upload(xidx.data(), std::max<size_t>(xidx.size(), 0), st);
@@ -1392,8 +1447,8 @@ int lres_density_last(lres_plan* P, double* d_dens) {
         require(P != nullptr, "null plan");
         require(d_dens != nullptr, "null density buffer");
         require(!P->step_queued, "a queued density step is pending: call lres_step_finish first");
-        hip_check(launch_density(P->d_wins.p, (int)P->nwin, P->kmax, P->d_D.p, P->d_own_g.p, P->d_xptr.p, P->d_xidx.p,
-                                 P->d_xval.p, d_dens, density_blocks(P), nullptr, P->st),
+        hip_check(launch_density(P->d_wins.p, (int)P->nwin, P->kmax, P->d_D.p, density_rows(P), d_dens, nullptr,
+                                 P->st),
                   "density kernel");
     });
 }

@@ -8,8 +8,10 @@ arrays stay on the device (torch); only DIIS's (k+1) x (k+1) system and scalars
 go to the host.  Potentials follow lrft/three_d.py:102-148."""
 from __future__ import annotations
 
+import gc
 import math
 import time
+import weakref
 
 import numpy as np
 
@@ -193,7 +195,12 @@ class PotentialStep:
 
     def __init__(self, cr, grid, solver, vion, Nh, Rh, mix, graph=True):
         torch = grid.torch
-        self.torch, self.cr, self.grid, self.solver, self.vion = torch, cr, grid, solver, vion
+        # the solver keeps this step (solver._scf_step): a weak reference back, so that a
+        # dropped solver and its captured graph are freed at once, not by a later cyclic
+        # collection (which may run while another graph is being captured, where
+        # destroying a graph is not permitted)
+        self.torch, self.cr, self.grid, self.vion = torch, cr, grid, vion
+        self._solver = weakref.ref(solver)
         self.Nh, self.Rh, self.mix = Nh, Rh, float(mix)
         self.use_graph = bool(graph)
         self.key = (id(grid), id(vion), vion.data_ptr(), Nh.shape[0], self.mix, self.use_graph)
@@ -205,6 +212,10 @@ class PotentialStep:
         self.v = None
         self.graph = None
         self.replays = 0
+
+    @property
+    def solver(self):
+        return self._solver()
 
     def _body(self):
         torch = self.torch
@@ -222,6 +233,8 @@ class PotentialStep:
         home = plan.stream
         side = torch.cuda.Stream()
         torch.cuda.synchronize()
+        # garbage holding other graphs is collected before the capture, not during it
+        gc.collect()
         g = torch.cuda.CUDAGraph()
         plan.set_stream(side)
         try:
@@ -344,32 +357,32 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
             # 2 n_out - n_in, stores the history row and the Gram row
             din = dens.reshape(-1).contiguous()
 
-            def residual(nraw):
+            def residual(nraw, stats_out):
                 nat.check(nat.lib().lres_scf_residual(
                     nraw.data_ptr(), 2.0, din.data_ptr(), cr.ng, ndiis, slot, Nh.data_ptr(), Rh.data_ptr(),
-                    grid._workspace().data_ptr(), stats_dev.data_ptr(),
+                    grid._workspace().data_ptr(), stats_out.data_ptr(),
                     nat.C.c_void_p(torch.cuda.current_stream().cuda_stream)))
 
             out = None
             queued = queue_ok and solver.density_queue(v, potential_set=preset)
             if queued:
-                # the warm step and the residual of its density queued together, one wait
-                residual(solver.plan.dens)
-                stats_pin.copy_(stats_dev, non_blocking=True)
+                # the warm step and the residual of its density queued together, one wait;
+                # the residual's totals go straight to pinned host memory
+                residual(solver.plan.dens, stats_pin)
                 out = solver.density_collect()
                 if out is not None:
                     nraw, eband, mu = out
                     if solver.step_batches > 1:
                         # the search took a further batch: the residual above ran
                         # before this density was formed
-                        residual(nraw)
+                        residual(nraw, stats_dev)
                         stats = stats_dev.cpu().numpy()
                     else:
                         stats = stats_pin.numpy().copy()
             if out is None:
                 # first step, new filter edge or stale mu: the host-driven paths
                 nraw, eband, mu = solver.density_device(v, potential_set=preset or queued, spin_scaled=False)
-                residual(nraw)
+                residual(nraw, stats_dev)
                 stats = stats_dev.cpu().numpy()
             t_solver += time.perf_counter() - ts
             nout = None

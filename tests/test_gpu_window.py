@@ -65,8 +65,8 @@ def test_fixed_mu_pass_matches_oracle(sys3, M):
 
 
 def test_density_kernels_agree(sys3):
-    """The lane-per-point atom-block density kernel (the default for this basis,
-    whose rows are whole 6-function atom blocks) and the row-per-wave kernel
+    """The lane-per-point atom-block density kernel on 64-point chunks (the default for
+    this basis, whose rows are whole 6-function atom blocks) and the row-per-wave kernel
     (LRES_DENSITY_KERNEL=rows) both match the oracle pass to 1e-12 and each
     other to 1e-13 (different summation orders)."""
     import os
