@@ -272,6 +272,16 @@ int lres_windows(const int64_t* h_Hp, const int64_t* h_Hi, const double* h_Hv,
  * reset != 0 zeroes the counters after reading. */
 int lres_plan_stats(lres_plan* plan, double* h_out4, int reset);
 
+/* Keep-edge bound statistics (no counterpart in the reference: a diagnostic of how the
+ * keep-edge test of lcao3d.py:420 was decided): h_out2 = {warm steps whose test was
+ * checked against the bound, of them decided by the bound without inertia counts}.
+ * The bound: with S = X^T X dV and H(v) = T + X^T diag(v) X dV, every window eigenvalue
+ * moves by at most max |v - v_ref| from its value at the potential v_ref of the last
+ * lres_lambda_min, so the test's outcome is proved when that bracket, widened by the
+ * potential change, lies inside (e_lo + margin / 3, e_lo + 2 margin).  Cumulative over
+ * the plan's life; LRES_EDGE_BOUND=0 in the environment switches the bound off. */
+int lres_plan_edge_stats(lres_plan* plan, double* h_out2);
+
 /* Timing of the last lres_window_pass, in milliseconds, measured with HIP events
  * on the plan's stream: {gather (set_hamiltonian), pole solves, reduction}. */
 int lres_last_timing(const lres_plan* plan, double* h_ms3);

@@ -66,6 +66,7 @@ SIGNATURES = {
                                C.c_double, C.c_double, C.c_int, _f64p, _f64p, _f64p]),
     "lres_last_timing": (C.c_int, [_vp, _f64p]),
     "lres_plan_stats": (C.c_int, [_vp, _f64p, C.c_int]),
+    "lres_plan_edge_stats": (C.c_int, [_vp, _f64p]),
     "lres_plan_layout": (C.c_int, [_vp, _i64p, _i64p]),
     "lres_set_window_blocks": (C.c_int, [_vp, _vp, _vp]),
     "lres_get_window_blocks": (C.c_int, [_vp, _vp, _vp]),

@@ -45,6 +45,60 @@ hipError_t launch_mu_begin(lres::StepState* st, const unsigned long long* itot, 
                            double slope, double keep_tol, double ftol, double xtol, int lag, double* circ,
                            hipStream_t stream);
 hipError_t launch_mu_step(lres::StepState* st, int stage, double* circ, void* pin, hipStream_t stream);
+// Keep-edge bound (lres_kernels.hip): edge = {lo, hi, delta, skipped, checked}.
+// launch_edge_delta copies v to vcur and sets delta = max |v - vref| (kEdgeBlocks
+// partial maxima in part); launch_edge_ref starts a reference bracket [lo, hi] (delta
+// 0); launch_edge_check decides the keep-edge test at e_lo from the bound when it can,
+// writing the inertia totals to itot and *skip = 1 (the inertia kernels then leave at
+// once), else *skip = 0.
+constexpr int kEdgeBlocks = 256;
+// lambda_min on the device (lres_kernels.hip): the search state, read by the count
+// launches of a round through &ncand ({ncand, K, done}); kLminProbe bracket probes below
+// the Rayleigh bound, about kLminWaves (window, shift) factorisations per round (one
+// wave each: about two per SIMD, one generation of waves, so a round costs about one
+// factorisation's latency), at least kLminMinK and at most kLminMaxK shifts; the host
+// search uses the same rule.
+struct LminState {
+    double lo, hi, tol;
+    double cell;                // the dyadic cell [cell, cell + width) holding lambda_min (lmin_snap)
+    int ncand, K, done, fail;   // fail: 1 no count at the Rayleigh bound, 2 no probe counts zero
+};
+// The result does not depend on the path of the search: the final bracket [lo, hi)
+// (hi - lo <= tol) is replaced by the cell of the dyadic grid of the given width (the
+// smallest power of two >= tol) that holds lambda_min, which needs the count at the
+// grid point g inside the bracket when there is one.  lmin_grid: g, or NAN when the
+// bracket lies in one cell (*cell is then set); exact arithmetic (powers of two).
+__host__ __device__ inline double lmin_grid(double lo, double hi, double width, double* cell) {
+    const double g = ceil(lo / width) * width;
+    if (lo == g) {
+        *cell = g;
+        return NAN;
+    }
+    if (hi <= g) {
+        *cell = g - width;
+        return NAN;
+    }
+    return g;
+}
+constexpr int kLminProbe = 2, kLminWaves = 2048, kLminMinK = 3, kLminMaxK = 255;
+hipError_t launch_lmin_probe(const double* rq, int nwin, LminState* s, double tol, double* sig,
+                             unsigned long long* itot, hipStream_t st);
+hipError_t launch_lmin_bracket(LminState* s, double* sig, unsigned long long* itot, const int32_t* ids, int nwin,
+                               int32_t* cand, hipStream_t st);
+hipError_t launch_lmin_round(LminState* s, double* sig, unsigned long long* itot, const int32_t* counts, int kcap,
+                             int32_t* cand, int32_t* tmp, hipStream_t st);
+// the snap of a finished search: launch_lmin_snap sets sig[0] = g and *skip = 0 when the
+// count at g is needed (else *skip = 1 and the cell is set), launch_lmin_snap_end takes
+// the cell from that count (itot[0])
+hipError_t launch_lmin_snap(LminState* s, double width, double* sig, unsigned long long* itot, int* skip,
+                            hipStream_t st);
+hipError_t launch_lmin_snap_end(LminState* s, double width, const double* sig, const unsigned long long* itot,
+                                const int* skip, hipStream_t st);
+hipError_t launch_edge_delta(const double* v, double* vcur, const double* vref, int64_t ng, double* part,
+                             double* edge, hipStream_t st);
+hipError_t launch_edge_ref(double* edge, double lo, double hi, hipStream_t st);
+hipError_t launch_edge_check(double* edge, double e_lo, double margin, double safety, unsigned long long* itot,
+                             int* skip, hipStream_t st);
 hipError_t launch_sum_step(const double* a, const double* b, int n, double* out, lres::StepState* st, int stage,
                            double* circ, void* pin, hipStream_t stream);
 hipError_t launch_pole_gj(int T, int G, int wide, int fp32, const lres::WinDesc* wins, const int32_t* ids, int nids,
@@ -93,7 +147,7 @@ hipError_t launch_diag_ratio(const lres::WinDesc* wins, int nwin, const double* 
                              hipStream_t st);
 hipError_t launch_inertia(int T, int G, const lres::WinDesc* wins, const int32_t* ids, int nids, const double* SW,
                           const double* HW, const double* sigma, int nshift, int32_t* counts,
-                          unsigned long long* total, hipStream_t st);
+                          unsigned long long* total, const int* skip, hipStream_t st);
 hipError_t launch_pair_v(int nf_max, int npair, const int32_t* pairA, const int32_t* pairB, const int64_t* pent_off,
                          const int32_t* ent_r, const int32_t* ent_a, const int32_t* ent_b, const int64_t* xat_off,
                          const int32_t* nf_of, const double* xat, const int64_t* vp_off, const double* v, double dV,
@@ -111,7 +165,8 @@ hipError_t launch_pole_big(int fp32, int bmax, int nmax, const lres::WinDesc* wi
 // factored at once (at least nids slots, else hipErrorInvalidValue).
 hipError_t launch_inertia_big(int bmax, int nmax, const lres::WinDesc* wins, const int32_t* ids, int nids,
                               const double* SW, const double* HW, const double* sigma, int nshift, double* work,
-                              int64_t work_len, int32_t* counts, unsigned long long* total, hipStream_t st);
+                              int64_t work_len, int32_t* counts, unsigned long long* total, const int* skip,
+                              hipStream_t st);
 // Pole solves on the matrix cores for windows of up to 128 functions: nt = 16 x 16
 // tiles per dimension (2..8), nwc = waves per (window, pole group) splitting the
 // tile columns (1..nt).  One object per (precision, nt),
@@ -131,13 +186,17 @@ LRES_MFMA_PARTS(LRES_MFMA_PART_DECL)
 #define LRES_MFMA_INERTIA_DECL(nt)                                                                              \
     hipError_t lres_mfma_inertia_##nt(const lres::WinDesc* wins, const int32_t* ids, int nids, const double* SW,   \
                                       const double* HW, const double* sigma, int nshift, int32_t* counts,       \
-                                      unsigned long long* total, hipStream_t st);
+                                      unsigned long long* total, const int* skip, const int* lin, hipStream_t st);
 #define LRES_MFMA_INERTIA_PARTS(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8)
 LRES_MFMA_INERTIA_PARTS(LRES_MFMA_INERTIA_DECL)
 // inertia counts on the matrix cores for windows of 16 nt functions at most (nt in 2..8)
+// lin (may be null): a linear launch of nids workgroups over lin = {ncand, K, done}:
+// workgroup i counts window ids[i / K] at shift i % K (none once done or past ncand),
+// per-window counts at counts[w * nshift + shift]
 hipError_t launch_inertia_mfma(int nt, const lres::WinDesc* wins, const int32_t* ids, int nids, const double* SW,
                                const double* HW, const double* sigma, int nshift, int32_t* counts,
-                               unsigned long long* total, hipStream_t st);
+                               unsigned long long* total, const int* skip, hipStream_t st,
+                               const int* lin = nullptr);
 // acc > 0 (fp64): the group partials are accumulated over the group's poles in LDS, as
 // packed upper triangles of acc entries (at least k(k+1)/2 of every window launched)
 hipError_t launch_pole_mfma(int fp32, int nt, int nwc, const lres::WinDesc* wins, const int32_t* ids, int nids,

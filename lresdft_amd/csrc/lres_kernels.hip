@@ -821,6 +821,260 @@ __device__ void circle_off(double* circ) {
     circ[3] = 1.0;
 }
 
+// ---------------------------------------------------------------------------
+// Keep-edge bound.  With S = X^T X dV and H(v) = T + X^T diag(v) X dV, a potential change
+// dv moves every window pencil's eigenvalues by at most max_g |dv(g)| (for any c,
+// |c^T X^T diag(dv) X c dV| <= max |dv| c^T S c; min-max).  So with the lowest window
+// eigenvalue bracketed as [lo, hi] at a reference potential v_ref, the keep-edge test's
+// outcome at v (no eigenvalue below e_lo + m / 3, one below e_lo + 2 m) is proved when
+// lo - delta > e_lo + m / 3 and hi + delta < e_lo + 2 m, delta = max |v - v_ref|, and the
+// two inertia counts are then not needed.  edge = {lo, hi, delta, tests skipped, tests
+// checked}.
+
+// vcur = v and per-block maxima of |v - vref| (a NaN anywhere propagates)
+__global__ void __launch_bounds__(256) k_edge_delta(const double* __restrict__ v, double* __restrict__ vcur,
+                                                    const double* __restrict__ vref, int64_t ng,
+                                                    double* __restrict__ part) {
+    __shared__ double red[256];
+    double m = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < ng; i += (int64_t)gridDim.x * 256) {
+        const double x = v[i];
+        vcur[i] = x;
+        const double d = fabs(x - vref[i]);
+        if (!(d <= m)) m = d;
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            const double o = red[threadIdx.x + s];
+            if (!(o <= red[threadIdx.x])) red[threadIdx.x] = o;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256) k_edge_final(const double* __restrict__ part, int nb,
+                                                    double* __restrict__ edge) {
+    __shared__ double red[256];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < nb; i += 256)
+        if (!(part[i] <= m)) m = part[i];
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            const double o = red[threadIdx.x + s];
+            if (!(o <= red[threadIdx.x])) red[threadIdx.x] = o;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) edge[2] = red[0];
+}
+
+// a new reference: the lowest eigenvalue bracket [lo, hi] at the current potential
+__global__ void k_edge_ref(double* __restrict__ edge, double lo, double hi) {
+    edge[0] = lo;
+    edge[1] = hi;
+    edge[2] = 0.0;
+}
+
+// the keep-edge test at e_lo decided from the bound when it can be: then the inertia
+// totals it would give (0 below e_lo + m / 3, 1 below e_lo + 2 m) go to itot and *skip
+// tells the inertia kernels to leave at once; safety covers the rounding of the
+// factorisations the bound stands in for
+__global__ void k_edge_check(double* __restrict__ edge, double e_lo, double margin, double safety,
+                             unsigned long long* __restrict__ itot, int* __restrict__ skip) {
+    const double lo = edge[0] - edge[2], hi = edge[1] + edge[2];
+    const bool proved = lo > e_lo + margin / 3 + safety && hi < e_lo + 2 * margin - safety;
+    if (proved) {
+        itot[0] = 0;
+        itot[1] = 1;
+        edge[3] += 1.0;
+    }
+    edge[4] += 1.0;
+    *skip = proved ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// lambda_min on the device: the host search of lres_lambda_min (Rayleigh bound, a
+// bracket from counts at hi - 1, hi - 3, hi - 7, ..., then multisection rounds over the
+// windows that can still hold the minimum) with every decision taken by one-block
+// controller kernels between the count launches, so the whole search waits once.  The
+// arithmetic is the host's operation for operation (no contraction), so both give the
+// same bracket bit for bit.
+
+// shifts of a multisection round over ncand windows: as lres_lambda_min's
+__device__ int lmin_shifts(int ncand) {
+    const int k = kLminWaves / (ncand > 1 ? ncand : 1);
+    return k < kLminMinK ? kLminMinK : (k > kLminMaxK ? kLminMaxK : k);
+}
+
+// the next round's K shifts over [lo, hi] (totals zeroed), by the whole block; s->lo,
+// s->hi and s->ncand are set and visible to every thread
+__device__ void lmin_setup_round(LminState* s, double* sig, unsigned long long* itot) {
+#pragma clang fp contract(off)
+    const int K = lmin_shifts(s->ncand);
+    const double lo = s->lo, hi = s->hi;
+    for (int q = threadIdx.x; q < K; q += blockDim.x) {
+        sig[q] = lo + (hi - lo) * (q + 1) / (K + 1);
+        itot[q] = 0;
+    }
+    if (threadIdx.x == 0) s->K = K;
+}
+
+// the Rayleigh bound hi0 = min_w min_a H_aa / S_aa (+ 1e-12 (1 + |hi0|)) and the probe
+// shifts hi0, hi0 - 1, hi0 - 3, hi0 - 7, ... (kLminProbe below hi0), totals zeroed
+__global__ void __launch_bounds__(256) k_lmin_probe(const double* __restrict__ rq, int nwin, LminState* __restrict__ s,
+                                                    double tol, double* __restrict__ sig,
+                                                    unsigned long long* __restrict__ itot) {
+#pragma clang fp contract(off)
+    __shared__ double red[256];
+    double m = 1.0 / 0.0;
+    for (int w = threadIdx.x; w < nwin; w += 256) m = fmin(m, rq[w]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int q = 128; q > 0; q >>= 1) {
+        if (threadIdx.x < q) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + q]);
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    double hi = red[0];
+    hi += 1e-12 * (1.0 + fabs(hi));
+    s->tol = tol;
+    s->hi = hi;
+    s->lo = hi;
+    s->ncand = s->K = 0;
+    s->done = s->fail = 0;
+    sig[0] = hi;
+    itot[0] = 0;
+    double step = 1.0, lo = hi - step;
+    for (int j = 0; j < kLminProbe; ++j) {
+        sig[1 + j] = lo;
+        itot[1 + j] = 0;
+        step *= 2.0;
+        lo -= step;
+    }
+}
+
+// the bracket from the probe counts (the first probe counting nothing is lo, the one
+// above it hi), all windows as candidates and the first round's shifts
+__global__ void __launch_bounds__(1024) k_lmin_bracket(LminState* __restrict__ s, double* __restrict__ sig,
+                                                       unsigned long long* __restrict__ itot,
+                                                       const int32_t* __restrict__ ids, int nwin,
+                                                       int32_t* __restrict__ cand) {
+#pragma clang fp contract(off)
+    for (int w = threadIdx.x; w < nwin; w += 1024) cand[w] = ids[w];
+    if (threadIdx.x == 0) {
+        if (itot[0] == 0) {            // no eigenvalue below the Rayleigh bound: S_W not definite
+            s->fail = 1;
+            s->done = 1;
+        } else {
+            int j = 0;
+            while (j < kLminProbe && itot[1 + j] != 0) ++j;
+            if (j == kLminProbe) {     // the spectrum reaches further down: the host search
+                s->fail = 2;
+                s->done = 1;
+            } else {
+                s->lo = sig[1 + j];
+                s->hi = sig[j];
+                s->ncand = nwin;
+                s->done = s->hi - s->lo > s->tol ? 0 : 1;
+            }
+        }
+    }
+    __syncthreads();
+    if (!s->done) lmin_setup_round(s, sig, itot);
+}
+
+// one multisection round's outcome: the new bracket, the candidates that count an
+// eigenvalue below its upper end (while more than 64 take part), the next round's shifts
+__global__ void __launch_bounds__(1024) k_lmin_round(LminState* __restrict__ s, double* __restrict__ sig,
+                                                     unsigned long long* __restrict__ itot,
+                                                     const int32_t* __restrict__ counts, int kcap,
+                                                     int32_t* __restrict__ cand, int32_t* __restrict__ tmp) {
+#pragma clang fp contract(off)
+    __shared__ int first_s, base_s, next_s;
+    __shared__ int scan[1024];
+    if (s->done) return;
+    const int ncand = s->ncand, K = s->K;
+    if (threadIdx.x == 0) {
+        first_s = K;
+        base_s = 0;
+    }
+    __syncthreads();
+    // the first shift with a positive total
+    for (int q = threadIdx.x; q < K; q += 1024)
+        if (itot[q] != 0) atomicMin(&first_s, q);
+    __syncthreads();
+    const int fp = first_s < K ? first_s : -1;
+    if (fp >= 0 && ncand > 64) {
+        // ordered compaction of the candidates with a positive count at shift fp
+        for (int c0 = 0; c0 < ncand; c0 += 1024) {
+            const int c = c0 + threadIdx.x;
+            const int keep = c < ncand && counts[(int64_t)cand[c] * kcap + fp] > 0 ? 1 : 0;
+            scan[threadIdx.x] = keep;
+            __syncthreads();
+            for (int o = 1; o < 1024; o <<= 1) {
+                const int x = threadIdx.x >= o ? scan[threadIdx.x - o] : 0;
+                __syncthreads();
+                scan[threadIdx.x] += x;
+                __syncthreads();
+            }
+            if (keep) tmp[base_s + scan[threadIdx.x] - 1] = cand[c];
+            __syncthreads();
+            if (threadIdx.x == 0) base_s += scan[1023];
+            __syncthreads();
+        }
+        const int nn = base_s;
+        if (nn > 0) {
+            for (int c = threadIdx.x; c < nn; c += 1024) cand[c] = tmp[c];
+            if (threadIdx.x == 0) s->ncand = nn;
+        }
+    }
+    if (threadIdx.x == 0) {
+        // as the host's scan: shifts counting nothing raise lo, the first positive one is hi
+        const double lo = s->lo, hi = s->hi;
+        const double nlo = fp < 0 ? sig[K - 1] : (fp > 0 ? sig[fp - 1] : lo);
+        const double nhi = fp < 0 ? hi : sig[fp];
+        if (nlo == lo && nhi == hi) {   // no progress at double resolution
+            s->done = 1;
+        } else {
+            s->lo = nlo;
+            s->hi = nhi;
+            s->done = nhi - nlo > s->tol ? 0 : 1;
+        }
+        next_s = s->done ? 0 : 1;
+    }
+    __syncthreads();
+    if (next_s) lmin_setup_round(s, sig, itot);
+}
+
+__global__ void k_lmin_snap(LminState* __restrict__ s, double width, double* __restrict__ sig,
+                            unsigned long long* __restrict__ itot, int* __restrict__ skip) {
+    if (!s->done || s->fail) {   // not finished in this batch (or failed): nothing to snap
+        *skip = 1;
+        return;
+    }
+    double cell = 0.0;
+    const double g = lmin_grid(s->lo, s->hi, width, &cell);
+    if (isnan(g)) {
+        s->cell = cell;
+        *skip = 1;
+    } else {
+        sig[0] = g;
+        itot[0] = 0;
+        *skip = 0;
+    }
+}
+
+__global__ void k_lmin_snap_end(LminState* __restrict__ s, double width, const double* __restrict__ sig,
+                                const unsigned long long* __restrict__ itot, const int* __restrict__ skip) {
+    if (*skip != 0) return;
+    s->cell = itot[0] > 0 ? sig[0] - width : sig[0];   // an eigenvalue below g: the cell under it
+}
+
 // itot (may be null): inertia totals at (e_lo + margin / 3, e_lo + 2 margin); the
 // filter edge is kept only when the first is zero and the second positive
 // (lcao3d.py:420), else the step stops with kWarmEdge before any pass
@@ -929,8 +1183,10 @@ template <int T>
 __global__ void __launch_bounds__(1024) k_inertia(const WinDesc* __restrict__ wins, const int32_t* __restrict__ ids,
                                                   const double* __restrict__ SW, const double* __restrict__ HW,
                                                   int G, const double* __restrict__ sigma, int nshift,
-                                                  int32_t* __restrict__ counts, unsigned long long* __restrict__ total) {
+                                                  int32_t* __restrict__ counts, unsigned long long* __restrict__ total,
+                                                  const int* __restrict__ skip) {
     __shared__ double rowbuf[2][kMaxG * T];
+    if (skip != nullptr && *skip != 0) return;   // decided by the keep-edge bound
     __shared__ double colbuf[2][kMaxG * T];
     const int w = ids[blockIdx.x];
     const int s_id = blockIdx.y;
@@ -1102,9 +1358,9 @@ hipError_t launch_pole_mfma(int fp32, int nt, int nwc, const WinDesc* wins, cons
 
 hipError_t launch_inertia_mfma(int nt, const WinDesc* wins, const int32_t* ids, int nids, const double* SW,
                                const double* HW, const double* sigma, int nshift, int32_t* counts,
-                               unsigned long long* total, hipStream_t st) {
+                               unsigned long long* total, const int* skip, hipStream_t st, const int* lin) {
 #define LRES_INERTIA_CALL(NT) \
-    if (nt == NT) return lres_mfma_inertia_##NT(wins, ids, nids, SW, HW, sigma, nshift, counts, total, st);
+    if (nt == NT) return lres_mfma_inertia_##NT(wins, ids, nids, SW, HW, sigma, nshift, counts, total, skip, lin, st);
     LRES_MFMA_INERTIA_PARTS(LRES_INERTIA_CALL)
 #undef LRES_INERTIA_CALL
     return hipErrorInvalidValue;
@@ -1234,6 +1490,55 @@ hipError_t launch_set_circle(double* circ, double c, double R, double skip, doub
     return hipGetLastError();
 }
 
+hipError_t launch_lmin_probe(const double* rq, int nwin, LminState* s, double tol, double* sig,
+                             unsigned long long* itot, hipStream_t st) {
+    hipLaunchKernelGGL(k_lmin_probe, dim3(1), dim3(256), 0, st, rq, nwin, s, tol, sig, itot);
+    return hipGetLastError();
+}
+
+hipError_t launch_lmin_bracket(LminState* s, double* sig, unsigned long long* itot, const int32_t* ids, int nwin,
+                               int32_t* cand, hipStream_t st) {
+    hipLaunchKernelGGL(k_lmin_bracket, dim3(1), dim3(1024), 0, st, s, sig, itot, ids, nwin, cand);
+    return hipGetLastError();
+}
+
+hipError_t launch_lmin_round(LminState* s, double* sig, unsigned long long* itot, const int32_t* counts, int kcap,
+                             int32_t* cand, int32_t* tmp, hipStream_t st) {
+    hipLaunchKernelGGL(k_lmin_round, dim3(1), dim3(1024), 0, st, s, sig, itot, counts, kcap, cand, tmp);
+    return hipGetLastError();
+}
+
+hipError_t launch_lmin_snap(LminState* s, double width, double* sig, unsigned long long* itot, int* skip,
+                            hipStream_t st) {
+    hipLaunchKernelGGL(k_lmin_snap, dim3(1), dim3(1), 0, st, s, width, sig, itot, skip);
+    return hipGetLastError();
+}
+
+hipError_t launch_lmin_snap_end(LminState* s, double width, const double* sig, const unsigned long long* itot,
+                                const int* skip, hipStream_t st) {
+    hipLaunchKernelGGL(k_lmin_snap_end, dim3(1), dim3(1), 0, st, s, width, sig, itot, skip);
+    return hipGetLastError();
+}
+
+hipError_t launch_edge_delta(const double* v, double* vcur, const double* vref, int64_t ng, double* part,
+                             double* edge, hipStream_t st) {
+    const int nb = (int)std::min<int64_t>(kEdgeBlocks, (ng + 255) / 256);
+    hipLaunchKernelGGL(k_edge_delta, dim3(nb), dim3(256), 0, st, v, vcur, vref, ng, part);
+    hipLaunchKernelGGL(k_edge_final, dim3(1), dim3(256), 0, st, part, nb, edge);
+    return hipGetLastError();
+}
+
+hipError_t launch_edge_ref(double* edge, double lo, double hi, hipStream_t st) {
+    hipLaunchKernelGGL(k_edge_ref, dim3(1), dim3(1), 0, st, edge, lo, hi);
+    return hipGetLastError();
+}
+
+hipError_t launch_edge_check(double* edge, double e_lo, double margin, double safety, unsigned long long* itot,
+                             int* skip, hipStream_t st) {
+    hipLaunchKernelGGL(k_edge_check, dim3(1), dim3(1), 0, st, edge, e_lo, margin, safety, itot, skip);
+    return hipGetLastError();
+}
+
 hipError_t launch_mu_begin(StepState* st, const unsigned long long* itot, double N, double e_lo, double mu0,
                            double slope, double keep_tol, double ftol, double xtol, int lag, double* circ,
                            hipStream_t stream) {
@@ -1262,17 +1567,17 @@ hipError_t launch_sum2(const double* a, const double* b, int n, double* out, con
 
 hipError_t launch_inertia(int T, int G, const WinDesc* wins, const int32_t* ids, int nids, const double* SW,
                           const double* HW, const double* sigma, int nshift, int32_t* counts,
-                          unsigned long long* total, hipStream_t st) {
+                          unsigned long long* total, const int* skip, hipStream_t st) {
     if (G < 1 || G > kMaxG) return hipErrorInvalidValue;
     dim3 grid(nids, nshift), blk(G * G);
     switch (T) {
-        case 2: hipLaunchKernelGGL((k_inertia<2>), grid, blk, 0, st, wins, ids, SW, HW, G, sigma, nshift, counts, total); break;
-        case 3: hipLaunchKernelGGL((k_inertia<3>), grid, blk, 0, st, wins, ids, SW, HW, G, sigma, nshift, counts, total); break;
-        case 4: hipLaunchKernelGGL((k_inertia<4>), grid, blk, 0, st, wins, ids, SW, HW, G, sigma, nshift, counts, total); break;
-        case 5: hipLaunchKernelGGL((k_inertia<5>), grid, blk, 0, st, wins, ids, SW, HW, G, sigma, nshift, counts, total); break;
-        case 6: hipLaunchKernelGGL((k_inertia<6>), grid, blk, 0, st, wins, ids, SW, HW, G, sigma, nshift, counts, total); break;
-        case 7: hipLaunchKernelGGL((k_inertia<7>), grid, blk, 0, st, wins, ids, SW, HW, G, sigma, nshift, counts, total); break;
-        case 8: hipLaunchKernelGGL((k_inertia<8>), grid, blk, 0, st, wins, ids, SW, HW, G, sigma, nshift, counts, total); break;
+        case 2: hipLaunchKernelGGL((k_inertia<2>), grid, blk, 0, st, wins, ids, SW, HW, G, sigma, nshift, counts, total, skip); break;
+        case 3: hipLaunchKernelGGL((k_inertia<3>), grid, blk, 0, st, wins, ids, SW, HW, G, sigma, nshift, counts, total, skip); break;
+        case 4: hipLaunchKernelGGL((k_inertia<4>), grid, blk, 0, st, wins, ids, SW, HW, G, sigma, nshift, counts, total, skip); break;
+        case 5: hipLaunchKernelGGL((k_inertia<5>), grid, blk, 0, st, wins, ids, SW, HW, G, sigma, nshift, counts, total, skip); break;
+        case 6: hipLaunchKernelGGL((k_inertia<6>), grid, blk, 0, st, wins, ids, SW, HW, G, sigma, nshift, counts, total, skip); break;
+        case 7: hipLaunchKernelGGL((k_inertia<7>), grid, blk, 0, st, wins, ids, SW, HW, G, sigma, nshift, counts, total, skip); break;
+        case 8: hipLaunchKernelGGL((k_inertia<8>), grid, blk, 0, st, wins, ids, SW, HW, G, sigma, nshift, counts, total, skip); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1849,8 +2154,10 @@ __global__ void __launch_bounds__(kBigThreads) k_inertia_big(const WinDesc* __re
                                                      const double* __restrict__ sigma, int nshift, int bmax,
                                                      double* __restrict__ work, int64_t stride,
                                                      int32_t* __restrict__ counts,
-                                                     unsigned long long* __restrict__ total) {
+                                                     unsigned long long* __restrict__ total,
+                                                     const int* __restrict__ skip) {
     extern __shared__ double lds_raw[];
+    if (skip != nullptr && *skip != 0) return;   // decided by the keep-edge bound
     const int w = ids[blockIdx.x];
     const WinDesc d = wins[w];
     const int n = d.n;
@@ -1905,7 +2212,7 @@ hipError_t launch_pole_big(int fp32, int bmax, int nmax, const WinDesc* wins, co
 
 hipError_t launch_inertia_big(int bmax, int nmax, const WinDesc* wins, const int32_t* ids, int nids, const double* SW,
                               const double* HW, const double* sigma, int nshift, double* work, int64_t work_len,
-                              int32_t* counts, unsigned long long* total, hipStream_t st) {
+                              int32_t* counts, unsigned long long* total, const int* skip, hipStream_t st) {
     const size_t lds = big_lds_bytes(nmax, bmax, 8);
     const int64_t stride = (int64_t)nmax * nmax;
     const int64_t slots = work_len / stride;
@@ -1915,7 +2222,7 @@ hipError_t launch_inertia_big(int bmax, int nmax, const WinDesc* wins, const int
     const int ny = (int)std::min<int64_t>(nshift, slots / nids);
     dim3 grid(nids, ny), blk(kBigThreads);
     hipLaunchKernelGGL(k_inertia_big, grid, blk, lds, st, wins, ids, SW, HW, sigma, nshift, bmax, work, stride, counts,
-                       total);
+                       total, skip);
     return hipGetLastError();
 }
 

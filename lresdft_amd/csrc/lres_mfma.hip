@@ -597,7 +597,8 @@ template <int NT>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NT <= 6 ? 2 : 1)))
 k_inertia_mfma(const WinDesc* __restrict__ wins, const int32_t* __restrict__ ids, const double* __restrict__ SW,
                const double* __restrict__ HW, const double* __restrict__ sigma, int nshift,
-               int32_t* __restrict__ counts, unsigned long long* __restrict__ total) {
+               int32_t* __restrict__ counts, unsigned long long* __restrict__ total, const int* __restrict__ skip,
+               const int* __restrict__ lin) {
     using MF = Mfma<double>;
     using V = typename MF::V;
     constexpr int NR = 16 * NT;
@@ -605,8 +606,16 @@ k_inertia_mfma(const WinDesc* __restrict__ wins, const int32_t* __restrict__ ids
     // bytes (a 32-byte stride puts the column reads of lanes j and j + 8 on the same
     // LDS banks)
     __shared__ __attribute__((aligned(16))) double rp[2][NR][5];
-    const int w = ids[blockIdx.x];
-    const int s_id = blockIdx.y;
+    if (skip != nullptr && *skip != 0) return;   // decided by the keep-edge bound
+    int c = blockIdx.x, s_id = blockIdx.y;
+    if (lin != nullptr) {   // linear launch of a device lambda_min round: (candidate, shift)
+        if (lin[2] != 0) return;
+        const int K = lin[1];
+        c = blockIdx.x / K;
+        if (c >= lin[0]) return;
+        s_id = blockIdx.x - c * K;
+    }
+    const int w = ids[c];
     const WinDesc d = wins[w];
     const int n = d.n;
     const int lane = threadIdx.x & 63;
@@ -826,9 +835,10 @@ hipError_t LRES_MFMA_NAME(LRES_MFMA_R, LRES_MFMA_NT)(int nwc, const lres::WinDes
 hipError_t LRES_MFMA_CAT(lres_mfma_inertia_, LRES_MFMA_NT)(const lres::WinDesc* wins, const int32_t* ids, int nids,
                                                           const double* SW, const double* HW, const double* sigma,
                                                           int nshift, int32_t* counts, unsigned long long* total,
-                                                          hipStream_t st) {
-    hipLaunchKernelGGL((lres::k_inertia_mfma<LRES_MFMA_NT>), dim3(nids, nshift), dim3(64), 0, st, wins, ids, SW, HW,
-                       sigma, nshift, counts, total);
+                                                          const int* skip, const int* lin, hipStream_t st) {
+    const dim3 grid = lin != nullptr ? dim3(nids, 1) : dim3(nids, nshift);
+    hipLaunchKernelGGL((lres::k_inertia_mfma<LRES_MFMA_NT>), grid, dim3(64), 0, st, wins, ids, SW, HW, sigma, nshift,
+                       counts, total, skip, lin);
     return hipGetLastError();
 }
 #endif

@@ -225,6 +225,12 @@ struct lres_plan {
     DevBuf<unsigned long long> d_itot;
     DevBuf<int32_t> d_cand, d_icnt;              // lambda_min: candidate windows, per-window counts
     DevBuf<double> d_rq;                         // lambda_min: per-window Rayleigh bound
+    // lambda_min on the device: search state, shifts, totals, candidate list and its
+    // compaction buffer
+    DevBuf<LminState> d_lmin;
+    DevBuf<double> d_lsig;
+    DevBuf<unsigned long long> d_litot;
+    DevBuf<int32_t> d_lcand, d_ltmp;
     std::vector<int32_t> bucket_of;              // window -> bucket index
     std::vector<int64_t> fn_at;                  // per window (Woff offsets): global function at each T-last position
     DevBuf<double> d_adjG, d_adjZ;               // adjoint workspaces (complex entries as double pairs)
@@ -238,6 +244,13 @@ struct lres_plan {
     DevBuf<int64_t> d_pent_off, d_xat_off, d_vp_off;
     DevBuf<double> d_xat, d_vpair, d_TW;
     bool have_H = false;
+    // keep-edge bound (k_edge_*): the potential of the current H (vcur, copied by every
+    // lres_set_potential), the potential of the last lambda_min (vref) with its lowest
+    // eigenvalue bracket in d_edge; pot_h: the current H was assembled from a potential;
+    // edge_ok: d_edge and vref describe a lambda_min on such an H
+    DevBuf<double> d_vcur, d_vref, d_edge_part, d_edge;
+    DevBuf<int> d_edge_skip;
+    bool pot_h = false, edge_ok = false;
     // forces (lres_force_setup / lres_band_forces): the basis on per-atom boxes, the
     // atom pairs whose boxes meet with their window entries, grid-point owners
     bool have_forces = false;
@@ -344,6 +357,7 @@ void gather_h(lres_plan* P, const int64_t* Hp, const int64_t* Hi, const double* 
               "gather kernel");
     hip_check(hipEventRecord(P->ev[1], P->st), "event");
     P->have_H = true;
+    P->pot_h = P->edge_ok = false;
 }
 
 // Windows above the register-resident limit of the vector kernels: in fp32 the
@@ -594,8 +608,25 @@ bool inertia_on_mfma(int nmax) {
 
 // queue the inertia counts at ns shifts (totals into d_itot, per window into d_counts
 // if given) without waiting for them
+// The keep-edge bound replaces the two inertia counts of a warm step when it proves
+// their outcome (k_edge_check); LRES_EDGE_BOUND=0 switches it off (the counts always
+// run; same decisions, so the same results).  kEdgeSafety (Ha) keeps a proved decision
+// clear of the rounding of the factorisations it replaces.
+constexpr double kEdgeSafety = 1e-8;
+
+bool edge_bound_enabled() {
+    const char* e = getenv("LRES_EDGE_BOUND");
+    return !(e && std::string(e) == "0");
+}
+
+// edge: the two shifts of a keep-edge test at (e_lo, margin), decided first from the
+// keep-edge bound when the plan holds one (the counts then run only when it cannot)
+struct EdgeTest {
+    double e_lo, margin;
+};
+
 void inertia_launch(lres_plan* P, const double* sig, int ns, int32_t* d_counts,
-                    const std::vector<int32_t>* cand = nullptr) {
+                    const std::vector<int32_t>* cand = nullptr, const EdgeTest* edge = nullptr) {
     require(P->have_H, "no Hamiltonian loaded");
     require(ns > 0 && ns <= 1024, "nshift must be in 1..1024");
     require(ns <= lres_plan::kPinSlots, "too many shifts");
@@ -603,6 +634,14 @@ void inertia_launch(lres_plan* P, const double* sig, int ns, int32_t* d_counts,
     const double* d_sig = P->d_pin + lres_plan::kPinSlots;
     if ((int)P->d_itot.n < ns) P->d_itot.alloc(ns);
     hip_check(hipMemsetAsync(P->d_itot.p, 0, ns * sizeof(unsigned long long), P->st), "memset");
+    const int* skip = nullptr;
+    if (edge && P->edge_ok && edge_bound_enabled()) {
+        require(ns == 2, "a keep-edge test has two shifts");
+        hip_check(launch_edge_check(P->d_edge.p, edge->e_lo, edge->margin, kEdgeSafety, P->d_itot.p,
+                                    P->d_edge_skip.p, P->st),
+                  "keep-edge bound");
+        skip = P->d_edge_skip.p;
+    }
     std::vector<int32_t> sub;        // candidate ids grouped by bucket
     std::vector<int64_t> sub_off;    // per bucket: first entry in sub
     if (cand) {
@@ -641,7 +680,7 @@ void inertia_launch(lres_plan* P, const double* sig, int ns, int32_t* d_counts,
             nids += cand ? (int)(sub_off[bi + 1] - sub_off[bi]) : (int)P->buckets[bi].ids.size();
         if (nids > 0)
             hip_check(launch_inertia_big(bmax, nmax, P->d_wins.p, ids, nids, P->d_SW.p, P->d_HW.p, d_sig, ns,
-                                         P->d_work.p, (int64_t)P->d_work.n, d_counts, P->d_itot.p, P->st),
+                                         P->d_work.p, (int64_t)P->d_work.n, d_counts, P->d_itot.p, skip, P->st),
                       "blocked inertia kernel");
     }
     int used = 0;
@@ -652,18 +691,19 @@ void inertia_launch(lres_plan* P, const double* sig, int ns, int32_t* d_counts,
         if (nids == 0 || (merged && b.big)) continue;
         if (b.big) {   // blocked buckets share the workspace: they stay on the plan stream
             hip_check(launch_inertia_big(b.bmax, b.nmax, P->d_wins.p, ids, nids, P->d_SW.p, P->d_HW.p, d_sig,
-                                         ns, P->d_work.p, (int64_t)P->d_work.n, d_counts, P->d_itot.p, P->st),
+                                         ns, P->d_work.p, (int64_t)P->d_work.n, d_counts, P->d_itot.p, skip,
+                                         P->st),
                       "blocked inertia kernel");
             continue;
         }
         const hipStream_t sq = spread ? side_stream(P, used++) : P->st;
         if (inertia_on_mfma(b.nmax))
             hip_check(launch_inertia_mfma((b.nmax + 15) / 16, P->d_wins.p, ids, nids, P->d_SW.p, P->d_HW.p,
-                                          d_sig, ns, d_counts, P->d_itot.p, sq),
+                                          d_sig, ns, d_counts, P->d_itot.p, skip, sq),
                       "matrix-core inertia kernel");
         else
             hip_check(launch_inertia(b.ishape.T, b.ishape.G, P->d_wins.p, ids, nids, P->d_SW.p, P->d_HW.p,
-                                     d_sig, ns, d_counts, P->d_itot.p, sq),
+                                     d_sig, ns, d_counts, P->d_itot.p, skip, sq),
                       "inertia kernel");
     }
     if (spread) side_join(P, used);
@@ -705,6 +745,100 @@ void step_batch(lres_plan* P, int depth, double* d_dens) {
                    i + 1 < depth ? lres::kStepFeed : lres::kStepFeedFinal);
     queue_pass(P, d_dens, nullptr, nullptr, P->d_sums.p, P->ev_slot[2 * depth], P->ev_slot[2 * depth + 1],
                lres::kStepPublish);
+}
+
+}  // namespace
+
+namespace {
+
+// the dyadic cell width of a lambda_min tolerance: the smallest power of two >= tol
+double lmin_width(double tol) {
+    int e = 0;
+    const double m = std::frexp(tol, &e);   // tol = m 2^e, m in [0.5, 1)
+    return m == 0.5 ? tol : std::ldexp(1.0, e);
+}
+
+// lambda_min's result from its cell [lo, hi) (no window eigenvalue below lo, one below
+// hi): the midpoint; the cell is also the keep-edge reference at the potential of this H
+void lambda_min_finish(lres_plan* P, double lo, double hi, double* out) {
+    *out = 0.5 * (lo + hi);
+    P->edge_ok = false;
+    if (P->pot_h) {
+        hip_check(hipMemcpyAsync(P->d_vref.p, P->d_vcur.p, P->ng * sizeof(double), hipMemcpyDeviceToDevice, P->st),
+                  "copy");
+        hip_check(launch_edge_ref(P->d_edge.p, lo, hi, P->st), "keep-edge bound");
+        P->edge_ok = true;
+    }
+}
+
+// lambda_min searched on the device (k_lmin_*) when every window is in one bucket on the
+// matrix-core inertia kernel; LRES_LMIN=host keeps the host-driven search (the same
+// bracket bit for bit)
+bool lmin_on_device(const lres_plan* P) {
+    const char* e = getenv("LRES_LMIN");
+    if (e && std::string(e) == "host") return false;
+    if (P->buckets.size() != 1) return false;
+    const auto& b = P->buckets[0];
+    return !b.big && inertia_on_mfma(b.nmax) && (int64_t)b.ids.size() == P->nwin;
+}
+
+// The device search: probe and bracket, then rounds of (linear count launch, controller)
+// queued kLminRounds at a time with one wait per batch.  Returns false (nothing decided)
+// when the probes find no count-free shift: the host search then starts over.
+constexpr int kLminRounds = 12;
+
+bool lambda_min_device(lres_plan* P, double tol, double* cell_out) {
+    const int nwin = (int)P->nwin;
+    const auto& b = P->buckets[0];
+    const int nt = (b.nmax + 15) / 16;
+    if (P->d_lmin.n < 1) P->d_lmin.alloc(1);
+    if ((int)P->d_lsig.n < kLminMaxK) P->d_lsig.alloc(kLminMaxK);
+    if ((int)P->d_litot.n < kLminMaxK) P->d_litot.alloc(kLminMaxK);
+    if ((int)P->d_lcand.n < nwin) P->d_lcand.alloc(nwin);
+    if ((int)P->d_ltmp.n < nwin) P->d_ltmp.alloc(nwin);
+    if ((int64_t)P->d_icnt.n < (int64_t)nwin * kLminMaxK) P->d_icnt.alloc((size_t)nwin * kLminMaxK);
+    if ((int64_t)P->d_rq.n < nwin) P->d_rq.alloc(nwin);
+    if (P->d_edge_skip.n < 1) P->d_edge_skip.alloc(1);
+    const double width = lmin_width(tol);
+    LminState* st = P->d_lmin.p;
+    const int32_t* ids = P->d_bucket_ids.p + b.id_off;
+    hip_check(launch_diag_ratio(P->d_wins.p, nwin, P->d_SW.p, P->d_HW.p, P->d_rq.p, P->st), "diag kernel");
+    hip_check(launch_lmin_probe(P->d_rq.p, nwin, st, tol, P->d_lsig.p, P->d_litot.p, P->st), "lambda_min controller");
+    hip_check(launch_inertia_mfma(nt, P->d_wins.p, ids, nwin, P->d_SW.p, P->d_HW.p, P->d_lsig.p, 1 + kLminProbe,
+                                  nullptr, P->d_litot.p, nullptr, P->st),
+              "matrix-core inertia kernel");
+    hip_check(launch_lmin_bracket(st, P->d_lsig.p, P->d_litot.p, ids, nwin, P->d_lcand.p, P->st),
+              "lambda_min controller");
+    // every round has ncand K <= max(kLminWaves, kLminMinK nwin) (window, shift) pairs
+    const int G = std::max(kLminWaves, kLminMinK * nwin);
+    const int* lin = &st->ncand;
+    for (int batch = 0;; ++batch) {
+        for (int r = 0; r < kLminRounds; ++r) {
+            hip_check(launch_inertia_mfma(nt, P->d_wins.p, P->d_lcand.p, G, P->d_SW.p, P->d_HW.p, P->d_lsig.p,
+                                          kLminMaxK, P->d_icnt.p, P->d_litot.p, nullptr, P->st, lin),
+                      "matrix-core inertia kernel");
+            hip_check(launch_lmin_round(st, P->d_lsig.p, P->d_litot.p, P->d_icnt.p, kLminMaxK, P->d_lcand.p,
+                                        P->d_ltmp.p, P->st),
+                      "lambda_min controller");
+        }
+        // the cell: one count at the grid point inside the final bracket when it has one
+        int* skip = P->d_edge_skip.p;
+        hip_check(launch_lmin_snap(st, width, P->d_lsig.p, P->d_litot.p, skip, P->st), "lambda_min controller");
+        hip_check(launch_inertia_mfma(nt, P->d_wins.p, ids, nwin, P->d_SW.p, P->d_HW.p, P->d_lsig.p, 1, nullptr,
+                                      P->d_litot.p, skip, P->st),
+                  "matrix-core inertia kernel");
+        hip_check(launch_lmin_snap_end(st, width, P->d_lsig.p, P->d_litot.p, skip, P->st), "lambda_min controller");
+        LminState h;
+        hip_check(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, P->st), "copy");
+        hip_check(hipStreamSynchronize(P->st), "sync");
+        require(h.fail != 1, "inertia count at the Rayleigh bound is zero (indefinite S?)");
+        if (h.fail == 2) return false;
+        if (h.done) {
+            *cell_out = h.cell;
+            return true;
+        }
+        require(batch < 16, "lambda_min: no convergence of the multisection");
+    }
 }
 
 }  // namespace
@@ -1248,6 +1382,12 @@ int lres_plan_set_basis(lres_plan* P, const int64_t* aof, int64_t natom, const i
         P->d_vpair.alloc(std::max<int64_t>(1, vpoff[npair]));
         P->d_hmap.upload(hmap.data(), hmap.size(), st);
         P->d_TW.upload(TW.data(), TW.size(), st);
+        P->d_vcur.alloc(ng);
+        P->d_vref.alloc(ng);
+        P->d_edge_part.alloc(kEdgeBlocks);
+        const double edge0[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        P->d_edge.upload(edge0, 5, st);
+        P->d_edge_skip.alloc(1);
         hip_check(hipStreamSynchronize(st), "sync");
         P->nf_max = nf_max;
         P->npair = npair;
@@ -1271,7 +1411,11 @@ int lres_set_potential(lres_plan* P, const double* d_v) {
         hip_check(launch_window_hv(P->d_TW.n, P->d_hmap.p, P->d_vpair.p, P->d_TW.p, P->d_HW.p, P->st),
                   "window H kernel");
         if (timed) hip_check(hipEventRecord(P->ev[1], P->st), "event");
+        // the potential of this H and its distance from the keep-edge reference
+        hip_check(launch_edge_delta(d_v, P->d_vcur.p, P->d_vref.p, P->ng, P->d_edge_part.p, P->d_edge.p, P->st),
+                  "keep-edge bound");
         P->have_H = true;
+        P->pot_h = true;
     });
 }
 
@@ -1466,6 +1610,12 @@ int lres_lambda_min(lres_plan* P, double tol, double* out) {
         require(tol > 0, "tol must be positive");
         require(P->have_H, "no Hamiltonian loaded");
         const int nwin = (int)P->nwin;
+        const double width = lmin_width(tol);
+        double cell = 0.0;
+        if (lmin_on_device(P) && lambda_min_device(P, tol, &cell)) {
+            lambda_min_finish(P, cell, cell + width, out);
+            return;
+        }
         // upper bound: the smallest Rayleigh quotient H_aa / S_aa over all windows
         if ((int64_t)P->d_rq.n < nwin) P->d_rq.alloc(nwin);
         hip_check(launch_diag_ratio(P->d_wins.p, nwin, P->d_SW.p, P->d_HW.p, P->d_rq.p, P->st), "diag kernel");
@@ -1492,7 +1642,7 @@ int lres_lambda_min(lres_plan* P, double tol, double* out) {
         // bracket narrows, and K grows as it does (about kWaves waves per launch, at
         // most kMaxK shifts: log2(K + 1) bits of the bracket per round) on the
         // one-wave-per-shift kernels.
-        constexpr int kWaves = 8192, kMaxK = 255;
+        constexpr int kWaves = kLminWaves, kMaxK = kLminMaxK;
         // the blocked inertia kernel of windows above 128 functions factors each
         // (window, shift) in a workgroup of its own: about kBlocks of them per launch
         // (a pass of the whole chip), so bisection while every window takes part and
@@ -1509,7 +1659,7 @@ int lres_lambda_min(lres_plan* P, double tol, double* out) {
         while (hi - lo > tol) {
             const int ncand = all ? nwin : (int)cand.size();
             const int K = blocked ? std::max(1, std::min(kMaxK, kBlocks / std::max(1, ncand)))
-                                  : std::max(15, std::min(kMaxK, kWaves / std::max(1, ncand)));
+                                  : std::max(kLminMinK, std::min(kMaxK, kWaves / std::max(1, ncand)));
             for (int s = 0; s < K; ++s) sg[s] = lo + (hi - lo) * (s + 1) / (K + 1);
             // the per-window counts come back while the set is large enough for
             // shrinking it to pay (small sets run at up to kMaxK shifts anyway), and
@@ -1540,7 +1690,13 @@ int lres_lambda_min(lres_plan* P, double tol, double* out) {
             lo = nlo;
             hi = nhi;
         }
-        *out = 0.5 * (lo + hi);
+        // the dyadic cell holding lambda_min, whatever path the search took
+        const double g = lmin_grid(lo, hi, width, &cell);
+        if (!std::isnan(g)) {
+            inertia(P, &g, 1, &cnt, nullptr);
+            cell = cnt > 0 ? g - width : g;
+        }
+        lambda_min_finish(P, cell, cell + width, out);
     });
 }
 
@@ -1628,7 +1784,8 @@ int lres_step_enqueue(lres_plan* P, double N, double mu0, double e_lo, double ma
         // the keep-edge test (lcao3d.py:420) on the device: counts at e_lo + margin / 3 and
         // e_lo + 2 margin, read by the first controller stage
         const double sig[2] = {e_lo + margin / 3, e_lo + 2 * margin};
-        inertia_launch(P, sig, 2, nullptr);
+        const EdgeTest et{e_lo, margin};
+        inertia_launch(P, sig, 2, nullptr, nullptr, &et);
         hip_check(launch_mu_begin(P->d_step.p, P->d_itot.p, N, e_lo, mu0, slope, keep_tol, ftol, xtol, lag,
                                   P->d_circ.p, P->st),
                   "mu controller");
@@ -1744,6 +1901,8 @@ int lres_set_window_blocks(lres_plan* P, const double* d_HW, const double* d_SW)
             P->have_H = true;
         }
         if (d_SW) hip_check(hipMemcpyAsync(P->d_SW.p, d_SW, nn * sizeof(double), hipMemcpyDeviceToDevice, P->st), "copy");
+        // blocks from elsewhere: no longer H(v) on the plan's S_W
+        P->pot_h = P->edge_ok = false;
     });
 }
 
@@ -1992,6 +2151,20 @@ int lres_plan_stats(lres_plan* P, double* out4, int reset) {
             P->stat_passes[0] = P->stat_passes[1] = 0;
             P->stat_ms[0] = P->stat_ms[1] = 0.0;
         }
+    });
+}
+
+int lres_plan_edge_stats(lres_plan* P, double* h_out2) {
+    return guarded([&] {
+        require(P != nullptr, "null plan");
+        require(h_out2 != nullptr, "null output");
+        h_out2[0] = h_out2[1] = 0.0;
+        if (P->d_edge.n < 5) return;
+        double e[5];
+        hip_check(hipMemcpyAsync(e, P->d_edge.p, sizeof(e), hipMemcpyDeviceToHost, P->st), "copy");
+        hip_check(hipStreamSynchronize(P->st), "sync");
+        h_out2[0] = e[4];
+        h_out2[1] = e[3];
     });
 }
 

@@ -265,6 +265,13 @@ class WindowPlan:
         nat.check(nat.lib().lres_plan_stats(self.h, nat.pf(t), int(reset)))
         return {"fp64": (int(t[0]), float(t[1])), "fp32": (int(t[2]), float(t[3]))}
 
+    def edge_stats(self):
+        """(warm keep-edge tests checked against the potential-change bound, of them
+        decided by the bound without inertia counts), cumulative."""
+        t = np.zeros(2)
+        nat.check(nat.lib().lres_plan_edge_stats(self.h, nat.pf(t)))
+        return int(t[0]), int(t[1])
+
     def flops_per_pass(self):
         """Algorithmic flops of one pass: sum_B (M/2) [(8/3) n^3 + 8 n^2 k] (complex LU + two
         triangular solves with k right-hand sides, a complex multiply-add counted as 8)."""

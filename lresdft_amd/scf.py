@@ -260,7 +260,7 @@ class PotentialStep:
 
 
 def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion=None, dens0=None, fp32_floor=3e-5,
-             graph=True, count_rel=1e-2):
+             graph=True, count_rel=1e-2, count_cap=1e-4):
     """Device SCF; ``solver.density_device(v)`` returns (density tensor [ng], e_band, mu).
 
     A solver with ``count_tol``/``refine`` (``LRFTLCAOCpp``) solves mu each step
@@ -346,7 +346,7 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
             # costs the iteration nothing (the first step, before any residual is
             # known, takes count_rel x 1e-2); the converged step is polished in fp64
             # to the strict tolerance below
-            solver.count_tol = min(1e-4, max(count_rel * (1e-2 if err is None else err), 1e-11))
+            solver.count_tol = min(count_cap, max(count_rel * (1e-2 if err is None else err), 1e-11))
         ts = time.perf_counter()
         # the new history slot (the oldest one once the history is full)
         free = [q for q in range(ndiis) if q not in slots]

@@ -77,3 +77,56 @@ def test_single_warm_step_equals_host_step(sys3):
     assert out[True][3] == out[False][3] == "warm-resolve"
     assert out[True][2] == out[False][2] and out[True][1] == out[False][1] and out[True][4] == out[False][4]
     assert np.array_equal(out[True][0], out[False][0])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_keep_edge_bound_changes_no_decision(sys3, precision, monkeypatch):
+    """The keep-edge test decided from the potential-change bound (every window
+    eigenvalue moves by at most max |v - v_ref| since the last lambda_min) where it
+    can be: the same decisions as the two inertia counts, so the SCF is the same bit
+    for bit, and most warm steps need no counts."""
+    print(f"[gpu] keep-edge bound on and off, SCF n=3 R_w 8 M 32 {precision}", flush=True)
+    monkeypatch.setenv("LRES_EDGE_BOUND", "0")
+    r0, C0 = _scf(sys3, precision)
+    assert C0.plan.edge_stats()[1] == 0
+    monkeypatch.setenv("LRES_EDGE_BOUND", "1")
+    r1, C1 = _scf(sys3, precision)
+    checked, skipped = C1.plan.edge_stats()
+    print(f"    warm keep-edge tests {checked}, decided by the bound {skipped}; histories {C1.history}", flush=True)
+    assert checked > 0 and skipped > 0
+    assert C0.history == C1.history and C0.pass_log == C1.pass_log
+    assert r0["iterations"] == r1["iterations"]
+    assert r0["mu"] == r1["mu"] and r0["energy"] == r1["energy"]
+    assert np.array_equal(r0["density"].cpu().numpy(), r1["density"].cpu().numpy())
+
+
+@pytest.mark.parametrize("n", [3, 8])
+def test_device_lambda_min_equals_host_search(n, monkeypatch):
+    """lambda_min searched on the device (probe, bracket and multisection rounds decided
+    by controller kernels, one wait) against the host-driven search: the same bracket
+    arithmetic on the same counts, so the same value bit for bit, at the guess
+    potential and at a shifted one; both bracket the smallest window eigenvalue
+    (inertia counts on either side)."""
+    print(f"[gpu] device vs host lambda_min, {n}^3 H2 lattice, R_w 8", flush=True)
+    import torch
+    from lresdft_amd import system
+    from lresdft_amd.lcao3d import LRFTLCAOCpp
+    from lresdft_amd.scf import GridOps
+    cr = system.h2_crystal((n, n, n), a=6.0, pts_per_cell=10, rc=0.6)
+    b = system.LCAOBasis(cr, **system.DEFAULT_BASIS)
+    g = GridOps(cr)
+    vion = g.ion_potential()
+    C = LRFTLCAOCpp(b, system.cell_centres((n, n, n), 6.0), R_w=8.0, M=32)
+    assert len(C.plan.buckets()) == 1
+    dens = torch.full(cr.shape, cr.n_elec / (cr.ng * cr.dV), dtype=torch.float64, device="cuda")
+    v = g.potential(dens, vion).reshape(-1).contiguous()
+    for shift in (0.0, 0.3):
+        C.plan.set_potential(v + shift)
+        monkeypatch.setenv("LRES_LMIN", "host")
+        lh = C.plan.lambda_min()
+        monkeypatch.setenv("LRES_LMIN", "device")
+        ld = C.plan.lambda_min()
+        print(f"    shift {shift}: lambda_min {ld!r} (host {lh!r})", flush=True)
+        assert ld == lh
+        lo, hi = C.plan.inertia([ld - 1e-9, ld + 1e-9])
+        assert lo == 0 and hi > 0
