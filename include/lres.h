@@ -248,6 +248,22 @@ int lres_mu_solve_warm(lres_plan* plan, double N, double e_lo, double mu0, doubl
 int lres_step_enqueue(lres_plan* plan, double N, double mu0, double e_lo, double margin, double keep_tol,
                       double ftol, double xtol, double slope, int lag, int depth, double* d_dens);
 
+/* lres_step_enqueue in two halves, so that the step can be captured into a graph
+ * (no counterpart in the reference: its warm step runs on the host, lcao3d.py:412-448):
+ * lres_step_params writes the step's inputs (those of lres_step_enqueue but depth and
+ * d_dens) to the plan's parameter block in pinned host memory, which the step's
+ * kernels read when they run; lres_step_launch queues the first batch from it on the
+ * plan's stream.  Launched while that stream is captured, the batch records nothing
+ * host-dependent (no allocation, wait, memset or timing event; the kernels must have
+ * been tuned by one launch outside a capture) and a replay of the graph runs the step
+ * with the parameters written before it; lres_step_arm(depth) then tells the plan that
+ * a replay queued a step of `depth` slots, and lres_step_finish waits for it as for an
+ * enqueued one.  lres_step_enqueue = lres_step_params + lres_step_launch. */
+int lres_step_params(lres_plan* plan, double N, double mu0, double e_lo, double margin, double keep_tol,
+                     double ftol, double xtol, double slope, int lag);
+int lres_step_launch(lres_plan* plan, int depth, double* d_dens);
+int lres_step_arm(lres_plan* plan, int depth);
+
 /* Waits for the queued step (queuing more trial batches when the search needs them)
  * and returns h_out[10] = {status (0: done, d_dens holds the density; 1: the keep-edge
  * test failed and nothing was computed; 2: no mu bracket within 60 secant steps),

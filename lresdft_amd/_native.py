@@ -61,6 +61,10 @@ SIGNATURES = {
     "lres_step_enqueue": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                     C.c_double, C.c_double, C.c_int, C.c_int, _vp]),
     "lres_step_finish": (C.c_int, [_vp, _vp, _f64p]),
+    "lres_step_params": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                   C.c_double, C.c_double, C.c_int]),
+    "lres_step_launch": (C.c_int, [_vp, C.c_int, _vp]),
+    "lres_step_arm": (C.c_int, [_vp, C.c_int]),
     "lres_windows": (C.c_int, [_i64p, _i64p, _f64p, _i64p, _i64p, _f64p, _i64p, _i64p, _f64p,
                                _i64p, _i64p, C.c_int64, _i64p, _i64p, C.c_int64, C.c_int64, C.c_double,
                                C.c_double, C.c_double, C.c_int, _f64p, _f64p, _f64p]),

@@ -41,16 +41,19 @@ hipError_t launch_set_circle(double* circ, double c, double R, double skip, doub
 // the inertia totals itot if given); k_sum_step sums a slot's N_B and E_B into out and
 // runs one controller stage (StepStage) on them; kStepPublish copies the state to `pin`
 // (mapped host memory); k_mu_step runs a stage without sums (kStepResume).
-hipError_t launch_mu_begin(lres::StepState* st, const unsigned long long* itot, double N, double e_lo, double mu0,
-                           double slope, double keep_tol, double ftol, double xtol, int lag, double* circ,
+// The inputs of a device step, in this order, in a parameter block (doubles) the host
+// writes before launching the step or replaying a graph that holds it.
+enum StepPrm {
+    kStepPrmN = 0, kStepPrmMu0, kStepPrmElo, kStepPrmMargin, kStepPrmKeepTol, kStepPrmFtol, kStepPrmXtol,
+    kStepPrmSlope, kStepPrmLag, kStepPrmCount
+};
+hipError_t launch_mu_begin(lres::StepState* st, const unsigned long long* itot, const double* prm, double* circ,
                            hipStream_t stream);
 hipError_t launch_mu_step(lres::StepState* st, int stage, double* circ, void* pin, hipStream_t stream);
 // Keep-edge bound (lres_kernels.hip): edge = {lo, hi, delta, skipped, checked}.
 // launch_edge_delta copies v to vcur and sets delta = max |v - vref| (kEdgeBlocks
 // partial maxima in part); launch_edge_ref starts a reference bracket [lo, hi] (delta
-// 0); launch_edge_check decides the keep-edge test at e_lo from the bound when it can,
-// writing the inertia totals to itot and *skip = 1 (the inertia kernels then leave at
-// once), else *skip = 0.
+// 0); the keep-edge test of a device step is decided from it by launch_step_edge.
 constexpr int kEdgeBlocks = 256;
 // lambda_min on the device (lres_kernels.hip): the search state, read by the count
 // launches of a round through &ncand ({ncand, K, done}); kLminProbe bracket probes below
@@ -97,8 +100,10 @@ hipError_t launch_lmin_snap_end(LminState* s, double width, const double* sig, c
 hipError_t launch_edge_delta(const double* v, double* vcur, const double* vref, int64_t ng, double* part,
                              double* edge, hipStream_t st);
 hipError_t launch_edge_ref(double* edge, double lo, double hi, hipStream_t st);
-hipError_t launch_edge_check(double* edge, double e_lo, double margin, double safety, unsigned long long* itot,
-                             int* skip, hipStream_t st);
+// the keep-edge test's set-up for a device step (shifts, zeroed totals, the bound's
+// decision when edge is given), from the parameter block prm
+hipError_t launch_step_edge(const double* prm, double* sig, unsigned long long* itot, double* edge, double safety,
+                            int* skip, hipStream_t st);
 hipError_t launch_sum_step(const double* a, const double* b, int n, double* out, lres::StepState* st, int stage,
                            double* circ, void* pin, hipStream_t stream);
 hipError_t launch_pole_gj(int T, int G, int wide, int fp32, const lres::WinDesc* wins, const int32_t* ids, int nids,

@@ -884,16 +884,29 @@ __global__ void k_edge_ref(double* __restrict__ edge, double lo, double hi) {
 // totals it would give (0 below e_lo + m / 3, 1 below e_lo + 2 m) go to itot and *skip
 // tells the inertia kernels to leave at once; safety covers the rounding of the
 // factorisations the bound stands in for
-__global__ void k_edge_check(double* __restrict__ edge, double e_lo, double margin, double safety,
-                             unsigned long long* __restrict__ itot, int* __restrict__ skip) {
-    const double lo = edge[0] - edge[2], hi = edge[1] + edge[2];
-    const bool proved = lo > e_lo + margin / 3 + safety && hi < e_lo + 2 * margin - safety;
-    if (proved) {
-        itot[0] = 0;
-        itot[1] = 1;
-        edge[3] += 1.0;
+// The keep-edge test of a device step, set up on the device from the parameter block:
+// its shifts e_lo + m / 3 and e_lo + 2 m into sig (as the host forms them), the totals
+// zeroed (no memset: the step may be captured into a graph), and with edge given the
+// decision from the bound where it proves it (a reference bracket of NaN proves nothing)
+__global__ void k_step_edge(const double* __restrict__ prm, double* __restrict__ sig,
+                            unsigned long long* __restrict__ itot, double* __restrict__ edge, double safety,
+                            int* __restrict__ skip) {
+#pragma clang fp contract(off)
+    const double e_lo = prm[kStepPrmElo], margin = prm[kStepPrmMargin];
+    sig[0] = e_lo + margin / 3;
+    sig[1] = e_lo + 2 * margin;
+    itot[0] = 0;
+    itot[1] = 0;
+    bool proved = false;
+    if (edge != nullptr) {
+        const double lo = edge[0] - edge[2], hi = edge[1] + edge[2];
+        proved = lo > e_lo + margin / 3 + safety && hi < e_lo + 2 * margin - safety;
+        if (proved) {
+            itot[1] = 1;
+            edge[3] += 1.0;
+        }
+        edge[4] += 1.0;
     }
-    edge[4] += 1.0;
     *skip = proved ? 1 : 0;
 }
 
@@ -1078,11 +1091,13 @@ __global__ void k_lmin_snap_end(LminState* __restrict__ s, double width, const d
 // itot (may be null): inertia totals at (e_lo + margin / 3, e_lo + 2 margin); the
 // filter edge is kept only when the first is zero and the second positive
 // (lcao3d.py:420), else the step stops with kWarmEdge before any pass
-__global__ void k_mu_begin(StepState* __restrict__ st, const unsigned long long* __restrict__ itot, double N,
-                           double e_lo, double mu0, double slope, double keep_tol, double ftol, double xtol, int lag,
-                           double* __restrict__ circ) {
+// (the step's inputs from the parameter block prm, kStepPrm* order, written by the host
+// before the step is launched or its graph replayed)
+__global__ void k_mu_begin(StepState* __restrict__ st, const unsigned long long* __restrict__ itot,
+                           const double* __restrict__ prm, double* __restrict__ circ) {
     WarmSearch& s = st->s;
-    warm_begin_keep(s, N, e_lo, mu0, slope, keep_tol, ftol, xtol, lag);
+    warm_begin_keep(s, prm[kStepPrmN], prm[kStepPrmElo], prm[kStepPrmMu0], prm[kStepPrmSlope], prm[kStepPrmKeepTol],
+                    prm[kStepPrmFtol], prm[kStepPrmXtol], (int)prm[kStepPrmLag]);
     st->final_pass = 0;
     st->final_ran = 0;
     if (itot != nullptr && !(itot[0] == 0 && itot[1] > 0)) s.phase = kWarmEdge;
@@ -1533,17 +1548,15 @@ hipError_t launch_edge_ref(double* edge, double lo, double hi, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_edge_check(double* edge, double e_lo, double margin, double safety, unsigned long long* itot,
-                             int* skip, hipStream_t st) {
-    hipLaunchKernelGGL(k_edge_check, dim3(1), dim3(1), 0, st, edge, e_lo, margin, safety, itot, skip);
+hipError_t launch_step_edge(const double* prm, double* sig, unsigned long long* itot, double* edge, double safety,
+                            int* skip, hipStream_t st) {
+    hipLaunchKernelGGL(k_step_edge, dim3(1), dim3(1), 0, st, prm, sig, itot, edge, safety, skip);
     return hipGetLastError();
 }
 
-hipError_t launch_mu_begin(StepState* st, const unsigned long long* itot, double N, double e_lo, double mu0,
-                           double slope, double keep_tol, double ftol, double xtol, int lag, double* circ,
+hipError_t launch_mu_begin(StepState* st, const unsigned long long* itot, const double* prm, double* circ,
                            hipStream_t stream) {
-    hipLaunchKernelGGL(k_mu_begin, dim3(1), dim3(1), 0, stream, st, itot, N, e_lo, mu0, slope, keep_tol, ftol, xtol,
-                       lag, circ);
+    hipLaunchKernelGGL(k_mu_begin, dim3(1), dim3(1), 0, stream, st, itot, prm, circ);
     return hipGetLastError();
 }
 

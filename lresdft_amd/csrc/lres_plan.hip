@@ -250,7 +250,7 @@ struct lres_plan {
     // edge_ok: d_edge and vref describe a lambda_min on such an H
     DevBuf<double> d_vcur, d_vref, d_edge_part, d_edge;
     DevBuf<int> d_edge_skip;
-    bool pot_h = false, edge_ok = false;
+    bool pot_h = false;
     // forces (lres_force_setup / lres_band_forces): the basis on per-atom boxes, the
     // atom pairs whose boxes meet with their window entries, grid-point owners
     bool have_forces = false;
@@ -280,6 +280,10 @@ struct lres_plan {
     // after the two blocks above: the device step's state (StepState), published there
     // by its last controller stage
     static constexpr int kStepPin = 64;
+    // after that: the parameter block of the next device step (StepPrm order), written by
+    // the host before the step is launched or its graph replayed
+    static constexpr int kPrmPin = 16;
+    static constexpr int kPrmOff = 2 * kPinSlots + kStepPin;
     double* h_pin = nullptr;
     double* d_pin = nullptr;
     // the circle of the pass in flight in device memory: {c, R, skip, skip_density}
@@ -291,6 +295,8 @@ struct lres_plan {
     DevBuf<lres::StepState> d_step;
     bool step_queued = false;
     int step_depth = 0;
+    bool step_timed = false;                     // the queued step's slots carry timing events
+    DevBuf<double> d_ssig;                        // the keep-edge shifts of a device step
     std::vector<hipEvent_t> ev_slot;
     ~lres_plan() {
         for (auto& e : ev)
@@ -350,6 +356,23 @@ int guarded(F&& fn) {
     }
 }
 
+// The keep-edge bound replaces the two inertia counts of a warm step when it proves
+// their outcome (k_edge_check); LRES_EDGE_BOUND=0 switches it off (the counts always
+// run; same decisions, so the same results).  kEdgeSafety (Ha) keeps a proved decision
+// clear of the rounding of the factorisations it replaces.
+constexpr double kEdgeSafety = 1e-8;
+
+bool edge_bound_enabled() {
+    const char* e = getenv("LRES_EDGE_BOUND");
+    return !(e && std::string(e) == "0");
+}
+
+// no keep-edge reference (the current H is not H(v) of the plan's basis): a NaN
+// bracket, which proves no decision
+void edge_clear(lres_plan* P) {
+    if (P->d_edge.n >= 5) hip_check(launch_edge_ref(P->d_edge.p, NAN, NAN, P->st), "keep-edge bound");
+}
+
 void gather_h(lres_plan* P, const int64_t* Hp, const int64_t* Hi, const double* Hv) {
     hip_check(hipEventRecord(P->ev[0], P->st), "event");
     hip_check(launch_gather_csr(P->d_wins.p, (int)P->nwin, P->d_Wflat.p, P->d_Woff.p, P->d_pos.p, Hp, Hi, Hv,
@@ -357,7 +380,8 @@ void gather_h(lres_plan* P, const int64_t* Hp, const int64_t* Hi, const double* 
               "gather kernel");
     hip_check(hipEventRecord(P->ev[1], P->st), "event");
     P->have_H = true;
-    P->pot_h = P->edge_ok = false;
+    P->pot_h = false;
+    edge_clear(P);
 }
 
 // Windows above the register-resident limit of the vector kernels: in fp32 the
@@ -608,40 +632,11 @@ bool inertia_on_mfma(int nmax) {
 
 // queue the inertia counts at ns shifts (totals into d_itot, per window into d_counts
 // if given) without waiting for them
-// The keep-edge bound replaces the two inertia counts of a warm step when it proves
-// their outcome (k_edge_check); LRES_EDGE_BOUND=0 switches it off (the counts always
-// run; same decisions, so the same results).  kEdgeSafety (Ha) keeps a proved decision
-// clear of the rounding of the factorisations it replaces.
-constexpr double kEdgeSafety = 1e-8;
-
-bool edge_bound_enabled() {
-    const char* e = getenv("LRES_EDGE_BOUND");
-    return !(e && std::string(e) == "0");
-}
-
-// edge: the two shifts of a keep-edge test at (e_lo, margin), decided first from the
-// keep-edge bound when the plan holds one (the counts then run only when it cannot)
-struct EdgeTest {
-    double e_lo, margin;
-};
-
-void inertia_launch(lres_plan* P, const double* sig, int ns, int32_t* d_counts,
-                    const std::vector<int32_t>* cand = nullptr, const EdgeTest* edge = nullptr) {
-    require(P->have_H, "no Hamiltonian loaded");
-    require(ns > 0 && ns <= 1024, "nshift must be in 1..1024");
-    require(ns <= lres_plan::kPinSlots, "too many shifts");
-    std::memcpy(P->h_pin + lres_plan::kPinSlots, sig, ns * sizeof(double));
-    const double* d_sig = P->d_pin + lres_plan::kPinSlots;
-    if ((int)P->d_itot.n < ns) P->d_itot.alloc(ns);
-    hip_check(hipMemsetAsync(P->d_itot.p, 0, ns * sizeof(unsigned long long), P->st), "memset");
-    const int* skip = nullptr;
-    if (edge && P->edge_ok && edge_bound_enabled()) {
-        require(ns == 2, "a keep-edge test has two shifts");
-        hip_check(launch_edge_check(P->d_edge.p, edge->e_lo, edge->margin, kEdgeSafety, P->d_itot.p,
-                                    P->d_edge_skip.p, P->st),
-                  "keep-edge bound");
-        skip = P->d_edge_skip.p;
-    }
+// the count kernels at the ns shifts d_sig (device-readable), totals added into d_itot
+// (zeroed by the caller), per-window counts into d_counts if given; skip (may be null):
+// every kernel leaves at once when *skip != 0
+void inertia_kernels(lres_plan* P, const double* d_sig, int ns, int32_t* d_counts,
+                     const std::vector<int32_t>* cand = nullptr, const int* skip = nullptr) {
     std::vector<int32_t> sub;        // candidate ids grouped by bucket
     std::vector<int64_t> sub_off;    // per bucket: first entry in sub
     if (cand) {
@@ -709,6 +704,17 @@ void inertia_launch(lres_plan* P, const double* sig, int ns, int32_t* d_counts,
     if (spread) side_join(P, used);
 }
 
+void inertia_launch(lres_plan* P, const double* sig, int ns, int32_t* d_counts,
+                    const std::vector<int32_t>* cand = nullptr) {
+    require(P->have_H, "no Hamiltonian loaded");
+    require(ns > 0 && ns <= 1024, "nshift must be in 1..1024");
+    require(ns <= lres_plan::kPinSlots, "too many shifts");
+    std::memcpy(P->h_pin + lres_plan::kPinSlots, sig, ns * sizeof(double));
+    if ((int)P->d_itot.n < ns) P->d_itot.alloc(ns);
+    hip_check(hipMemsetAsync(P->d_itot.p, 0, ns * sizeof(unsigned long long), P->st), "memset");
+    inertia_kernels(P, P->d_pin + lres_plan::kPinSlots, ns, d_counts, cand);
+}
+
 void inertia(lres_plan* P, const double* sig, int ns, int64_t* h_total, int32_t* d_counts,
              const std::vector<int32_t>* cand = nullptr, int32_t* h_counts = nullptr) {
     inertia_launch(P, sig, ns, d_counts, cand);
@@ -734,17 +740,48 @@ constexpr int kMaxStepDepth = 8, kMoreStepDepth = 4;
 // when it was not the last trial; the density either way) and the publish stage, which
 // copies the search state to the mapped staging.  The slots the search does not need
 // leave at once.  Pole-kernel events per slot for the pass statistics.
-void step_batch(lres_plan* P, int depth, double* d_dens) {
+void step_batch(lres_plan* P, int depth, double* d_dens, bool timed) {
     if ((int)P->ev_slot.size() < 2 * (kMaxStepDepth + 1)) {
         P->ev_slot.resize(2 * (kMaxStepDepth + 1), nullptr);
         for (auto& e : P->ev_slot)
             if (!e) hip_check(hipEventCreate(&e), "hipEventCreate");
     }
+    auto ev = [&](int i) { return timed ? P->ev_slot[i] : nullptr; };
     for (int i = 0; i < depth; ++i)
-        queue_pass(P, nullptr, nullptr, nullptr, P->d_sums.p, P->ev_slot[2 * i], P->ev_slot[2 * i + 1],
+        queue_pass(P, nullptr, nullptr, nullptr, P->d_sums.p, ev(2 * i), ev(2 * i + 1),
                    i + 1 < depth ? lres::kStepFeed : lres::kStepFeedFinal);
-    queue_pass(P, d_dens, nullptr, nullptr, P->d_sums.p, P->ev_slot[2 * depth], P->ev_slot[2 * depth + 1],
-               lres::kStepPublish);
+    queue_pass(P, d_dens, nullptr, nullptr, P->d_sums.p, ev(2 * depth), ev(2 * depth + 1), lres::kStepPublish);
+}
+
+// is the plan's stream being captured into a graph
+bool capturing(lres_plan* P) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    hip_check(hipStreamIsCapturing(P->st, &cs), "capture status");
+    return cs != hipStreamCaptureStatusNone;
+}
+
+// the buckets of this precision are tuned (a step can then be captured: tuning runs and
+// waits for kernels)
+bool passes_prepared(const lres_plan* P, int fp32) {
+    for (const auto& b : P->buckets)
+        if (!(b.big && !big_on_mfma(b.nmax, fp32)) && !b.tuned[fp32]) return false;
+    return true;
+}
+
+// The first batch of a device step from the parameter block at P->d_pin + kPrmOff: the
+// keep-edge test (lcao3d.py:420; its shifts, zeroed totals and the bound's decision by
+// k_step_edge, then the counts unless the bound decided), the controller's start on the
+// totals, depth trial slots, the final slot and the publish stage.  No host value of the
+// step is baked into the launches, and nothing here allocates, waits or sets memory, so
+// the batch can be captured into a graph and replayed with new parameters.
+void step_launch(lres_plan* P, int depth, double* d_dens, bool timed) {
+    const double* prm = P->d_pin + lres_plan::kPrmOff;
+    double* edge = edge_bound_enabled() && P->d_edge.n >= 5 ? P->d_edge.p : nullptr;
+    hip_check(launch_step_edge(prm, P->d_ssig.p, P->d_itot.p, edge, kEdgeSafety, P->d_edge_skip.p, P->st),
+              "keep-edge test");
+    inertia_kernels(P, P->d_ssig.p, 2, nullptr, nullptr, P->d_edge_skip.p);
+    hip_check(launch_mu_begin(P->d_step.p, P->d_itot.p, prm, P->d_circ.p, P->st), "mu controller");
+    step_batch(P, depth, d_dens, timed);
 }
 
 }  // namespace
@@ -762,12 +799,12 @@ double lmin_width(double tol) {
 // hi): the midpoint; the cell is also the keep-edge reference at the potential of this H
 void lambda_min_finish(lres_plan* P, double lo, double hi, double* out) {
     *out = 0.5 * (lo + hi);
-    P->edge_ok = false;
     if (P->pot_h) {
         hip_check(hipMemcpyAsync(P->d_vref.p, P->d_vcur.p, P->ng * sizeof(double), hipMemcpyDeviceToDevice, P->st),
                   "copy");
         hip_check(launch_edge_ref(P->d_edge.p, lo, hi, P->st), "keep-edge bound");
-        P->edge_ok = true;
+    } else {
+        edge_clear(P);
     }
 }
 
@@ -990,7 +1027,8 @@ int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, 
         for (auto& e : P->ev) hip_check(hipEventCreate(&e), "hipEventCreate");
         static_assert(sizeof(lres::StepState) <= lres_plan::kStepPin * sizeof(double), "step state staging");
         static_assert(sizeof(lres::StepState) % 8 == 0, "step state copied as 8-byte words");
-        hip_check(hipHostMalloc((void**)&P->h_pin, (2 * lres_plan::kPinSlots + lres_plan::kStepPin) * sizeof(double),
+        static_assert(kStepPrmCount <= lres_plan::kPrmPin, "step parameter block");
+        hip_check(hipHostMalloc((void**)&P->h_pin, (lres_plan::kPrmOff + lres_plan::kPrmPin) * sizeof(double),
                                 hipHostMallocMapped | hipHostMallocCoherent),
                   "hipHostMalloc");
         hip_check(hipHostGetDevicePointer((void**)&P->d_pin, P->h_pin, 0), "hipHostGetDevicePointer");
@@ -1155,6 +1193,9 @@ int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, 
         P->d_sums.alloc(2);
         P->d_circ.alloc(4);
         P->d_step.alloc(1);
+        P->d_ssig.alloc(2);
+        if (P->d_itot.n < 2) P->d_itot.alloc(2);
+        if (P->d_edge_skip.n < 1) P->d_edge_skip.alloc(1);
         // whole atom blocks in every owned row: 6 consecutive T-local columns per
         // block, blocks ascending (the density kernel's lane-per-point form)
         P->dens_blocks6 = 1;
@@ -1770,29 +1811,64 @@ int lres_mu_solve_warm(lres_plan* P, double N, double e_lo, double mu0, double f
     });
 }
 
-int lres_step_enqueue(lres_plan* P, double N, double mu0, double e_lo, double margin, double keep_tol, double ftol,
-                      double xtol, double slope, int lag, int depth, double* d_dens) {
+int lres_step_params(lres_plan* P, double N, double mu0, double e_lo, double margin, double keep_tol, double ftol,
+                     double xtol, double slope, int lag) {
+    return guarded([&] {
+        require(P != nullptr, "null plan");
+        require(!P->step_queued, "a queued density step is pending: call lres_step_finish first");
+        require(mu0 > e_lo, "mu must lie above e_lo");
+        double* q = P->h_pin + lres_plan::kPrmOff;
+        q[kStepPrmN] = N;
+        q[kStepPrmMu0] = mu0;
+        q[kStepPrmElo] = e_lo;
+        q[kStepPrmMargin] = margin;
+        q[kStepPrmKeepTol] = keep_tol;
+        q[kStepPrmFtol] = ftol;
+        q[kStepPrmXtol] = xtol;
+        q[kStepPrmSlope] = slope;
+        q[kStepPrmLag] = lag ? 1.0 : 0.0;
+    });
+}
+
+int lres_step_launch(lres_plan* P, int depth, double* d_dens) {
     return guarded([&] {
         require(P != nullptr, "null plan");
         require(P->have_H, "no Hamiltonian loaded");
         require(!P->step_queued, "a queued density step is pending: call lres_step_finish first");
-        require(mu0 > e_lo, "mu must lie above e_lo");
         require(depth >= 1 && depth <= kMaxStepDepth, "depth must be in 1.." + std::to_string(kMaxStepDepth));
         require(d_dens != nullptr, "null density buffer");
         const int fp32 = P->precision == LRES_MIXED ? 1 : 0;
-        prepare_passes(P, fp32, 0.5 * (mu0 + e_lo), 0.5 * (mu0 - e_lo));
-        // the keep-edge test (lcao3d.py:420) on the device: counts at e_lo + margin / 3 and
-        // e_lo + 2 margin, read by the first controller stage
-        const double sig[2] = {e_lo + margin / 3, e_lo + 2 * margin};
-        const EdgeTest et{e_lo, margin};
-        inertia_launch(P, sig, 2, nullptr, nullptr, &et);
-        hip_check(launch_mu_begin(P->d_step.p, P->d_itot.p, N, e_lo, mu0, slope, keep_tol, ftol, xtol, lag,
-                                  P->d_circ.p, P->st),
-                  "mu controller");
-        step_batch(P, depth, d_dens);
+        const bool cap = capturing(P);
+        if (cap) {
+            require(passes_prepared(P, fp32), "a step is captured before its kernels were tuned: launch one first");
+        } else {
+            const double* q = P->h_pin + lres_plan::kPrmOff;
+            prepare_passes(P, fp32, 0.5 * (q[kStepPrmMu0] + q[kStepPrmElo]), 0.5 * (q[kStepPrmMu0] - q[kStepPrmElo]));
+        }
+        step_launch(P, depth, d_dens, !cap);
+        if (!cap) {
+            P->step_queued = true;
+            P->step_depth = depth;
+            P->step_timed = true;
+        }
+    });
+}
+
+int lres_step_arm(lres_plan* P, int depth) {
+    return guarded([&] {
+        require(P != nullptr, "null plan");
+        require(!P->step_queued, "a queued density step is pending: call lres_step_finish first");
+        require(depth >= 1 && depth <= kMaxStepDepth, "depth must be in 1.." + std::to_string(kMaxStepDepth));
         P->step_queued = true;
         P->step_depth = depth;
+        P->step_timed = false;
     });
+}
+
+int lres_step_enqueue(lres_plan* P, double N, double mu0, double e_lo, double margin, double keep_tol, double ftol,
+                      double xtol, double slope, int lag, int depth, double* d_dens) {
+    const int rc = lres_step_params(P, N, mu0, e_lo, margin, keep_tol, ftol, xtol, slope, lag);
+    return rc ? rc : lres_step_launch(P, depth, d_dens);
 }
 
 int lres_step_finish(lres_plan* P, double* d_dens, double* h_out) {
@@ -1807,10 +1883,11 @@ int lres_step_finish(lres_plan* P, double* d_dens, double* h_out) {
         const int more = std::max(depth, kMoreStepDepth);
         const lres::StepState* st = reinterpret_cast<const lres::StepState*>(P->h_pin + 2 * lres_plan::kPinSlots);
         int evals_before = 0, final_before = 0, batches = 1;
+        bool timed = P->step_timed;
         for (;;) {
             hip_check(hipStreamSynchronize(P->st), "sync");
             // pole-kernel time of the slots that ran: the trials first, then the final pass
-            const int ran = st->s.evals - evals_before;
+            const int ran = timed ? st->s.evals - evals_before : 0;
             for (int i = 0; i < ran && i < depth; ++i) {
                 float ms = 0;
                 if (hipEventElapsedTime(&ms, P->ev_slot[2 * i], P->ev_slot[2 * i + 1]) == hipSuccess) {
@@ -1818,7 +1895,7 @@ int lres_step_finish(lres_plan* P, double* d_dens, double* h_out) {
                     P->stat_ms[fp32] += ms;
                 }
             }
-            if (st->final_ran > final_before) {
+            if (timed && st->final_ran > final_before) {
                 float ms = 0;
                 if (hipEventElapsedTime(&ms, P->ev_slot[2 * depth], P->ev_slot[2 * depth + 1]) == hipSuccess) {
                     P->stat_passes[fp32] += 1;
@@ -1834,7 +1911,8 @@ int lres_step_finish(lres_plan* P, double* d_dens, double* h_out) {
                                      P->st),
                       "mu controller");
             depth = more;
-            step_batch(P, depth, d_dens);
+            timed = true;
+            step_batch(P, depth, d_dens, true);
         }
         const lres::WarmSearch& s = st->s;
         const bool done = s.phase == lres::kWarmDone;
@@ -1902,7 +1980,8 @@ int lres_set_window_blocks(lres_plan* P, const double* d_HW, const double* d_SW)
         }
         if (d_SW) hip_check(hipMemcpyAsync(P->d_SW.p, d_SW, nn * sizeof(double), hipMemcpyDeviceToDevice, P->st), "copy");
         // blocks from elsewhere: no longer H(v) on the plan's S_W
-        P->pot_h = P->edge_ok = false;
+        P->pot_h = false;
+        edge_clear(P);
     });
 }
 

@@ -327,6 +327,7 @@ class LRFTLCAOCpp:
         self.device_step = bool(device_step)
         self.step_depth = int(step_depth)
         self._queued = None     # N of the queued step
+        self._eager_steps = {}  # warm steps queued outside a graph capture, per precision
         self._p_step = None     # plan.passes when the step was queued
         self._new_edge = False  # the queued step's keep-edge test failed: go to a new edge
         self.step_batches = 0   # trial batches of the last device step (more than one: work
@@ -446,13 +447,47 @@ class LRFTLCAOCpp:
         keep = 1e-8 if self.count_tol is None else min(1e-8, self.count_tol)
         return max(keep * N, self._ftol(N))
 
+    def _step_params(self, N):
+        """The warm step's inputs at the current (mu, e_lo) to the plan's parameter
+        block (lres_step_params), read by the step's kernels when they run."""
+        nat.check(nat.lib().lres_step_params(self.plan.h, float(N), float(self.mu), float(self.e_lo),
+                                             float(self.margin), float(self._keep_tol(N)), float(self._ftol(N)),
+                                             float(self.mu_xtol), float(self._slope or 0.0), int(bool(self.lag_mu))))
+
     def _step_queue(self, N):
         """Queue the warm step at the current (mu, e_lo) on the device: keep-edge test,
-        keep pass, search, density into plan.dens (lres_step_enqueue)."""
-        nat.check(nat.lib().lres_step_enqueue(self.plan.h, float(N), float(self.mu), float(self.e_lo),
-                                              float(self.margin), float(self._keep_tol(N)), float(self._ftol(N)),
-                                              float(self.mu_xtol), float(self._slope or 0.0), int(bool(self.lag_mu)),
-                                              int(self.step_depth), nat.C.c_void_p(self.plan.dens.data_ptr())))
+        keep pass, search, density into plan.dens (lres_step_params + lres_step_launch)."""
+        self._step_params(N)
+        self.step_launch()
+        self._queued = N
+        self._eager_steps[self.precision] = self._eager_steps.get(self.precision, 0) + 1
+
+    def step_launch(self):
+        """Queue the first batch of a warm step from the parameter block on the plan's
+        stream (lres_step_launch); inside a graph capture it records the step for replays."""
+        nat.check(nat.lib().lres_step_launch(self.plan.h, int(self.step_depth),
+                                             nat.C.c_void_p(self.plan.dens.data_ptr())))
+
+    def step_capturable(self):
+        """A warm step of the current precision can be captured into a graph: the plan
+        queues warm steps on the device, there is a previous mu, and one step of this
+        precision has run outside a capture (its kernels are tuned)."""
+        return (self.device_h and self.device_step and self.mu is not None and not self._new_edge
+                and self._eager_steps.get(self.precision, 0) > 0)
+
+    def density_params(self, N=None):
+        """Write the next warm step's parameters before replaying a graph that holds
+        the step (see step_capturable / density_armed)."""
+        self._step_params(self.b.cr.n_elec if N is None else N)
+
+    def density_armed(self, v_dev, N=None):
+        """After a graph replay that assembled H(v_dev) and launched the warm step: the
+        bookkeeping of density_queue (lres_step_arm); then call density_collect."""
+        N = self.b.cr.n_elec if N is None else N
+        self.plan.last = None
+        self.plan._v_keep = v_dev
+        self._p_step = self.plan.passes
+        nat.check(nat.lib().lres_step_arm(self.plan.h, int(self.step_depth)))
         self._queued = N
 
     def _step_collect(self):

@@ -209,8 +209,14 @@ class PotentialStep:
         self.coef = torch.zeros(nh, dtype=torch.float64, device=grid.device)
         self.dens = torch.empty(cr.ng, dtype=torch.float64, device=grid.device)
         self.part = torch.empty(nat.LRES_MIX_PARTS, dtype=torch.float64, device=grid.device)
+        # the residual statistics (sum |r| and the Gram row): pinned, written by the
+        # residual kernel (also from a replayed iteration graph), and on the device
+        self.stats_pin = torch.empty(nh + 1, dtype=torch.float64).pin_memory()
+        self.stats_dev = torch.empty(nh + 1, dtype=torch.float64, device=grid.device)
         self.v = None
         self.graph = None
+        # whole-iteration graphs per (precision, history slot): (graph, potential)
+        self.iter_graphs = {}
         self.replays = 0
 
     @property
@@ -227,7 +233,9 @@ class PotentialStep:
         self.solver.plan.set_potential(v)
         return v
 
-    def _capture(self):
+    def _capture(self, tail=None):
+        """Capture _body() (and tail(), queued behind it) into a new graph on a side
+        stream, the plan switched to that stream meanwhile; returns (graph, potential)."""
         torch = self.torch
         plan = self.solver.plan
         home = plan.stream
@@ -239,10 +247,12 @@ class PotentialStep:
         plan.set_stream(side)
         try:
             with torch.cuda.graph(g, stream=side):
-                self.v = self._body()
+                v = self._body()
+                if tail is not None:
+                    tail()
         finally:
             plan.set_stream(home)
-        self.graph = g
+        return g, v
 
     def run(self, coef):
         """Mix with the coefficient vector `coef` (numpy, one per history row) and
@@ -251,12 +261,28 @@ class PotentialStep:
         self.coef.copy_(self.coef_host, non_blocking=True)
         if self.use_graph:
             if self.graph is None:
-                self._capture()
+                self.graph, self.v = self._capture()
             self.graph.replay()
             self.replays += 1
         else:
             self.v = self._body()
         return self.dens, self.v
+
+    def run_iteration(self, coef, key, tail):
+        """The whole iteration as one graph: the step of run() and the work tail()
+        queues behind it (the solver's warm density step from its parameter block and
+        the residual of its density into history slot key[1]), captured at the first
+        use of `key` and replayed after; returns (density [ng], potential [ng]).  The
+        step's inputs are read from pinned memory when the graph runs, so the caller
+        writes them before this call and arms the solver after it."""
+        self.coef_host.numpy()[:] = coef
+        self.coef.copy_(self.coef_host, non_blocking=True)
+        entry = self.iter_graphs.get(key)
+        if entry is None:
+            entry = self.iter_graphs[key] = self._capture(tail)
+        entry[0].replay()
+        self.replays += 1
+        return self.dens, entry[1]
 
 
 def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion=None, dens0=None, fp32_floor=3e-5,
@@ -319,20 +345,28 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
     adaptive = hasattr(solver, "count_tol") and hasattr(solver, "refine")
     cfull = None
     if step is not None:
-        stats_dev = torch.empty(ndiis + 1, dtype=torch.float64, device=grid.device)
-        stats_pin = torch.empty(ndiis + 1, dtype=torch.float64, pin_memory=True)
+        # residual statistics buffers held by the step (a captured iteration graph writes
+        # them by address)
+        stats_dev, stats_pin = step.stats_dev, step.stats_pin
         if ndiis > nat.LRES_MAX_HIST:
             raise ValueError(f"ndiis must be at most {nat.LRES_MAX_HIST}")
         # warm steps queued on the device with the residual behind them: one wait per
         # iteration (needs the solver's plan on this stream)
         queue_ok = hasattr(solver, "density_queue") and solver.plan.stream == torch.cuda.current_stream()
     for it in range(1, maxiter + 1):
+        # a warm step of a solver that can hold it in a graph: the whole iteration (mixing,
+        # potential, H(v), the step and its residual) is one replay, below
+        whole = False
         if cfull is None:
             v = grid.potential(dens, vion)
             preset = False
         elif step is not None:
-            dflat, v = step.run(cfull)
-            dens = dflat.reshape(cr.shape)
+            whole = step.use_graph and queue_ok and solver.step_capturable()
+            if not whole:
+                dflat, v = step.run(cfull)
+                dens = dflat.reshape(cr.shape)
+            else:
+                dens = step.dens.reshape(cr.shape)
             preset = True
         else:
             cvec = torch.from_numpy(cfull).to(grid.device)
@@ -364,11 +398,25 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
                     nat.C.c_void_p(torch.cuda.current_stream().cuda_stream)))
 
             out = None
-            queued = queue_ok and solver.density_queue(v, potential_set=preset)
+            if whole:
+                # the step's parameters to pinned memory, then one replay (captured at the
+                # first use of this precision and history slot)
+                solver.density_params()
+
+                def tail():
+                    solver.step_launch()
+                    residual(solver.plan.dens, stats_pin)
+
+                dflat, v = step.run_iteration(cfull, (solver.precision, slot), tail)
+                solver.density_armed(v)
+                queued = True
+            else:
+                queued = queue_ok and solver.density_queue(v, potential_set=preset)
+                if queued:
+                    residual(solver.plan.dens, stats_pin)
             if queued:
                 # the warm step and the residual of its density queued together, one wait;
                 # the residual's totals go straight to pinned host memory
-                residual(solver.plan.dens, stats_pin)
                 out = solver.density_collect()
                 if out is not None:
                     nraw, eband, mu = out

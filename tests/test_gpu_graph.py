@@ -92,3 +92,33 @@ def test_scf_with_graph_equals_eager_scf():
     assert r1["graph_replays"] == r1["iterations"] - 1 and r0["graph_replays"] == 0
     assert r0["iterations"] == r1["iterations"] and r0["residuals"] == r1["residuals"]
     assert r0["energy"] == r1["energy"]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_whole_iteration_graph_scf_equals_eager_scf(precision):
+    """Warm iterations as one graph replay each (mixing, potential, H(v), the warm
+    density step with its keep-edge test and mu search, the residual; one graph per
+    precision and history slot, the step's inputs read from pinned memory): the SCF
+    equals the eager one bit for bit, the graphs are replayed in later SCFs of the
+    same solver, and those SCFs equal the eager one too (eager work and waits between
+    every replay)."""
+    import torch
+    cr, C, g, vion = _system(3)
+    from lresdft_amd.scf import scf_lcao
+    print(f"[gpu] device SCF n=3 {precision}: whole-iteration graphs vs eager", flush=True)
+    C.set_precision(precision)
+    r0 = scf_lcao(cr, C, tol=1e-8, grid=g, vion=vion, graph=False)
+    h0 = list(C.history)
+    runs = []
+    for _ in range(2):
+        C.mu = None
+        C.history = []
+        runs.append(scf_lcao(cr, C, tol=1e-8, grid=g, vion=vion, graph=True))
+        assert C.history == h0
+    step = C._scf_step
+    print(f"    iterations {r0['iterations']}, iteration graphs {sorted(step.iter_graphs)}", flush=True)
+    assert len(step.iter_graphs) > 0
+    for r1 in runs:
+        assert r1["converged"] and r1["iterations"] == r0["iterations"]
+        assert r1["residuals"] == r0["residuals"] and r1["energy"] == r0["energy"] and r1["mu"] == r0["mu"]
+        assert torch.equal(r0["density"], r1["density"])
