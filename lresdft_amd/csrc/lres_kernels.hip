@@ -610,6 +610,9 @@ __global__ void __launch_bounds__(kDensBlkThreads) k_density_chunk(const WinDesc
                     double xbn[6];
 #pragma unroll
                     for (int j = 0; j < 6; ++j) xbn[j] = xv[64 * (6 * Bn + j)];
+                    // a padding block (zero values, column 0) after a real one reads entries
+                    // below the diagonal, which passes leave as the plan created them (zero)
+                    // or as the mirror kernel filled them, and multiplies them by zero
                     double u = 0.0;
 #pragma unroll
                     for (int i = 0; i < 6; ++i) {

@@ -1186,6 +1186,10 @@ int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, 
         P->d_Q.upload(Q_all.data(), Q_all.size(), st);
         P->d_D.alloc(std::max<size_t>(1, Q_all.size()));
         P->d_F.alloc(std::max<size_t>(1, Q_all.size()));
+        // passes write the upper triangles only; the density kernel's zero padding blocks
+        // multiply lower-triangle entries by zero, so those start as zeros (never NaN)
+        hip_check(hipMemsetAsync(P->d_D.p, 0, P->d_D.n * sizeof(double), P->st), "memset");
+        hip_check(hipMemsetAsync(P->d_F.p, 0, P->d_F.n * sizeof(double), P->st), "memset");
         P->d_Dpart.alloc(std::max<size_t>(1, Q_all.size() * P->ngroup));
         P->d_Fpart.alloc(std::max<size_t>(1, Q_all.size() * P->ngroup));
         P->d_NB.alloc(nwin);
@@ -1426,7 +1430,8 @@ int lres_plan_set_basis(lres_plan* P, const int64_t* aof, int64_t natom, const i
         P->d_vcur.alloc(ng);
         P->d_vref.alloc(ng);
         P->d_edge_part.alloc(kEdgeBlocks);
-        const double edge0[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        // no keep-edge reference until a lambda_min (a NaN bracket proves no decision)
+        const double edge0[5] = {NAN, NAN, 0.0, 0.0, 0.0};
         P->d_edge.upload(edge0, 5, st);
         P->d_edge_skip.alloc(1);
         hip_check(hipStreamSynchronize(st), "sync");
