@@ -710,7 +710,6 @@ void inertia_launch(lres_plan* P, const double* sig, int ns, int32_t* d_counts,
     require(ns > 0 && ns <= 1024, "nshift must be in 1..1024");
     require(ns <= lres_plan::kPinSlots, "too many shifts");
     std::memcpy(P->h_pin + lres_plan::kPinSlots, sig, ns * sizeof(double));
-    if ((int)P->d_itot.n < ns) P->d_itot.alloc(ns);
     hip_check(hipMemsetAsync(P->d_itot.p, 0, ns * sizeof(unsigned long long), P->st), "memset");
     inertia_kernels(P, P->d_pin + lres_plan::kPinSlots, ns, d_counts, cand);
 }
@@ -1198,7 +1197,8 @@ int lres_plan_create(lres_plan** out, int64_t nb, int64_t ng, double dV, int M, 
         P->d_circ.alloc(4);
         P->d_step.alloc(1);
         P->d_ssig.alloc(2);
-        if (P->d_itot.n < 2) P->d_itot.alloc(2);
+        // inertia totals at their largest (1024 shifts): a captured step holds this address
+        P->d_itot.alloc(1024);
         if (P->d_edge_skip.n < 1) P->d_edge_skip.alloc(1);
         // whole atom blocks in every owned row: 6 consecutive T-local columns per
         // block, blocks ascending (the density kernel's lane-per-point form)
