@@ -99,6 +99,8 @@ def measured_traffic(tag):
 
 def kernel_name(bucket, fp32):
     T, G, kind = bucket[5:8] if fp32 else bucket[2:5]
+    if kind == 2 and fp32 and G == 0:
+        return f"k_pole_mfma<float, tiles {T}, window-fused: 4 waves per window, pole sums and reduction inside>"
     if kind == 2:
         return f"k_pole_mfma<{'float' if fp32 else 'double'}, tiles {T}, waves {G}>"
     if T == 0:

@@ -173,6 +173,8 @@ __global__ void __launch_bounds__(64 * NWC * NP) __attribute__((amdgpu_waves_per
     // end instead of a read-modify-write of global memory per pole; NP > 1: the window's
     // D and F sums over all poles
     extern __shared__ double pacc[];
+    // NP > 1: one tile of every wave's weighted values per round, [wave][register][lane]
+    __shared__ __attribute__((aligned(16))) float2 stage[NP > 1 ? NP * 256 : 1];
     // the circle (c, R) of the pass; circ[2] != 0 marks a pass the device-side mu search
     // did not need (every workgroup leaves at once)
     if (circ[2] != 0.0) return;
@@ -406,46 +408,43 @@ __global__ void __launch_bounds__(64 * NWC * NP) __attribute__((amdgpu_waves_per
         }   // active
 
         if constexpr (NP > 1) {
-            // this round's poles into the window's sums, in wave (= pole) order, each
-            // value rounded to float as a per-pole slot holds it
+            // this round's poles into the window's sums, one tile at a time: every wave
+            // stages its Re(w G) and Re(w z G) of the tile, rounded to float as a per-pole
+            // slot holds them; then thread (wave q, lane l) adds entry (register q, lane l)
+            // of the round's poles, in pole order, to the double sums.  The same additions
+            // in the same order as the slots and k_reduce, with the four waves adding at
+            // once.  Tiles with no T row or column are skipped (m is the workgroup's)
             const int ei = lane & 15, eg = lane >> 4;
-            for (int q = 0; q < NP; ++q) {
-                __syncthreads();
-                // the accumulators made opaque per phase: their weighted values are then not
-                // hoisted out of the phase loop (which would hold every entry's pair at once)
 #pragma unroll
-                for (int I = 0; I < NT; ++I)
+            for (int I = 0; I < NT; ++I)
 #pragma unroll
-                    for (int j = 0; j < TJ; ++j)
-                        if (I < held(j)) asm volatile("" : "+v"(are[I][j]), "+v"(aim[I][j]));
-                if (q != mat || !active) continue;
-#pragma unroll
-                for (int I = 0; I < NT; ++I)
-#pragma unroll
-                    for (int j = 0; j < TJ; ++j) {
-                        if (I >= held(j)) continue;
-                        const int col = 16 * j + ei - m;
+                for (int j = 0; j < TJ; ++j) {
+                    if (I >= held(j) || 16 * (j + 1) <= m || 16 * (I + 1) <= m) continue;
+                    if (active) {
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
-                            const int row = 16 * I + MF::row(eg, r) - m;
-                            if (row >= 0 && row <= col && col < k) {
-                                const double gr = (double)are[I][j][r], gi = (double)aim[I][j][r];
-                                const double gd = (double)(float)fma(wt.re, gr, -(wt.im * gi));
-                                const double gf = (double)(float)fma(wz.re, gr, -(wz.im * gi));
-                                const int u = row * k - ((row * (row - 1)) >> 1) + (col - row);
-                                if (rd == 0 && q == 0) {
-                                    pacc[u] = 0.0 + gd;
-                                    pacc[acc + u] = 0.0 + gf;
-                                } else {
-                                    pacc[u] += gd;
-                                    pacc[acc + u] += gf;
-                                }
-                            }
+                            const double gr = (double)are[I][j][r], gi = (double)aim[I][j][r];
+                            stage[(4 * mat + r) * 64 + lane] = make_float2((float)fma(wt.re, gr, -(wt.im * gi)),
+                                                                           (float)fma(wz.re, gr, -(wz.im * gi)));
                         }
-                        // one tile's read-modify-writes at a time (bounded live registers)
-                        asm volatile("" ::: "memory");
                     }
-            }
+                    __syncthreads();
+                    const int col = 16 * j + ei - m;
+                    const int row = 16 * I + MF::row(eg, mat) - m;
+                    if (row >= 0 && row <= col && col < k) {
+                        const int u = row * k - ((row * (row - 1)) >> 1) + (col - row);
+                        double sd = rd == 0 ? 0.0 : pacc[u];
+                        double sf = rd == 0 ? 0.0 : pacc[acc + u];
+                        for (int q = 0; q < NP && NP * rd + q < npole; ++q) {
+                            const float2 g = stage[(4 * q + mat) * 64 + lane];
+                            sd += (double)g.x;
+                            sf += (double)g.y;
+                        }
+                        pacc[u] = sd;
+                        pacc[acc + u] = sf;
+                    }
+                    __syncthreads();   // the next tile rewrites the stage
+                }
             continue;
         }
 

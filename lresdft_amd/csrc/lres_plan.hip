@@ -81,15 +81,14 @@ bool use_mfma(int /*fp32*/) {
     return true;
 }
 
-// The window-fused complex64 form (G = 0): LRES_POLE_FUSED=1 makes it the only candidate
-// where it exists, =2 adds it to the autotuner's candidates; by default it is left out,
-// as it measured slower than one wave per matrix with the per-pole slots and k_reduce at
-// every size timed (its pole-ordered sums in LDS take four serial phases per round of
-// four poles).  Both forms give the same bits.
+// The window-fused complex64 form (G = 0) is one of the autotuner's candidates where it
+// exists (the tuner then times the per-pole-slot candidates together with k_reduce of the
+// bucket's windows, which the fused form does inside its kernel); LRES_POLE_FUSED=1 makes
+// it the only candidate, =0 leaves it out.  Both forms give the same bits.
 int fused_mode() {
     const char* e = getenv("LRES_POLE_FUSED");
-    if (!e) return 0;
-    return std::string(e) == "1" ? 1 : std::string(e) == "2" ? 2 : 0;
+    if (!e) return 2;
+    return std::string(e) == "1" ? 1 : std::string(e) == "0" ? 0 : 2;
 }
 
 std::vector<Shape> mfma_shapes(int nmax, int fp32) {
@@ -114,9 +113,9 @@ std::vector<Shape> mfma_shapes(int nmax, int fp32) {
     }
     if (out.empty() && least > 0 && least_bytes <= 256) out.push_back({nt, least, kMfma});
     // complex64: the window-fused form (G = 0: four waves per window, one pole each, the
-    // reduction in the kernel; the same bits as one wave per matrix with k_reduce), when
-    // asked for.  Its spills sit outside the block sweep (values kept across a pole), so
-    // up to 256 bytes of them are allowed
+    // reduction in the kernel; the same bits as one wave per matrix with k_reduce).  Its
+    // spills sit outside the block sweep (values kept across a pole), so up to 256 bytes
+    // of them are allowed
     if (fp32 && nt <= 8 && force <= 0 && fused_mode() != 0) {
         const int scratch = mfma_scratch_bytes(1, nt, 0);
         if (scratch >= 0 && scratch <= 256) {
@@ -425,12 +424,25 @@ int pole_acc_entries(const lres_plan* P, const lres_plan::Bucket& b, const Shape
 // Autotuner: time every valid shape of a bucket on the loaded H (two launches
 // after one warm-up, HIP events) and keep the fastest.  All shapes apply the
 // same operations to every matrix entry, so the choice never changes results.
+// When the window-fused complex64 form is a candidate, the per-pole-slot candidates
+// are timed with k_reduce of the bucket's windows (the others marked as reduced),
+// the work the fused form does inside its kernel.
 void tune_bucket(lres_plan* P, lres_plan::Bucket& b, int fp32, double c, double R) {
     const auto cands = pole_shapes(b.nmax, fp32);
     require(!cands.empty(), "no kernel shape for windows of " + std::to_string(b.nmax) + " functions");
     b.shape[fp32] = default_pole_shape(b.nmax, fp32);
     b.tuned[fp32] = true;
     if (!P->autotune || cands.size() == 1) return;
+    auto fused = [&](const Shape& sh) { return fp32 && sh.wide == kMfma && sh.G == 0; };
+    bool with_reduce = false;
+    for (const auto& sh : cands) with_reduce = with_reduce || fused(sh);
+    std::vector<int8_t> mark;
+    if (with_reduce) {
+        mark.assign(P->nwin, 2);
+        for (int32_t w : b.ids) mark[w] = 1;
+        P->d_by_pole.upload(mark.data(), mark.size(), P->st);
+        P->by_pole.clear();   // prepare_passes uploads the passes' table after tuning
+    }
     hipEvent_t e0, e1;
     hip_check(hipEventCreate(&e0), "event");
     hip_check(hipEventCreate(&e1), "event");
@@ -443,6 +455,11 @@ void tune_bucket(lres_plan* P, lres_plan::Bucket& b, int fp32, double c, double 
                                   P->st, pole_acc_entries(P, b, sh, fp32),
                                   lres::WinOut{P->d_D.p, P->d_F.p, P->d_Q.p, P->d_NB.p, P->d_EB.p}),
                       "pole kernel (tuning)");
+            if (with_reduce && !fused(sh))
+                hip_check(launch_reduce(P->d_wins.p, (int)P->nwin, P->ngroup, P->d_Dpart.p, P->d_Fpart.p,
+                                        P->d_slots.p, P->M / 2, P->d_by_pole.p, P->d_Q.p, P->d_D.p, P->d_F.p,
+                                        P->d_NB.p, P->d_EB.p, P->d_circ.p, P->st),
+                          "reduce kernel (tuning)");
         };
         launch();
         hip_check(hipEventRecord(e0, P->st), "event");
@@ -453,10 +470,10 @@ void tune_bucket(lres_plan* P, lres_plan::Bucket& b, int fp32, double c, double 
         float ms = 0;
         hip_check(hipEventElapsedTime(&ms, e0, e1), "event time");
         if (P->tune_log)
-            fprintf(stderr, "[lres tune] nmax=%d windows=%zu %s %s T=%d G=%d: %.4f ms/launch\n", b.nmax,
+            fprintf(stderr, "[lres tune] nmax=%d windows=%zu %s %s T=%d G=%d: %.4f ms/launch%s\n", b.nmax,
                     b.ids.size(), fp32 ? "fp32" : "fp64",
                     sh.wide == kMfma ? (sh.G == 0 ? "mfma-fused" : "mfma") : (sh.wide ? "valu512" : "valu128"), sh.T,
-                    sh.G, ms / 2);
+                    sh.G, ms / 2, with_reduce && !fused(sh) ? " (with k_reduce)" : "");
         if (ms < best) {
             best = ms;
             b.shape[fp32] = sh;
