@@ -166,15 +166,18 @@ __global__ void __launch_bounds__(64 * NWC * NP) __attribute__((amdgpu_waves_per
     // would share LDS banks
     constexpr int RS = sizeof(R) == 8 ? 5 : 4;
     constexpr int NBUF = NP > 1 ? NP : 2;
-    __shared__ __attribute__((aligned(16))) R rpr[NBUF][NR][RS];
-    __shared__ __attribute__((aligned(16))) R rpi[NBUF][NR][RS];
+    // NP > 1: between rounds the two planes are the two buffers of the stage of the
+    // round's sums (one tile of every wave's float pairs, [wave][register][lane], 8 KB:
+    // the planes are 128 rows deep then)
+    constexpr int NRP = NP > 1 ? 128 : NR;
+    static_assert(NP == 1 || (NR <= NRP && NBUF * NRP * RS == NP * 256 * 2), "stage buffer size");
+    __shared__ __attribute__((aligned(16))) R rpr[NBUF][NRP][RS];
+    __shared__ __attribute__((aligned(16))) R rpi[NBUF][NRP][RS];
     // fp64 with acc > 0: the group's D and F partial sums over its poles, packed upper
     // triangles of acc entries each (dynamic LDS), stored to Dpart / Fpart once at the
     // end instead of a read-modify-write of global memory per pole; NP > 1: the window's
     // D and F sums over all poles
     extern __shared__ double pacc[];
-    // NP > 1: one tile of every wave's weighted values per round, [wave][register][lane]
-    __shared__ __attribute__((aligned(16))) float2 stage[NP > 1 ? NP * 256 : 1];
     // the circle (c, R) of the pass; circ[2] != 0 marks a pass the device-side mu search
     // did not need (every workgroup leaves at once)
     if (circ[2] != 0.0) return;
@@ -413,19 +416,25 @@ __global__ void __launch_bounds__(64 * NWC * NP) __attribute__((amdgpu_waves_per
             // slot holds them; then thread (wave q, lane l) adds entry (register q, lane l)
             // of the round's poles, in pole order, to the double sums.  The same additions
             // in the same order as the slots and k_reduce, with the four waves adding at
-            // once.  Tiles with no T row or column are skipped (m is the workgroup's)
+            // once.  The stage buffers are the two panel planes, free once every wave is
+            // past its sweep; two buffers, so one barrier per tile.  Tiles with no T row or
+            // column are skipped (m is the workgroup's)
             const int ei = lane & 15, eg = lane >> 4;
+            int tb = 0;
+            __syncthreads();
 #pragma unroll
             for (int I = 0; I < NT; ++I)
 #pragma unroll
                 for (int j = 0; j < TJ; ++j) {
                     if (I >= held(j) || 16 * (j + 1) <= m || 16 * (I + 1) <= m) continue;
+                    float* sb = (tb & 1) ? &rpi[0][0][0] : &rpr[0][0][0];
+                    ++tb;
                     if (active) {
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
                             const double gr = (double)are[I][j][r], gi = (double)aim[I][j][r];
-                            stage[(4 * mat + r) * 64 + lane] = make_float2((float)fma(wt.re, gr, -(wt.im * gi)),
-                                                                           (float)fma(wz.re, gr, -(wz.im * gi)));
+                            *reinterpret_cast<float2*>(sb + 2 * ((4 * mat + r) * 64 + lane)) =
+                                make_float2((float)fma(wt.re, gr, -(wt.im * gi)), (float)fma(wz.re, gr, -(wz.im * gi)));
                         }
                     }
                     __syncthreads();
@@ -436,15 +445,15 @@ __global__ void __launch_bounds__(64 * NWC * NP) __attribute__((amdgpu_waves_per
                         double sd = rd == 0 ? 0.0 : pacc[u];
                         double sf = rd == 0 ? 0.0 : pacc[acc + u];
                         for (int q = 0; q < NP && NP * rd + q < npole; ++q) {
-                            const float2 g = stage[(4 * q + mat) * 64 + lane];
+                            const float2 g = *reinterpret_cast<const float2*>(sb + 2 * ((4 * q + mat) * 64 + lane));
                             sd += (double)g.x;
                             sf += (double)g.y;
                         }
                         pacc[u] = sd;
                         pacc[acc + u] = sf;
                     }
-                    __syncthreads();   // the next tile rewrites the stage
                 }
+            __syncthreads();   // the stage is the next round's panel memory
             continue;
         }
 
