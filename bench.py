@@ -303,10 +303,20 @@ def main():
         cells = system.cell_centres((n, n, n), args.a)
         workload = (f"h2_crystal {n}x{n}x{n} a={args.a} ({pts} grid points per cell) jitter 0.15 seed 0, "
                     f"{cr.n_elec} e-, R_w {args.rw}, M {args.M}, SCF tol {args.tol}")
+    # per-geometry set-up, timed beside the SCFs (not part of their wall): basis on the host,
+    # grid operators and the ion potential on the device, windows and plan (device tables)
+    t_s = time.perf_counter()
     b = system.LCAOBasis(cr, **system.DEFAULT_BASIS)
+    t_basis = time.perf_counter() - t_s
+    t_s = time.perf_counter()
     grid = GridOps(cr)
     vion = grid.ion_potential()
+    torch.cuda.synchronize()
+    t_ion = time.perf_counter() - t_s
+    t_s = time.perf_counter()
     C = LRFTLCAOCpp(b, cells, R_w=args.rw, M=args.M)
+    torch.cuda.synchronize()
+    t_plan = time.perf_counter() - t_s
     flops = C.plan.flops_per_pass()
 
     if args.profile_pass:
@@ -365,6 +375,9 @@ def main():
         "roofline_fp64": roofline(st64["fp64"], flops, PEAK_FP64, kernel_name(bk[0], False),
                                   measured_traffic("fp64") if default_workload else None),
         "cpu_baseline": None,
+        "setup_s": {"basis_host": t_basis, "grid_ops_and_ion_potential": t_ion, "windows_and_plan": t_plan,
+                    "note": ("per geometry, outside the SCF wall (value); the first pass of each precision "
+                             "also runs the kernel autotuner, which the warm-up SCFs absorb")},
     }
     if not args.no_forces:
         line["adjoint"] = adjoint_timing(C, grid, vion, mixed[-1], cells, args)

@@ -218,6 +218,9 @@ class PotentialStep:
         # whole-iteration graphs per (precision, history slot): (graph, potential)
         self.iter_graphs = {}
         self.replays = 0
+        # the first iteration's potential and H(v) from a given density: (graph, potential)
+        self.pot_graph = None
+        self.pot_replays = 0
 
     @property
     def solver(self):
@@ -233,9 +236,14 @@ class PotentialStep:
         self.solver.plan.set_potential(v)
         return v
 
-    def _capture(self, tail=None):
-        """Capture _body() (and tail(), queued behind it) into a new graph on a side
-        stream, the plan switched to that stream meanwhile; returns (graph, potential)."""
+    def _potential_body(self):
+        v = self.grid.potential(self.dens, self.vion)
+        self.solver.plan.set_potential(v)
+        return v
+
+    def _capture(self, tail=None, body=None):
+        """Capture body() (default _body(); and tail(), queued behind it) into a new graph on
+        a side stream, the plan switched to that stream meanwhile; returns (graph, potential)."""
         torch = self.torch
         plan = self.solver.plan
         home = plan.stream
@@ -247,7 +255,7 @@ class PotentialStep:
         plan.set_stream(side)
         try:
             with torch.cuda.graph(g, stream=side):
-                v = self._body()
+                v = (self._body if body is None else body)()
                 if tail is not None:
                     tail()
         finally:
@@ -267,6 +275,23 @@ class PotentialStep:
         else:
             self.v = self._body()
         return self.dens, self.v
+
+    def run_potential(self, dens):
+        """The potential of a given density and the window blocks H_W(v) (the SCF's first
+        iteration, before any history): dens copied into the step's density buffer, then
+        the potential and H(v) of run(), replayed from a graph of their own with graphs on
+        (captured at the first use); returns the potential [ng]."""
+        self.dens.copy_(dens.reshape(-1))
+        if not self.use_graph:
+            return self._potential_body()
+        if self.pot_graph is None:
+            # once eagerly first: library state the potential needs (the FFT plans of this
+            # grid) is then set up outside the capture
+            self._potential_body()
+            self.pot_graph = self._capture(body=self._potential_body)
+        self.pot_graph[0].replay()
+        self.pot_replays += 1
+        return self.pot_graph[1]
 
     def run_iteration(self, coef, key, tail):
         """The whole iteration as one graph: the step of run() and the work tail()
@@ -358,8 +383,13 @@ def scf_lcao(cr, solver, tol=1e-6, maxiter=60, mix=0.4, ndiis=6, grid=None, vion
         # potential, H(v), the step and its residual) is one replay, below
         whole = False
         if cfull is None:
-            v = grid.potential(dens, vion)
-            preset = False
+            if step is not None:
+                # the first potential and its H(v) (one replay with graphs on)
+                v = step.run_potential(dens)
+                preset = True
+            else:
+                v = grid.potential(dens, vion)
+                preset = False
         elif step is not None:
             whole = step.use_graph and queue_ok and solver.step_capturable()
             if not whole:
