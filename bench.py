@@ -304,7 +304,10 @@ def main():
         workload = (f"h2_crystal {n}x{n}x{n} a={args.a} ({pts} grid points per cell) jitter 0.15 seed 0, "
                     f"{cr.n_elec} e-, R_w {args.rw}, M {args.M}, SCF tol {args.tol}")
     # per-geometry set-up, timed beside the SCFs (not part of their wall): basis on the host,
-    # grid operators and the ion potential on the device, windows and plan (device tables)
+    # grid operators and the ion potential on the device, windows and plan (device tables);
+    # the device is initialised first, so that its start-up is not counted
+    torch.zeros(1, device="cuda")
+    torch.cuda.synchronize()
     t_s = time.perf_counter()
     b = system.LCAOBasis(cr, **system.DEFAULT_BASIS)
     t_basis = time.perf_counter() - t_s
@@ -376,8 +379,10 @@ def main():
                                   measured_traffic("fp64") if default_workload else None),
         "cpu_baseline": None,
         "setup_s": {"basis_host": t_basis, "grid_ops_and_ion_potential": t_ion, "windows_and_plan": t_plan,
-                    "note": ("per geometry, outside the SCF wall (value); the first pass of each precision "
-                             "also runs the kernel autotuner, which the warm-up SCFs absorb")},
+                    "note": ("per geometry, outside the SCF wall (value); grid_ops_and_ion_potential includes the "
+                             "FFT library's first-use set-up (runtime kernel compilation, once per process); the "
+                             "first pass of each precision also runs the kernel autotuner, which the warm-up SCFs "
+                             "absorb")},
     }
     if not args.no_forces:
         line["adjoint"] = adjoint_timing(C, grid, vion, mixed[-1], cells, args)

@@ -55,6 +55,10 @@ hipError_t launch_mu_step(lres::StepState* st, int stage, double* circ, void* pi
 // partial maxima in part); launch_edge_ref starts a reference bracket [lo, hi] (delta
 // 0); the keep-edge test of a device step is decided from it by launch_step_edge.
 constexpr int kEdgeBlocks = 256;
+// keep-edge bound state: [0, 1] the reference bracket of the smallest window eigenvalue,
+// [2] the largest rise and [5] the largest fall of the potential since, [3] tests the
+// bound decided, [4] tests checked
+constexpr int kEdgeState = 6;
 // lambda_min on the device (lres_kernels.hip): the search state, read by the count
 // launches of a round through &ncand ({ncand, K, done}); kLminProbe bracket probes below
 // the Rayleigh bound, about kLminWaves (window, shift) factorisations per round (one

@@ -835,45 +835,67 @@ __device__ void circle_off(double* circ) {
 // checked}.
 
 // vcur = v and per-block maxima of |v - vref| (a NaN anywhere propagates)
+// The largest of a and b, NaN when either is NaN (so that a NaN anywhere in the
+// potential reaches the bound, which then proves nothing)
+__device__ __forceinline__ double edge_max(double a, double b) {
+    return (a != a || b != b) ? a + b : (a > b ? a : b);
+}
+
+// The potential v copied to vcur, and per block the largest rise max(v - vref) and the
+// largest fall max(vref - v) (part[2 b], part[2 b + 1]).  With H_W(v) = T_W + X^T diag(v)
+// X dV and S_W = X^T X dV, x^T dH x / x^T S x is a weighted mean of v - vref, so every
+// window eigenvalue moves by at least -fall and at most +rise.
 __global__ void __launch_bounds__(256) k_edge_delta(const double* __restrict__ v, double* __restrict__ vcur,
                                                     const double* __restrict__ vref, int64_t ng,
                                                     double* __restrict__ part) {
-    __shared__ double red[256];
-    double m = 0.0;
+    __shared__ double red[2][256];
+    double up = -INFINITY, dn = -INFINITY;
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < ng; i += (int64_t)gridDim.x * 256) {
         const double x = v[i];
         vcur[i] = x;
-        const double d = fabs(x - vref[i]);
-        if (!(d <= m)) m = d;
+        const double d = x - vref[i];
+        up = edge_max(up, d);
+        dn = edge_max(dn, -d);
     }
-    red[threadIdx.x] = m;
+    red[0][threadIdx.x] = up;
+    red[1][threadIdx.x] = dn;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if (threadIdx.x < s) {
-            const double o = red[threadIdx.x + s];
-            if (!(o <= red[threadIdx.x])) red[threadIdx.x] = o;
+            red[0][threadIdx.x] = edge_max(red[0][threadIdx.x], red[0][threadIdx.x + s]);
+            red[1][threadIdx.x] = edge_max(red[1][threadIdx.x], red[1][threadIdx.x + s]);
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = red[0][0];
+        part[2 * blockIdx.x + 1] = red[1][0];
+    }
 }
 
+// the blocks' rise and fall into edge[2] and edge[5]
 __global__ void __launch_bounds__(256) k_edge_final(const double* __restrict__ part, int nb,
                                                     double* __restrict__ edge) {
-    __shared__ double red[256];
-    double m = 0.0;
-    for (int i = threadIdx.x; i < nb; i += 256)
-        if (!(part[i] <= m)) m = part[i];
-    red[threadIdx.x] = m;
+    __shared__ double red[2][256];
+    double up = -INFINITY, dn = -INFINITY;
+    for (int i = threadIdx.x; i < nb; i += 256) {
+        up = edge_max(up, part[2 * i]);
+        dn = edge_max(dn, part[2 * i + 1]);
+    }
+    red[0][threadIdx.x] = up;
+    red[1][threadIdx.x] = dn;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if (threadIdx.x < s) {
-            const double o = red[threadIdx.x + s];
-            if (!(o <= red[threadIdx.x])) red[threadIdx.x] = o;
+            red[0][threadIdx.x] = edge_max(red[0][threadIdx.x], red[0][threadIdx.x + s]);
+            red[1][threadIdx.x] = edge_max(red[1][threadIdx.x], red[1][threadIdx.x + s]);
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) edge[2] = red[0];
+    if (threadIdx.x == 0) {
+        edge[2] = red[0][0];
+        edge[5] = red[1][0];
+    }
 }
 
 // a new reference: the lowest eigenvalue bracket [lo, hi] at the current potential
@@ -881,6 +903,7 @@ __global__ void k_edge_ref(double* __restrict__ edge, double lo, double hi) {
     edge[0] = lo;
     edge[1] = hi;
     edge[2] = 0.0;
+    edge[5] = 0.0;
 }
 
 // the keep-edge test at e_lo decided from the bound when it can be: then the inertia
@@ -902,7 +925,8 @@ __global__ void k_step_edge(const double* __restrict__ prm, double* __restrict__
     itot[1] = 0;
     bool proved = false;
     if (edge != nullptr) {
-        const double lo = edge[0] - edge[2], hi = edge[1] + edge[2];
+        // the smallest eigenvalue falls by at most edge[5] and rises by at most edge[2]
+        const double lo = edge[0] - edge[5], hi = edge[1] + edge[2];
         proved = lo > e_lo + margin / 3 + safety && hi < e_lo + 2 * margin - safety;
         if (proved) {
             itot[1] = 1;

@@ -356,7 +356,7 @@ int guarded(F&& fn) {
 }
 
 // The keep-edge bound replaces the two inertia counts of a warm step when it proves
-// their outcome (k_edge_check); LRES_EDGE_BOUND=0 switches it off (the counts always
+// their outcome (k_step_edge); LRES_EDGE_BOUND=0 switches it off (the counts always
 // run; same decisions, so the same results).  kEdgeSafety (Ha) keeps a proved decision
 // clear of the rounding of the factorisations it replaces.
 constexpr double kEdgeSafety = 1e-8;
@@ -369,7 +369,7 @@ bool edge_bound_enabled() {
 // no keep-edge reference (the current H is not H(v) of the plan's basis): a NaN
 // bracket, which proves no decision
 void edge_clear(lres_plan* P) {
-    if (P->d_edge.n >= 5) hip_check(launch_edge_ref(P->d_edge.p, NAN, NAN, P->st), "keep-edge bound");
+    if (P->d_edge.n >= kEdgeState) hip_check(launch_edge_ref(P->d_edge.p, NAN, NAN, P->st), "keep-edge bound");
 }
 
 void gather_h(lres_plan* P, const int64_t* Hp, const int64_t* Hi, const double* Hv) {
@@ -792,7 +792,7 @@ bool passes_prepared(const lres_plan* P, int fp32) {
 // the batch can be captured into a graph and replayed with new parameters.
 void step_launch(lres_plan* P, int depth, double* d_dens, bool timed) {
     const double* prm = P->d_pin + lres_plan::kPrmOff;
-    double* edge = edge_bound_enabled() && P->d_edge.n >= 5 ? P->d_edge.p : nullptr;
+    double* edge = edge_bound_enabled() && P->d_edge.n >= kEdgeState ? P->d_edge.p : nullptr;
     hip_check(launch_step_edge(prm, P->d_ssig.p, P->d_itot.p, edge, kEdgeSafety, P->d_edge_skip.p, P->st),
               "keep-edge test");
     inertia_kernels(P, P->d_ssig.p, 2, nullptr, nullptr, P->d_edge_skip.p);
@@ -1446,10 +1446,10 @@ int lres_plan_set_basis(lres_plan* P, const int64_t* aof, int64_t natom, const i
         P->d_TW.upload(TW.data(), TW.size(), st);
         P->d_vcur.alloc(ng);
         P->d_vref.alloc(ng);
-        P->d_edge_part.alloc(kEdgeBlocks);
+        P->d_edge_part.alloc(2 * kEdgeBlocks);
         // no keep-edge reference until a lambda_min (a NaN bracket proves no decision)
-        const double edge0[5] = {NAN, NAN, 0.0, 0.0, 0.0};
-        P->d_edge.upload(edge0, 5, st);
+        const double edge0[kEdgeState] = {NAN, NAN, 0.0, 0.0, 0.0, 0.0};
+        P->d_edge.upload(edge0, kEdgeState, st);
         P->d_edge_skip.alloc(1);
         hip_check(hipStreamSynchronize(st), "sync");
         P->nf_max = nf_max;
@@ -2260,8 +2260,8 @@ int lres_plan_edge_stats(lres_plan* P, double* h_out2) {
         require(P != nullptr, "null plan");
         require(h_out2 != nullptr, "null output");
         h_out2[0] = h_out2[1] = 0.0;
-        if (P->d_edge.n < 5) return;
-        double e[5];
+        if (P->d_edge.n < kEdgeState) return;
+        double e[kEdgeState];
         hip_check(hipMemcpyAsync(e, P->d_edge.p, sizeof(e), hipMemcpyDeviceToHost, P->st), "copy");
         hip_check(hipStreamSynchronize(P->st), "sync");
         h_out2[0] = e[4];
