@@ -146,6 +146,26 @@ __device__ __forceinline__ void static_for(F&& f) {
 // workgroup forms D_B, F_B, N_B and E_B exactly as k_reduce does from the slots (the
 // same additions in the same order, the trace sums over k_reduce's 1024 virtual threads
 // and reduction tree), so both forms give the same bits.
+// Tile columns of the matrix-core pole kernel's waves (see k_pole_mfma): with NWC > 1,
+// round j deals columns NT - 1 - j NWC down to NT - NWC - j NWC to waves 0..NWC-1, in
+// reverse wave order on odd rounds.  held: tile rows kept for a wave's j-th column (the
+// largest of the round's columns + 1); owner and index: where tile column c lives.
+template <int NT, int NWC>
+__host__ __device__ constexpr int pole_held(int j) {
+    if (NWC == 1) return j + 1 < NT ? j + 1 : NT;
+    return NT - j * NWC;
+}
+template <int NT, int NWC>
+__host__ __device__ constexpr int pole_col_index(int c) {
+    return NWC == 1 ? c : (NT - 1 - c) / NWC;
+}
+template <int NT, int NWC>
+__host__ __device__ constexpr int pole_col_owner(int c) {
+    if (NWC == 1) return 0;
+    const int j = (NT - 1 - c) / NWC, q = (NT - 1 - c) % NWC;
+    return (j & 1) ? NWC - 1 - q : q;
+}
+
 template <typename R, int NT, int NWC, int WPE, int NP>
 __global__ void __launch_bounds__(64 * NWC * NP) __attribute__((amdgpu_waves_per_eu(WPE))) k_pole_mfma(const WinDesc* __restrict__ wins,
                                                         const int32_t* __restrict__ ids,
@@ -158,7 +178,6 @@ __global__ void __launch_bounds__(64 * NWC * NP) __attribute__((amdgpu_waves_per
     static_assert(NP == 1 || (NWC == 1 && sizeof(R) == 4), "the window-fused form is complex64, one wave per matrix");
     constexpr int TJ = (NT + NWC - 1) / NWC;   // tile columns per wave
     constexpr int NR = 16 * NT;                 // padded order
-    constexpr bool kEven = NT % NWC == 0;
     // row panel of the current block, planar (real and imaginary parts apart) so
     // that accumulator registers go to LDS and come back as MFMA operands without
     // repacking: rp[b][j][s] = A[4p + s][j].  fp64 rows are padded to 40 bytes: the
@@ -220,10 +239,21 @@ __global__ void __launch_bounds__(64 * NWC * NP) __attribute__((amdgpu_waves_per
     float2* gs = reinterpret_cast<float2*>(Dpart) + d.kk_off * (M / 2);
     const int nst = (n + 3) >> 2;
 
-    // are[I][j], aim[I][j]: tile (I, wv + NWC j), held for I < held(j) only
+    // are[I][j], aim[I][j]: tile (I, cmap(j)), held for I < held(j) only.  One wave per
+    // matrix holds tile column j as its j-th; with NWC > 1 the columns (column c has c + 1
+    // upper tiles) are dealt largest first in snake order, so that the waves' shares of
+    // the upper triangle balance (NT 6: {5, 0} {4, 1} {3, 2} over 3 waves, {5} {4} {3, 0}
+    // {2, 1} over 4).  Every wave keeps and updates held(j) tile rows of its j-th column
+    // (the same register layout in every wave; the tiles below the diagonal are never
+    // read), so a wave's updates per block step are the sum of held(j): 8 tiles at NT 6 over
+    // 4 waves (10 with columns dealt round-robin), 7 over 5 (11)
     V are[NT][TJ], aim[NT][TJ];
-    auto owned = [&](int j) __attribute__((always_inline)) { return kEven || wv + NWC * j < NT; };
-    auto held = [](int j) __attribute__((always_inline)) { return NWC * (j + 1) < NT ? NWC * (j + 1) : NT; };
+    auto cmap = [&](int j) __attribute__((always_inline)) {
+        if constexpr (NWC == 1) return j;
+        return NT - 1 - (j * NWC + ((j & 1) ? NWC - 1 - wv : wv));   // < 0: no j-th column
+    };
+    auto owned = [&](int j) __attribute__((always_inline)) { return NWC == 1 || cmap(j) >= 0; };
+    auto held = [](int j) __attribute__((always_inline)) { return pole_held<NT, NWC>(j); };
 
     // pivot rows of block p = 4 JPc + pq into buffer b: tile row JPc of the owned
     // columns J >= JPc, and left of it the tile column JPc above the diagonal,
@@ -233,8 +263,8 @@ __global__ void __launch_bounds__(64 * NWC * NP) __attribute__((amdgpu_waves_per
     // B operands of columns J < JP take the opposite signs of their products)
     auto publish = [&](auto jpc, int pq, int b, int li, int lg) __attribute__((always_inline)) {
         constexpr int JPc = decltype(jpc)::value;
-        constexpr int jc = JPc / NWC;
-        if (JPc > 0 && wv == JPc % NWC && (li >> 2) == pq) {
+        constexpr int jc = pole_col_index<NT, NWC>(JPc);
+        if (JPc > 0 && wv == pole_col_owner<NT, NWC>(JPc) && (li >> 2) == pq) {
             const int sc = li & 3;
 #pragma unroll
             for (int I = 0; I < JPc; ++I)
@@ -247,8 +277,8 @@ __global__ void __launch_bounds__(64 * NWC * NP) __attribute__((amdgpu_waves_per
 #pragma unroll
         for (int j = 0; j < TJ; ++j) {
             if (JPc >= held(j) || !owned(j)) continue;
-            if (wv + NWC * j < JPc) continue;
-            const int col = 16 * (wv + NWC * j) + li;
+            if (cmap(j) < JPc) continue;
+            const int col = 16 * cmap(j) + li;
             if constexpr (sizeof(R) == 4) {   // lane group pq holds rows 4 pq + s in register s
                 if (lg == pq) {
                     *reinterpret_cast<V*>(&rpr[bsel(b)][col][0]) = are[JPc][j];
@@ -287,8 +317,8 @@ __global__ void __launch_bounds__(64 * NWC * NP) __attribute__((amdgpu_waves_per
         for (int I = 0; I < NT; ++I) {
 #pragma unroll
             for (int j = 0; j < TJ; ++j) {
-                if (I >= held(j)) continue;
-                const int col = 16 * (wv + NWC * j) + li;
+                if (I >= held(j) || !owned(j)) continue;   // an unowned column is never read
+                const int col = 16 * cmap(j) + li;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int row = 16 * I + MF::row(lg, r);
@@ -363,14 +393,14 @@ __global__ void __launch_bounds__(64 * NWC * NP) __attribute__((amdgpu_waves_per
                 for (int j = 0; j < TJ; ++j) {
                     nre[j] = rim[j] = nim[j] = R(0);
                     if (!owned(j)) continue;
-                    const int col = 16 * (wv + NWC * j) + li;
+                    const int col = 16 * cmap(j) + li;
                     R tr[4], ti[4];
 #pragma unroll
                     for (int s = 0; s < 4; ++s) {
                         tr[s] = rpr[bsel(b)][col][s];
                         ti[s] = rpi[bsel(b)][col][s];
                     }
-                    if (j == JP / NWC && wv == JP % NWC) {
+                    if (cmap(j) == JP) {
 #pragma unroll
                         for (int s = 0; s < 4; ++s)
                             if (li == 4 * pp + s) tr[s] += R(1);
@@ -383,7 +413,7 @@ __global__ void __launch_bounds__(64 * NWC * NP) __attribute__((amdgpu_waves_per
                     }
                     // columns left of the pivot tile read the panel without its sign:
                     // their products come out negated (exactly), so the signs swap
-                    const bool left = wv + NWC * j < JP;
+                    const bool left = cmap(j) < JP;
                     nre[j] = left ? xr : -xr;
                     rim[j] = left ? -xi : xi;
                     nim[j] = left ? xi : -xi;
@@ -468,7 +498,7 @@ __global__ void __launch_bounds__(64 * NWC * NP) __attribute__((amdgpu_waves_per
 #pragma unroll
             for (int j = 0; j < TJ; ++j) {
                 if (I >= held(j) || !owned(j)) continue;
-                const int col = 16 * (wv + NWC * j) + ei - mm;
+                const int col = 16 * cmap(j) + ei - mm;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int row = 16 * I + MF::row(eg, r) - mm;
@@ -559,7 +589,7 @@ __global__ void __launch_bounds__(64 * NWC * NP) __attribute__((amdgpu_waves_per
 #pragma unroll
                 for (int j = 0; j < TJ; ++j) {
                     if (I >= held(j) || !owned(j)) continue;
-                    const int col = 16 * (wv + NWC * j) + ei - m;
+                    const int col = 16 * cmap(j) + ei - m;
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const int row = 16 * I + MF::row(eg, r) - m;
@@ -744,7 +774,7 @@ template <typename R, int NT, int NWC>
 constexpr int mfma_acc() {
     constexpr int TJ = (NT + NWC - 1) / NWC;
     int t = 0;
-    for (int j = 0; j < TJ; ++j) t += NWC * (j + 1) < NT ? NWC * (j + 1) : NT;
+    for (int j = 0; j < TJ; ++j) t += pole_held<NT, NWC>(j);
     return t * (sizeof(R) == 4 ? 8 : 16);
 }
 
