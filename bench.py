@@ -374,7 +374,8 @@ def main():
         "graph": {"mixing_potential_H_step_replayed": not args.no_graph,
                   "replays_per_scf": {"fp64": fp64[-1]["graph_replays"], "mixed": mixed[-1]["graph_replays"]}},
         "roofline": roofline(st32["fp32"], flops, PEAK_FP32, kernel_name(bk[0], True),
-                             measured_traffic("fp32") if default_workload else None),
+                             measured_traffic("fp32_fused" if bk[0][6] == 0 and bk[0][7] == 2 else "fp32")
+                             if default_workload else None),
         "roofline_fp64": roofline(st64["fp64"], flops, PEAK_FP64, kernel_name(bk[0], False),
                                   measured_traffic("fp64") if default_workload else None),
         "cpu_baseline": None,
