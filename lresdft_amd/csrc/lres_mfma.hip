@@ -165,6 +165,31 @@ __host__ __device__ constexpr int pole_col_owner(int c) {
     const int j = (NT - 1 - c) / NWC, q = (NT - 1 - c) % NWC;
     return (j & 1) ? NWC - 1 - q : q;
 }
+// the column a wave holds as its j-th (the kernel's cmap), -1 for none
+template <int NT, int NWC>
+__host__ __device__ constexpr int pole_col_of(int wv, int j) {
+    if (NWC == 1) return j;
+    return NT - 1 - (j * NWC + ((j & 1) ? NWC - 1 - wv : wv));
+}
+// every tile column lives in exactly one (wave, j) with its c + 1 upper tiles held, and
+// every held (wave, j) names a column or none
+template <int NT, int NWC>
+constexpr bool pole_map_ok() {
+    constexpr int TJ = (NT + NWC - 1) / NWC;
+    for (int c = 0; c < NT; ++c) {
+        const int w = pole_col_owner<NT, NWC>(c), j = pole_col_index<NT, NWC>(c);
+        if (w < 0 || w >= NWC || j < 0 || j >= TJ) return false;
+        if (pole_col_of<NT, NWC>(w, j) != c || pole_held<NT, NWC>(j) < c + 1) return false;
+    }
+    int seen = 0;
+    for (int w = 0; w < NWC; ++w)
+        for (int j = 0; j < TJ; ++j) {
+            const int c = pole_col_of<NT, NWC>(w, j);
+            if (c >= NT || c < -NWC) return false;
+            if (c >= 0) ++seen;
+        }
+    return seen == NT;
+}
 
 template <typename R, int NT, int NWC, int WPE, int NP>
 __global__ void __launch_bounds__(64 * NWC * NP) __attribute__((amdgpu_waves_per_eu(WPE))) k_pole_mfma(const WinDesc* __restrict__ wins,
@@ -248,10 +273,8 @@ __global__ void __launch_bounds__(64 * NWC * NP) __attribute__((amdgpu_waves_per
     // read), so a wave's updates per block step are the sum of held(j): 8 tiles at NT 6 over
     // 4 waves (10 with columns dealt round-robin), 7 over 5 (11)
     V are[NT][TJ], aim[NT][TJ];
-    auto cmap = [&](int j) __attribute__((always_inline)) {
-        if constexpr (NWC == 1) return j;
-        return NT - 1 - (j * NWC + ((j & 1) ? NWC - 1 - wv : wv));   // < 0: no j-th column
-    };
+    static_assert(pole_map_ok<NT, NWC>(), "tile columns must map one to one onto (wave, j)");
+    auto cmap = [&](int j) __attribute__((always_inline)) { return pole_col_of<NT, NWC>(wv, j); };
     auto owned = [&](int j) __attribute__((always_inline)) { return NWC == 1 || cmap(j) >= 0; };
     auto held = [](int j) __attribute__((always_inline)) { return pole_held<NT, NWC>(j); };
 
