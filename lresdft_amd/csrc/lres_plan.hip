@@ -537,6 +537,15 @@ void prepare_passes(lres_plan* P, int fp32, double c, double R) {
     }
 }
 
+// A complex64 pass whose windows are all reduced inside the window-fused pole kernel
+// (by_pole 2 throughout the prepared table) launches no k_reduce.
+bool reduce_needed(const lres_plan* P, int fp32) {
+    if (!fp32 || P->by_pole.size() != (size_t)P->nwin) return true;
+    for (int8_t v : P->by_pole)
+        if (v != 2) return true;
+    return false;
+}
+
 // Queue one pass at the circle in P->d_circ: pole solves, reduction, the density into
 // d_dens if given (skipped on the device when d_circ[3] != 0) and the sums of N_B and
 // E_B into `sums` (device or mapped host memory).  Every kernel leaves at once when
@@ -594,9 +603,11 @@ void queue_pass(lres_plan* P, double* d_dens, double* d_NB, double* d_EB, double
     }
     if (spread) side_join(P, used);
     if (e1) hip_check(hipEventRecord(e1, P->st), "event");
-    hip_check(launch_reduce(P->d_wins.p, (int)P->nwin, P->ngroup, P->d_Dpart.p, P->d_Fpart.p, P->d_slots.p, P->M / 2,
-                            fp32 ? P->d_by_pole.p : nullptr, P->d_Q.p, P->d_D.p, P->d_F.p, NB, EB, circ, P->st),
-              "reduce kernel");
+    if (reduce_needed(P, fp32))
+        hip_check(launch_reduce(P->d_wins.p, (int)P->nwin, P->ngroup, P->d_Dpart.p, P->d_Fpart.p, P->d_slots.p,
+                                P->M / 2, fp32 ? P->d_by_pole.p : nullptr, P->d_Q.p, P->d_D.p, P->d_F.p, NB, EB, circ,
+                                P->st),
+                  "reduce kernel");
     if (d_dens)
         hip_check(launch_density(P->d_wins.p, (int)P->nwin, P->kmax, P->d_D.p, density_rows(P), d_dens, circ + 3,
                                  P->st),
