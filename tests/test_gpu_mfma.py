@@ -123,7 +123,7 @@ def test_every_wave_split_gives_the_same_bits(sys3, precision):
     W, _ = build_windows(b, cells, 8.0)
     e_lo, mu = _mu_window(H, b.S, W)
     ref = None
-    for nwc in (1, 2, 3, 4, 6):
+    for nwc in (1, 2, 3, 4, 5, 6):
         d, NB, EB, ns, es, bk = _pass(b, cells, H, mu, e_lo, precision=precision,
                                       env={"LRES_MFMA_WAVES": str(nwc), "LRES_POLE_KERNEL": "mfma"})
         col = 3 if precision == "fp64" else 6
