@@ -769,14 +769,25 @@ __global__ void __launch_bounds__(kMixThreads) k_harris(const double* __restrict
 // out[q] = sum_b part[q kMixBlocks + b] for q < nq (one block, fixed order)
 __global__ void __launch_bounds__(kMixThreads) k_rows_total(const double* __restrict__ part, int nq,
                                                             double* __restrict__ out) {
-    __shared__ double red[16];
+    // block_sum's additions for every row (each thread's strided sum, each wave's shuffle
+    // tree, the wave partials in order), the rows' loads and trees back to back with one
+    // barrier for all of them
+    constexpr int kWaves = kMixThreads / 64;
+    __shared__ double red[kMaxHist + 1][kWaves];   // nq <= kMaxHist + 1 (the entry points check it)
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int q = 0; q < nq; ++q) {
         double s = 0.0;
         for (int b = threadIdx.x; b < kMixBlocks; b += kMixThreads) s += part[(int64_t)q * kMixBlocks + b];
-        const double t = block_sum(s, red);
-        if (threadIdx.x == 0) out[q] = t;
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+        if (lane == 0) red[q][wave] = s;
     }
-    if (threadIdx.x == 0) __threadfence_system();   // out may be host memory read after the stream's wait
+    __syncthreads();
+    if (threadIdx.x < nq) {
+        double t = 0.0;
+        for (int w = 0; w < kWaves; ++w) t += red[threadIdx.x][w];
+        out[threadIdx.x] = t;
+        __threadfence_system();   // out may be host memory read after the stream's wait
+    }
 }
 
 // sum of nwin values in a fixed order into out[0] (one block)
