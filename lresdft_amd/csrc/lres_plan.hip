@@ -421,12 +421,16 @@ int pole_acc_entries(const lres_plan* P, const lres_plan::Bucket& b, const Shape
     return (size_t)u * 16 <= 64 * 1024 ? u : 0;
 }
 
-// Autotuner: time every valid shape of a bucket on the loaded H (two launches
+// Autotuner: time every valid shape of a bucket on the loaded H (kTuneLaunches launches
 // after one warm-up, HIP events) and keep the fastest.  All shapes apply the
 // same operations to every matrix entry, so the choice never changes results.
 // When the window-fused complex64 form is a candidate, the per-pole-slot candidates
 // are timed with k_reduce of the bucket's windows (the others marked as reduced),
 // the work the fused form does inside its kernel.
+// four launches: enough to order candidates a few percent apart (the complex64 slot form with
+// its k_reduce and the window-fused form come within about 2% of each other)
+constexpr int kTuneLaunches = 4;
+
 void tune_bucket(lres_plan* P, lres_plan::Bucket& b, int fp32, double c, double R) {
     const auto cands = pole_shapes(b.nmax, fp32);
     require(!cands.empty(), "no kernel shape for windows of " + std::to_string(b.nmax) + " functions");
@@ -463,8 +467,7 @@ void tune_bucket(lres_plan* P, lres_plan::Bucket& b, int fp32, double c, double 
         };
         launch();
         hip_check(hipEventRecord(e0, P->st), "event");
-        launch();
-        launch();
+        for (int i = 0; i < kTuneLaunches; ++i) launch();
         hip_check(hipEventRecord(e1, P->st), "event");
         hip_check(hipEventSynchronize(e1), "event sync");
         float ms = 0;
@@ -473,7 +476,7 @@ void tune_bucket(lres_plan* P, lres_plan::Bucket& b, int fp32, double c, double 
             fprintf(stderr, "[lres tune] nmax=%d windows=%zu %s %s T=%d G=%d: %.4f ms/launch%s\n", b.nmax,
                     b.ids.size(), fp32 ? "fp32" : "fp64",
                     sh.wide == kMfma ? (sh.G == 0 ? "mfma-fused" : "mfma") : (sh.wide ? "valu512" : "valu128"), sh.T,
-                    sh.G, ms / 2, with_reduce && !fused(sh) ? " (with k_reduce)" : "");
+                    sh.G, ms / kTuneLaunches, with_reduce && !fused(sh) ? " (with k_reduce)" : "");
         if (ms < best) {
             best = ms;
             b.shape[fp32] = sh;
