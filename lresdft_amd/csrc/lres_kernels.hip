@@ -719,11 +719,34 @@ __global__ void __launch_bounds__(kMixThreads) k_lda_potential(const double* __r
 
 // r = scale nout - nin; Rh[slot] = r, Nh[slot] = nin; partials of sum |r| (row 0)
 // and of R_q . r for every history row q < nh (rows 1..nh; row slot is r . r)
+// block_sum of each of the nq per-thread values v[q] into part[q kMixBlocks + block]: the
+// same additions (each wave's shuffle tree, then the wave partials in order from zero), the
+// rows' trees back to back and one barrier for all of them
+template <int N>
+__device__ __forceinline__ void rows_to_parts(const double (&v)[N], int nq, double (&red)[kMaxHist + 1][kMixThreads / 64],
+                                              double* __restrict__ part) {
+    static_assert(N <= kMaxHist + 1, "rows");
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        if (q >= nq) break;
+        double x = v[q];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+        if (lane == 0) red[q][wave] = x;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nq) {
+        double t = 0.0;
+        for (int w = 0; w < kMixThreads / 64; ++w) t += red[threadIdx.x][w];
+        part[(int64_t)threadIdx.x * kMixBlocks + blockIdx.x] = t;
+    }
+}
+
 __global__ void __launch_bounds__(kMixThreads) k_scf_residual(const double* __restrict__ nout, double scale,
                                                               const double* __restrict__ nin, int64_t ng, int nh,
                                                               int slot, double* __restrict__ Nh,
                                                               double* __restrict__ Rh, double* __restrict__ part) {
-    __shared__ double red[16];
+    __shared__ double red2[kMaxHist + 1][kMixThreads / 64];
     double s[kMaxHist + 1];
 #pragma unroll
     for (int q = 0; q <= kMaxHist; ++q) s[q] = 0.0;
@@ -738,18 +761,13 @@ __global__ void __launch_bounds__(kMixThreads) k_scf_residual(const double* __re
         for (int q = 0; q < kMaxHist; ++q)
             if (q < nh) s[1 + q] = fma(q == slot ? r : Rh[(int64_t)q * ng + g], r, s[1 + q]);
     }
-#pragma unroll
-    for (int q = 0; q <= kMaxHist; ++q) {
-        if (q > nh) break;
-        const double t = block_sum(s[q], red);
-        if (threadIdx.x == 0) part[(int64_t)q * kMixBlocks + blockIdx.x] = t;
-    }
+    rows_to_parts(s, nh + 1, red2, part);
 }
 
 // partials of sum n v_H (row 0), sum n e_xc(n) (row 1), sum n (v_H + v_xc(n)) (row 2)
 __global__ void __launch_bounds__(kMixThreads) k_harris(const double* __restrict__ n, const double* __restrict__ vh,
                                                         int64_t ng, double* __restrict__ part) {
-    __shared__ double red[16];
+    __shared__ double red2[kMaxHist + 1][kMixThreads / 64];
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     for (int64_t g = (int64_t)blockIdx.x * kMixThreads + threadIdx.x; g < ng; g += (int64_t)kMixBlocks * kMixThreads) {
         const double x = n[g], h = vh[g];
@@ -758,12 +776,8 @@ __global__ void __launch_bounds__(kMixThreads) k_harris(const double* __restrict
         s1 = fma(x, l.exc, s1);
         s2 = fma(h + l.vxc, x, s2);
     }
-    double t = block_sum(s0, red);
-    if (threadIdx.x == 0) part[blockIdx.x] = t;
-    t = block_sum(s1, red);
-    if (threadIdx.x == 0) part[kMixBlocks + blockIdx.x] = t;
-    t = block_sum(s2, red);
-    if (threadIdx.x == 0) part[2 * kMixBlocks + blockIdx.x] = t;
+    const double sv[3] = {s0, s1, s2};
+    rows_to_parts(sv, 3, red2, part);
 }
 
 // out[q] = sum_b part[q kMixBlocks + b] for q < nq (one block, fixed order)
